@@ -1,0 +1,153 @@
+"""Benchmark: plain Louvain on R-MAT scale 22 on one MI355X.
+
+Workload (the single-GPU configuration): R-MAT scale 22, edge factor 16,
+(a,b,c,d) = (0.57,0.19,0.19,0.05), seed 2, symmetrised, self-loops and
+duplicates dropped, ids of non-isolated vertices compacted; unit weights.
+One step = one full Louvain run to convergence (all levels, the reference's
+0.001 thresholds, HASH mode) starting from the CSR already resident in HBM.
+
+value = traversed directed entries per second over the whole run (sum over
+sweeps of the entries of the vertices examined, divided by the step time).
+The roofline block covers the best-move pass (every degree bucket's kernel of
+each sweep), timed with HIP events on the library stream; the CPU baseline is
+the single-threaded oracle on the first sweeps of level 0 of the same graph.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "louvain-fast-move-cuda_amd"))
+sys.path.insert(0, REPO)
+
+METRIC = "Louvain s-to-converge + edges/s vs HBM roofline; batched graphs/s, p99 lat"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(row, col, sweeps):
+    """Single-threaded oracle on the first `sweeps` sweeps of level 0."""
+    from oracle import oracle as O
+    og = O.Graph.from_csr(row, col)
+    t0 = time.perf_counter()
+    r = og.louvain(O.params(max_total_sweeps=sweeps))
+    dt = time.perf_counter() - t0
+    ent = sum(s["e_active"] for s in r["sweeps"])
+    return {"value": ent / dt, "unit": "edges/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (C, -O3, 1 thread) on level-0 sweeps 0..{sweeps - 1} of the same "
+                      f"R-MAT-22 graph: {ent} entries in {dt:.2f} s; CPU {cpu_model()}, "
+                      f"nproc {os.cpu_count()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--mode", default="hash")
+    ap.add_argument("--threshold", type=float, default=0.001)
+    ap.add_argument("--cpu-sweeps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is described in DESIGN.md and not implemented", flush=True)
+        return 2
+
+    import louvain_amd as L
+    if L.device_count() < 1:
+        raise SystemExit("bench.py: no GPU visible")
+    t = time.perf_counter()
+    ea, eb, n = L.gen_rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=2)
+    log(f"rmat{args.scale}: {len(ea)} undirected edges, n={n} ({time.perf_counter() - t:.1f}s)")
+    t = time.perf_counter()
+    g = L.Graph.from_edges(ea, eb)
+    del ea, eb
+    info = g.info()
+    log(f"csr: n={info['n']} E={info['E']} ({time.perf_counter() - t:.1f}s)")
+    t = time.perf_counter()
+    c = L.Community(g)
+    log(f"upload + level setup ({time.perf_counter() - t:.2f}s)")
+    prm = L.params(mode=args.mode, sweep_threshold=args.threshold,
+                   level_threshold=args.threshold, profile=1)
+    for i in range(args.warmup):
+        c.reset()
+        r = c.run(prm, best=False, n0=info["n"])
+        log(f"warmup {i}: {r['seconds'] * 1e3:.1f} ms, Q={r['modularity']:.6f}")
+    times, results, kstats = [], [], []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        c.reset()
+        r = c.run(prm, best=False, n0=info["n"])
+        dt = time.perf_counter() - t0
+        times.append(dt)
+        results.append(r)
+        kstats.append(c.kernel_stats())
+        log(f"step {i}: {dt * 1e3:.1f} ms (in-library {r['seconds'] * 1e3:.1f} ms), "
+            f"Q={r['modularity']:.6f}, comms={r['n_comm']}, levels={r['levels']}, "
+            f"sweeps={r['total_sweeps']}")
+    total = sum(times)
+    traversed = sum(r["traversed_entries"] for r in results)
+    last = results[-1]
+    ks = {k: sum(s[k] for s in kstats) for k in kstats[0]}
+    move_s = ks["move_ms"] / 1e3
+    achieved = ks["move_bytes"] / move_s / 1e9 if move_s > 0 else 0.0
+    sw0 = last["sweeps"][0]
+    out = {
+        "metric": METRIC,
+        "value": traversed / total,
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic",
+        "config": {"workload": f"plain Louvain, R-MAT scale {args.scale}, edge factor 16, "
+                               f"(0.57,0.19,0.19,0.05), seed 2, symmetrised, unit weights",
+                   "mode": args.mode, "threshold": args.threshold, "n": info["n"],
+                   "E": info["E"]},
+        "s_to_converge": total / args.steps,
+        "modularity": last["modularity"],
+        "n_communities": last["n_comm"],
+        "levels": last["levels"],
+        "sweeps": last["total_sweeps"],
+        "sweep0_entries": sw0["e_active"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "best-move pass (all degree-bucket kernels of a sweep)",
+                     "launches": ks["move_launches"],
+                     "avg_launch_ms": ks["move_ms"] / max(1, ks["move_launches"]),
+                     "avg_bytes": ks["move_bytes"] / max(1, ks["move_launches"])},
+    }
+    if not args.no_cpu:
+        row, col, _ = g.csr()
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(row, col, args.cpu_sweeps)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,191 @@
+/*
+ * lv_api.h -- C ABI of the MI355X (gfx950) Louvain library (liblouvain_amd.so).
+ *
+ * The reference (FedeFonto/louvain-fast-move-cuda) exposes only header-only
+ * C++ (community.h, modularityAlgorithms.h, phase*.cuh); it has no FFI.
+ * Each entry point below names the reference interface it replaces.  The C++
+ * mirror of those names lives in community.h / modularityAlgorithms.h /
+ * phaseOptimization.h / phaseAggregation.h / phaseRefine.h next to this file.
+ *
+ * Conventions: every call returns an lv_status (nothing throws across the
+ * ABI); output buffers are caller-owned; handles are opaque and used by one
+ * host thread at a time; streams are passed as void* (a hipStream_t, NULL for
+ * a library-owned stream); every HIP call is checked.
+ */
+#ifndef LV_API_H
+#define LV_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum lv_status {
+    LV_OK = 0,
+    LV_ERR_ARG = 1,        /* bad argument */
+    LV_ERR_OOM = 2,        /* device or host allocation failed (the reference's std::bad_alloc, main.cu:26-45) */
+    LV_ERR_PARSE = 3,      /* no edge parsed (the reference calls exit(0), graph.cu:52-55) */
+    LV_ERR_IO = 4,         /* file cannot be opened */
+    LV_ERR_NOT_DENSE = 5,  /* vertex ids do not cover [min_id, min_id+n) (graph.cuh:66-76 assumes they do) */
+    LV_ERR_RANGE = 6,      /* total weight 2m >= 2^31 or more than 2^32-1 vertices */
+    LV_ERR_HIP = 7,        /* a HIP runtime call failed */
+    LV_ERR_NO_DEVICE = 8,  /* no GPU visible */
+    LV_ERR_STATE = 9       /* call out of order */
+} lv_status;
+
+/* enum MODE { HASH, SORT, ADAPTIVE_SPEED, ADAPTIVE_MEMORY } (constants.h:13) */
+typedef enum lv_mode { LV_HASH = 0, LV_SORT = 1, LV_ADAPTIVE_SPEED = 2, LV_ADAPTIVE_MEMORY = 3 } lv_mode;
+/* sweep kinds scheduled by OptimizationPhase::optimize (phaseOptimization.cuh:116-155) */
+typedef enum lv_sweep_kind { LV_SWEEP_FAST = 0, LV_SWEEP_SORT = 1, LV_SWEEP_HASH = 2 } lv_sweep_kind;
+/* LV_GAIN_EXACT: integer gain 2m*(w_vc-self) + k*(tot_own-k-tot_c) in int64;
+ * LV_GAIN_REF_F32: the reference's f32-rounded delta-Q (operatorsDelta.h:19-20) */
+typedef enum lv_gain { LV_GAIN_EXACT = 0, LV_GAIN_REF_F32 = 1 } lv_gain;
+/* LV_TIE_LOWEST: lowest community id wins ties in every sweep;
+ * LV_TIE_REFERENCE: lowest in fast/sort sweeps, highest in hash sweeps
+ * (phaseOptimizationHash.cu:29-31) */
+typedef enum lv_tie { LV_TIE_LOWEST = 0, LV_TIE_REFERENCE = 1 } lv_tie;
+
+typedef struct lv_params {
+    int32_t mode;             /* lv_mode */
+    int32_t gain;             /* lv_gain */
+    int32_t tie;              /* lv_tie */
+    int32_t chunk_emulation;  /* 1: reproduce the STEP_ROUND chunking and the fast/sort
+                                 sweeps' empty-chunk early return (phaseOptimizationSort.cu:81-86, :235-240) */
+    uint64_t chunk_entries;   /* STEP_ROUND (constants.h:18) */
+    double sweep_threshold;   /* MODULARITY_CONVERGED_THRESHOLD for the sweep loop (phaseOptimization.cuh:179) */
+    double level_threshold;   /* ... and for the level loop (modularityAlgorithms.h:52) */
+    int64_t max_sweeps;       /* EARLY_STOP_LIMIT (constants.h:16) */
+    int32_t max_levels;       /* <= 0: no limit */
+    int32_t profile;          /* 1: time every best-move launch with HIP events */
+} lv_params;
+
+typedef struct lv_sweep_rec {
+    int32_t level, sweep, kind, _pad;
+    uint32_t n, n_active, n_moved, n_considered;
+    uint64_t E, e_active;      /* e_active: entries of active vertices with src != dst */
+    int64_t sigma_in;          /* weight of entries inside communities */
+    uint64_t sum_tot_sq;       /* sum over communities of tot_c^2 */
+    double q;
+    uint64_t comm_hash;        /* FNV-1a 64 over comm[] (0 unless requested) */
+} lv_sweep_rec;
+
+typedef struct lv_level_rec {
+    int32_t level, sweeps;
+    uint32_t n, n_comm;
+    uint64_t E;
+    double q_start, q_end;
+    uint64_t best_hash;        /* 0 unless requested */
+} lv_level_rec;
+
+typedef struct lv_result {
+    double modularity;
+    uint32_t n_of_best_communities;
+    int32_t levels, total_sweeps, hash_trace;  /* hash_trace (in): 1 = fill comm_hash/best_hash */
+    uint32_t *best_communities;  /* caller buffer of n (level-0 vertices), may be NULL */
+    lv_sweep_rec *sweeps; int32_t sweeps_cap;
+    lv_level_rec *level_recs; int32_t levels_cap;
+    double seconds;              /* device-resident CSR to final partition, all levels */
+    uint64_t traversed_entries;  /* sum of e_active over all sweeps */
+} lv_result;
+
+typedef struct lv_kernel_stats {
+    /* best-move kernels (all degree buckets), summed over the launches timed
+     * when lv_params.profile = 1 */
+    uint64_t move_launches;
+    double move_ms;             /* sum of HIP-event durations on the library stream */
+    uint64_t move_bytes;        /* algorithmic bytes of those launches */
+    uint64_t move_entries;      /* entries traversed by those launches */
+    double sweep_ms;            /* whole sweeps (move + commit + modularity), event timed */
+    uint64_t sweeps;
+} lv_kernel_stats;
+
+typedef struct lv_graph lv_graph;          /* GraphHost (graphHost.cuh:7-18): host CSR */
+typedef struct lv_community lv_community;  /* Community (community.h:15-107): device state */
+
+void lv_default_params(lv_params *p);
+const char *lv_status_string(lv_status s);
+lv_status lv_device_count(int *count);
+
+/* GraphHost::GraphHost(std::string name, bool weighted, int skip_line) (graph.cu:17-87).
+ * weighted = 1 reads a third integer column (an extension: the reference
+ * leaves weighted parsing as a TODO, graph.cu:32-34). */
+lv_status lv_graph_load_edge(const char *path, int weighted, int skip_line, lv_graph **out);
+/* Same semantics from in-memory pairs (w may be NULL for unit weights). */
+lv_status lv_graph_from_edges(uint64_t n_edges, const int64_t *a, const int64_t *b,
+                              const uint32_t *w, lv_graph **out);
+/* Direct CSR (row[n+1], col[E], w[E] or NULL); total_weight <= 0 means sum(w)/2. */
+lv_status lv_graph_from_csr(uint32_t n, uint64_t E, const uint64_t *row, const uint32_t *col,
+                            const uint32_t *w, double total_weight, lv_graph **out);
+void lv_graph_free(lv_graph *g);
+lv_status lv_graph_info(const lv_graph *g, uint32_t *n, uint64_t *E, double *total_weight,
+                        int64_t *min_id);
+lv_status lv_graph_copy_csr(const lv_graph *g, uint64_t *row, uint32_t *col, uint32_t *w);
+
+/* Community(GraphHost&) (community.h:31-46): uploads the CSR (GraphDevice,
+ * graph.cuh:46-99), identity partition, totals = degrees, modularity. */
+lv_status lv_community_create(const lv_graph *g, void *stream, lv_community **out);
+void lv_community_destroy(lv_community *c);
+/* OptimizationPhase::run(Community&, MODE) (phaseOptimization.cuh:184-206) */
+lv_status lv_optimization_phase(lv_community *c, const lv_params *p);
+/* AggregationPhase::run(Community&, bool aggregate, MODE) (phaseAggregation.cuh:19-175) */
+lv_status lv_aggregation_phase(lv_community *c, int aggregate, const lv_params *p);
+/* RefinePhase::run() (phaseRefine.cuh:1-3): a no-op, as in the reference */
+lv_status lv_refine_phase(lv_community *c);
+/* Community::modularity (community.h:26) */
+double lv_modularity(const lv_community *c);
+/* Community::best_communities / n_of_best_communities (community.h:20,25) */
+lv_status lv_best_communities(const lv_community *c, uint32_t *out, size_t n, uint32_t *n_comm);
+/* ModularityAlgorithms::Louvain(GraphHost&, MODE) (modularityAlgorithms.h:8-81) */
+lv_status lv_louvain(const lv_graph *g, const lv_params *p, lv_result *r);
+
+/* Device-resident re-runs (no reference counterpart; the reference builds a
+ * new Community per call, modularityAlgorithms.h:18): reset returns to the
+ * resident level-0 CSR with the identity partition; run executes the level
+ * loop of ModularityAlgorithms::Louvain on the community as it stands. */
+lv_status lv_community_reset(lv_community *c);
+lv_status lv_community_run(lv_community *c, const lv_params *p, lv_result *r);
+
+/* Snapshot access for parity checks (no reference counterpart: the reference
+ * reads these device_vectors directly). */
+lv_status lv_community_level_info(const lv_community *c, uint32_t *n, uint64_t *E, int32_t *level);
+lv_status lv_community_get_partition(const lv_community *c, uint32_t *comm, size_t n);
+lv_status lv_community_set_partition(lv_community *c, const uint32_t *comm, size_t n);
+lv_status lv_community_copy_graph(const lv_community *c, uint64_t *row, uint32_t *col, uint32_t *w,
+                                  uint32_t *k);
+/* One best-move decision from the current snapshot (no move applied):
+ * target[v] = community v would move to, or comm[v].  active may be NULL (all).
+ * kind LV_SWEEP_FAST requires the identity partition. */
+lv_status lv_best_moves(lv_community *c, int kind, const lv_params *p, const uint8_t *active,
+                        uint32_t *target, uint64_t *e_active);
+/* Exact modularity parts of the current partition, from scratch on the device. */
+lv_status lv_community_modularity_parts(lv_community *c, int64_t *sigma_in, uint64_t *sum_tot_sq,
+                                        double *q);
+lv_status lv_kernel_stats_get(const lv_community *c, lv_kernel_stats *s);
+lv_status lv_last_run_kernel_stats(lv_kernel_stats *s);
+
+/* Seeded synthetic graph generators (host, multithreaded; counter-based PRNG,
+ * so the output is independent of the thread count).  Edges are written as
+ * (a[i], b[i], w[i]) with dense ids; *m receives the count.  Pass NULL arrays
+ * to query the capacity needed in *m. */
+/* R-MAT: 2^scale vertices, edge_factor*2^scale draws, quadrant probabilities
+ * a,b,c (d = 1-a-b-c); symmetrised, self-loops and duplicates dropped, ids of
+ * non-isolated vertices compacted in order. */
+lv_status lv_gen_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed,
+                      int64_t *ea, int64_t *eb, uint64_t cap, uint64_t *m, uint32_t *n_out);
+/* Barabasi-Albert preferential attachment: n vertices, m_attach edges per new vertex. */
+lv_status lv_gen_ba(uint32_t n, uint32_t m_attach, uint64_t seed, int64_t *ea, int64_t *eb,
+                    uint64_t cap, uint64_t *m);
+/* Planted partition with power-law degrees and community sizes, mixing mu,
+ * integer weights uniform in [w_lo, w_hi]; membership written to planted[n]. */
+lv_status lv_gen_planted(uint32_t n, double deg_exp, double mean_deg, uint32_t max_deg,
+                         uint32_t n_comm, double size_exp, uint32_t size_lo, uint32_t size_hi,
+                         double mu, uint32_t w_lo, uint32_t w_hi, uint64_t seed,
+                         int64_t *ea, int64_t *eb, uint32_t *ew, uint64_t cap, uint64_t *m,
+                         uint32_t *planted);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

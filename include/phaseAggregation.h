@@ -1,0 +1,18 @@
+/*
+ * phaseAggregation.h -- renumbering and contraction (reference:
+ * struct AggregationPhase, phaseAggregation.cuh:18-190).
+ * Non-empty communities (tot > 0) are renumbered in id order by an inclusive
+ * scan, best_communities is composed with that map, and when `aggregate` is
+ * set the graph is contracted into a CSR of the communities (parallel edges
+ * summed, internal weight as one self-loop entry) and the partition reset.
+ */
+#ifndef LV_PHASE_AGGREGATION_H
+#define LV_PHASE_AGGREGATION_H
+
+#include "community.h"
+
+struct AggregationPhase {
+    static void run(Community &c, bool aggregate, MODE mode);
+};
+
+#endif

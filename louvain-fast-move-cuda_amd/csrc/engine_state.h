@@ -1,0 +1,75 @@
+// engine_state.h -- per-level working state of the sweep loop (degree
+// buckets, active lists, hub tables, statistics).  Internal to the library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/device_array.h"
+#include "../../include/graph.h"
+#include "lv_kernels.h"
+
+namespace lv {
+
+struct KernelStatsAcc {
+    uint64_t move_launches = 0;
+    double move_ms = 0;
+    uint64_t move_bytes = 0;
+    uint64_t move_entries = 0;
+    double sweep_ms = 0;
+    uint64_t sweeps = 0;
+};
+
+extern KernelStatsAcc g_last_run_stats;
+
+// host_stats layout: kNumStats*kStatSlots u64 counters, then kNumBuckets u32
+// next-list sizes
+constexpr size_t kStatWordsOffset = (size_t)kNumStats * kStatSlots;
+constexpr size_t kHostWords = kStatWordsOffset + kNumBuckets;
+
+struct SweepState {
+    uint32_t n = 0;
+    DeviceArray<uint8_t> bucket;
+    DeviceArray<uint8_t> mark;
+    DeviceArray<uint32_t> target;
+    DeviceArray<uint32_t> list_buf[2];
+    DeviceArray<uint32_t> counts_dev;
+    DeviceArray<unsigned long long> stats;
+    unsigned long long *host_stats = nullptr;   // pinned
+    uint64_t bucket_off[kNumBuckets] = {};
+    uint32_t bucket_size[kNumBuckets] = {};
+    uint32_t count[kNumBuckets] = {};           // sizes of the current lists
+    uint32_t *list_ptr[2][kNumBuckets] = {};
+    int cur = 0;
+    // hubs
+    uint32_t n_hubs = 0;
+    DeviceArray<uint32_t> hub_slot_of;
+    DeviceArray<uint64_t> hub_tab_off;
+    DeviceArray<uint32_t> hub_tab_log2;
+    DeviceArray<uint32_t> hub_key, hub_val;
+    DeviceArray<int64_t> hub_part_s;
+    DeviceArray<uint32_t> hub_part_c;
+    DeviceArray<uint32_t> hub_self;
+    // profiling
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    KernelStatsAcc acc;
+
+    SweepState() {
+        for (auto &e : ev) LV_HIP(hipEventCreate(&e));
+    }
+    ~SweepState();
+    SweepState(const SweepState &) = delete;
+    SweepState &operator=(const SweepState &) = delete;
+
+    void setup_level(const GraphDevice &g, hipStream_t s);
+    void build_static_lists(hipStream_t s);
+    uint32_t *const *lists(int which);
+    HubTables hub_tables();
+};
+
+LevelView level_view(const GraphDevice &g);
+uint64_t sum_slots(const unsigned long long *h, int stat);
+void launch_sweep(struct ::Community &c, const lv_params &p, int kind, bool apply, uint32_t v_end);
+
+}  // namespace lv

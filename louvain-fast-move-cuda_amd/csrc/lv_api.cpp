@@ -1,0 +1,328 @@
+// lv_api.cpp -- the C ABI (include/lv_api.h) over the C++ classes.
+// Nothing throws across the boundary: every entry point maps lv::Error and
+// std::bad_alloc to an lv_status.
+#include "../../include/lv_api.h"
+
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/community.h"
+#include "../../include/modularityAlgorithms.h"
+#include "engine_state.h"
+#include "lv_kernels.h"
+
+struct lv_graph {
+    GraphHost g;
+};
+struct lv_community {
+    Community c;
+    explicit lv_community(const GraphHost &g, hipStream_t s) : c(g, s) {}
+};
+
+namespace {
+
+template <typename F>
+lv_status guarded(F &&f) {
+    try {
+        f();
+        return LV_OK;
+    } catch (const lv::Error &e) {
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        return LV_ERR_OOM;
+    } catch (...) {
+        return LV_ERR_HIP;
+    }
+}
+
+lv_status need_device() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return LV_ERR_NO_DEVICE;
+    }
+    return LV_OK;
+}
+
+}  // namespace
+
+namespace lv {
+void community_totals(const uint32_t *comm, const uint32_t *kdeg, uint32_t n, uint32_t *tot,
+                      hipStream_t s);
+}
+
+extern "C" {
+
+void lv_default_params(lv_params *p) {
+    p->mode = LV_HASH;
+    p->gain = LV_GAIN_EXACT;
+    p->tie = LV_TIE_LOWEST;
+    p->chunk_emulation = 0;
+    p->chunk_entries = (uint64_t)STEP_ROUND;
+    p->sweep_threshold = MODULARITY_CONVERGED_THRESHOLD;
+    p->level_threshold = MODULARITY_CONVERGED_THRESHOLD;
+    p->max_sweeps = EARLY_STOP_LIMIT;
+    p->max_levels = 0;
+    p->profile = 0;
+}
+
+const char *lv_status_string(lv_status s) {
+    switch (s) {
+        case LV_OK: return "ok";
+        case LV_ERR_ARG: return "bad argument";
+        case LV_ERR_OOM: return "out of memory";
+        case LV_ERR_PARSE: return "no edge parsed";
+        case LV_ERR_IO: return "cannot open file";
+        case LV_ERR_NOT_DENSE: return "vertex ids are not dense";
+        case LV_ERR_RANGE: return "value out of supported range";
+        case LV_ERR_HIP: return "HIP runtime error";
+        case LV_ERR_NO_DEVICE: return "no GPU device";
+        case LV_ERR_STATE: return "call out of order";
+    }
+    return "unknown";
+}
+
+lv_status lv_device_count(int *count) {
+    if (!count) return LV_ERR_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        n = 0;
+    }
+    *count = n;
+    return LV_OK;
+}
+
+lv_status lv_graph_load_edge(const char *path, int weighted, int skip_line, lv_graph **out) {
+    if (!path || !out) return LV_ERR_ARG;
+    *out = nullptr;
+    return guarded([&] { *out = new lv_graph{GraphHost(std::string(path), weighted != 0, skip_line)}; });
+}
+
+lv_status lv_graph_from_edges(uint64_t n_edges, const int64_t *a, const int64_t *b,
+                              const uint32_t *w, lv_graph **out) {
+    if (!out || (n_edges && (!a || !b))) return LV_ERR_ARG;
+    *out = nullptr;
+    return guarded([&] { *out = new lv_graph{GraphHost::from_edges(n_edges, a, b, w)}; });
+}
+
+lv_status lv_graph_from_csr(uint32_t n, uint64_t E, const uint64_t *row, const uint32_t *col,
+                            const uint32_t *w, double total_weight, lv_graph **out) {
+    if (!out || !row || (E && !col)) return LV_ERR_ARG;
+    *out = nullptr;
+    return guarded([&] { *out = new lv_graph{GraphHost::from_csr(n, E, row, col, w, total_weight)}; });
+}
+
+void lv_graph_free(lv_graph *g) { delete g; }
+
+lv_status lv_graph_info(const lv_graph *g, uint32_t *n, uint64_t *E, double *total_weight,
+                        int64_t *min_id) {
+    if (!g) return LV_ERR_ARG;
+    if (n) *n = g->g.n_nodes;
+    if (E) *E = g->g.n_entries();
+    if (total_weight) *total_weight = g->g.total_weight;
+    if (min_id) *min_id = g->g.min_id;
+    return LV_OK;
+}
+
+lv_status lv_graph_copy_csr(const lv_graph *g, uint64_t *row, uint32_t *col, uint32_t *w) {
+    if (!g) return LV_ERR_ARG;
+    const GraphHost &h = g->g;
+    if (row) std::memcpy(row, h.row.data(), h.row.size() * sizeof(uint64_t));
+    if (col && !h.col.empty()) std::memcpy(col, h.col.data(), h.col.size() * sizeof(uint32_t));
+    if (w) {
+        if (h.unit_weights) std::fill(w, w + h.col.size(), 1u);
+        else std::memcpy(w, h.weights.data(), h.weights.size() * sizeof(uint32_t));
+    }
+    return LV_OK;
+}
+
+lv_status lv_community_create(const lv_graph *g, void *stream, lv_community **out) {
+    if (!g || !out) return LV_ERR_ARG;
+    *out = nullptr;
+    const lv_status d = need_device();
+    if (d != LV_OK) return d;
+    return guarded([&] { *out = new lv_community(g->g, (hipStream_t)stream); });
+}
+
+void lv_community_destroy(lv_community *c) { delete c; }
+
+lv_status lv_optimization_phase(lv_community *c, const lv_params *p) {
+    if (!c) return LV_ERR_ARG;
+    lv_params d;
+    lv_default_params(&d);
+    return guarded([&] { OptimizationPhase::run(c->c, p ? *p : d); });
+}
+
+lv_status lv_aggregation_phase(lv_community *c, int aggregate, const lv_params *p) {
+    if (!c) return LV_ERR_ARG;
+    const MODE m = (MODE)(p ? p->mode : LV_HASH);
+    return guarded([&] { AggregationPhase::run(c->c, aggregate != 0, m); });
+}
+
+lv_status lv_refine_phase(lv_community *c) {
+    if (!c) return LV_ERR_ARG;
+    RefinePhase::run();
+    return LV_OK;
+}
+
+double lv_modularity(const lv_community *c) { return c ? c->c.modularity : 0.0; }
+
+lv_status lv_best_communities(const lv_community *c, uint32_t *out, size_t n, uint32_t *n_comm) {
+    if (!c) return LV_ERR_ARG;
+    if (n_comm) *n_comm = c->c.n_of_best_communities;
+    if (!out) return LV_OK;
+    if (n < c->c.n_original) return LV_ERR_ARG;
+    return guarded([&] {
+        LV_HIP(hipMemcpy(out, c->c.best_communities.data(), c->c.n_original * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost));
+    });
+}
+
+lv_status lv_louvain(const lv_graph *g, const lv_params *p, lv_result *r) {
+    if (!g) return LV_ERR_ARG;
+    const lv_status d = need_device();
+    if (d != LV_OK) return d;
+    lv_params dp;
+    lv_default_params(&dp);
+    return guarded([&] { (void)ModularityAlgorithms::Louvain(g->g, p ? *p : dp, r, nullptr); });
+}
+
+lv_status lv_community_reset(lv_community *c) {
+    if (!c) return LV_ERR_ARG;
+    return guarded([&] { c->c.reset(); });
+}
+
+lv_status lv_community_run(lv_community *c, const lv_params *p, lv_result *r) {
+    if (!c) return LV_ERR_ARG;
+    lv_params d;
+    lv_default_params(&d);
+    return guarded([&] { ModularityAlgorithms::run(c->c, p ? *p : d, r); });
+}
+
+lv_status lv_community_level_info(const lv_community *c, uint32_t *n, uint64_t *E, int32_t *level) {
+    if (!c) return LV_ERR_ARG;
+    if (n) *n = c->c.graph.n_nodes;
+    if (E) *E = c->c.graph.n_entries;
+    if (level) *level = c->c.level;
+    return LV_OK;
+}
+
+lv_status lv_community_get_partition(const lv_community *c, uint32_t *comm, size_t n) {
+    if (!c || !comm || n < c->c.graph.n_nodes) return LV_ERR_ARG;
+    return guarded([&] {
+        LV_HIP(hipMemcpy(comm, c->c.communities.data(), c->c.graph.n_nodes * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost));
+    });
+}
+
+lv_status lv_community_set_partition(lv_community *c, const uint32_t *comm, size_t n) {
+    if (!c || !comm || n != c->c.graph.n_nodes) return LV_ERR_ARG;
+    for (size_t i = 0; i < n; ++i)
+        if (comm[i] >= n) return LV_ERR_ARG;
+    return guarded([&] {
+        Community &C = c->c;
+        hipStream_t s = C.stream();
+        LV_HIP(hipMemcpyAsync(C.communities.data(), comm, n * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, s));
+        LV_HIP(hipMemcpyAsync(C.state->target.data(), C.communities.data(), n * sizeof(uint32_t),
+                              hipMemcpyDeviceToDevice, s));
+        lv::community_totals(C.communities.data(), C.graph.tot_weight_per_nodes.data(),
+                             (uint32_t)n, C.communities_weight.data(), s);
+        C.compute_modularity();
+    });
+}
+
+lv_status lv_community_copy_graph(const lv_community *c, uint64_t *row, uint32_t *col, uint32_t *w,
+                                  uint32_t *k) {
+    if (!c) return LV_ERR_ARG;
+    return guarded([&] {
+        const GraphDevice &g = c->c.graph;
+        if (row) LV_HIP(hipMemcpy(row, g.neighboorhood_sum.data(), ((size_t)g.n_nodes + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        if (col && g.n_entries) LV_HIP(hipMemcpy(col, g.edge_destination.data(), g.n_entries * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (w) {
+            if (g.unit_weights) std::fill(w, w + g.n_entries, 1u);
+            else if (g.n_entries) LV_HIP(hipMemcpy(w, g.weights.data(), g.n_entries * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        }
+        if (k && g.n_nodes) LV_HIP(hipMemcpy(k, g.tot_weight_per_nodes.data(), g.n_nodes * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    });
+}
+
+lv_status lv_best_moves(lv_community *c, int kind, const lv_params *p, const uint8_t *active,
+                        uint32_t *target, uint64_t *e_active) {
+    if (!c || !target || kind < 0 || kind > 2) return LV_ERR_ARG;
+    lv_params d;
+    lv_default_params(&d);
+    const lv_params &P = p ? *p : d;
+    return guarded([&] {
+        Community &C = c->c;
+        lv::SweepState &st = *C.state;
+        hipStream_t s = C.stream();
+        const uint32_t n = C.graph.n_nodes;
+        if (!active) {
+            st.build_static_lists(s);
+        } else {
+            // marks -> lists through the finalize pass (target == comm, so no move is applied)
+            LV_HIP(hipMemcpyAsync(st.mark.data(), active, n, hipMemcpyHostToDevice, s));
+            lv::SweepArgs a{};
+            a.g = lv::level_view(C.graph);
+            a.comm = C.communities.data();
+            a.target = st.target.data();
+            a.tot = C.communities_weight.data();
+            a.mark = st.mark.data();
+            a.bucket = st.bucket.data();
+            a.stats = st.stats.data();
+            lv::fill_u64(st.stats.data(), 0ull, st.stats.size(), s);
+            lv::finalize_pass(a, st.lists(st.cur ^ 1), st.counts_dev.data() + lv::kNumBuckets, s);
+            uint32_t cnt[lv::kNumBuckets];
+            LV_HIP(hipMemcpyAsync(cnt, st.counts_dev.data() + lv::kNumBuckets, sizeof(cnt),
+                                  hipMemcpyDeviceToHost, s));
+            LV_HIP(hipStreamSynchronize(s));
+            st.cur ^= 1;
+            std::memcpy(st.count, cnt, sizeof(cnt));
+        }
+        lv::launch_sweep(C, P, kind, false, n);
+        LV_HIP(hipStreamSynchronize(s));
+        if (e_active) *e_active = lv::sum_slots(st.host_stats, lv::ST_E_ACTIVE);
+        LV_HIP(hipMemcpy(target, st.target.data(), n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        // restore target == comm
+        LV_HIP(hipMemcpyAsync(st.target.data(), C.communities.data(), n * sizeof(uint32_t),
+                              hipMemcpyDeviceToDevice, s));
+        LV_HIP(hipStreamSynchronize(s));
+    });
+}
+
+lv_status lv_community_modularity_parts(lv_community *c, int64_t *sigma_in, uint64_t *sum_tot_sq,
+                                        double *q) {
+    if (!c) return LV_ERR_ARG;
+    return guarded([&] {
+        c->c.compute_modularity();
+        if (sigma_in) *sigma_in = c->c.sigma_in;
+        if (sum_tot_sq) *sum_tot_sq = c->c.sum_tot_sq;
+        if (q) *q = c->c.modularity;
+    });
+}
+
+static void fill_stats(lv_kernel_stats *s, const lv::KernelStatsAcc &a) {
+    s->move_launches = a.move_launches;
+    s->move_ms = a.move_ms;
+    s->move_bytes = a.move_bytes;
+    s->move_entries = a.move_entries;
+    s->sweep_ms = a.sweep_ms;
+    s->sweeps = a.sweeps;
+}
+
+lv_status lv_kernel_stats_get(const lv_community *c, lv_kernel_stats *s) {
+    if (!c || !s) return LV_ERR_ARG;
+    fill_stats(s, c->c.state->acc);
+    return LV_OK;
+}
+
+lv_status lv_last_run_kernel_stats(lv_kernel_stats *s) {
+    if (!s) return LV_ERR_ARG;
+    fill_stats(s, lv::g_last_run_stats);
+    return LV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+// lv_kernels.h -- host launchers for the gfx950 kernels in lv_kernels.hip.
+//
+// Data layout of one level in HBM (all arrays device resident):
+//   row[n+1] u64   CSR offsets, entries of a row sorted by destination
+//   col[E]   u32   destinations
+//   wt[E]    u32   integer weights (absent at a unit-weight level 0)
+//   kdeg[n]  u32   weighted degree k_v
+//   comm[n]  u32   current community of each vertex
+//   target[n] u32  community chosen by the best-move pass (== comm unless moving)
+//   tot[n]   u32   community totals (sum of k over members), exact
+//   bucket[n] u8   degree bucket of each vertex
+//   mark[n]  u8    "re-examine next sweep" flags written by the commit pass
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace lv {
+
+constexpr int kNumBuckets = 9;        // 8 LDS-table buckets + global-table hubs
+constexpr int kHubBucket = 8;
+constexpr int kStatSlots = 64;        // spread of the per-block statistic atomics
+constexpr int kHubParts = 32;         // workgroups per hub vertex
+
+// Maximum degree handled by each LDS bucket; larger goes to the hub bucket.
+constexpr uint32_t kBucketMaxDeg[kNumBuckets - 1] = {4, 8, 16, 32, 64, 256, 1024, 4096};
+
+enum Stat : int {
+    ST_E_ACTIVE = 0,     // non-self entries of examined vertices
+    ST_CONSIDERED = 1,   // examined vertices with >= 1 non-self entry
+    ST_CANDIDATES = 2,   // distinct (vertex, neighbour community) candidates
+    ST_MOVED = 3,        // vertices that changed community
+    ST_DSIGMA = 4,       // change of sigma_in (two's complement)
+    ST_SUMSQ = 5,        // sum over communities of tot^2
+    ST_VERTS = 6,        // examined vertices
+    kNumStats = 7
+};
+
+struct LevelView {
+    const uint64_t *row;
+    const uint32_t *col;
+    const uint32_t *wt;       // nullptr: unit weights
+    const uint32_t *kdeg;
+    uint32_t n;
+    uint64_t E;
+};
+
+struct HubTables {
+    uint32_t *slot_of;        // [n] hub slot of a vertex (LV_EMPTY if not a hub)
+    uint64_t *tab_off;        // [n_hubs] offset of the hub's table
+    uint32_t *tab_log2;       // [n_hubs] log2 of its table size
+    uint32_t *key;            // pooled table keys (kept LV_EMPTY between uses)
+    uint32_t *val;            // pooled table weights (kept 0 between uses)
+    int64_t *part_s;          // [n_hubs * kHubParts] partial best scores
+    uint32_t *part_c;         // [n_hubs * kHubParts] partial best communities
+    uint32_t *self;           // [n_hubs] weight to own community
+};
+
+struct SweepArgs {
+    LevelView g;
+    const uint32_t *comm;
+    uint32_t *target;
+    uint32_t *tot;
+    uint8_t *mark;
+    const uint8_t *bucket;
+    unsigned long long *stats;   // [kNumStats * kStatSlots]
+    int64_t m2;                  // 2m as an integer
+    double tw;                   // the reference's total_weight (m)
+    uint32_t v_end;              // vertices >= v_end are not examined (chunk emulation)
+    int kind;                    // 0 fast, 1 sort, 2 hash
+    int gain;                    // 0 exact, 1 ref_f32
+    bool highest;                // ties to the highest community id
+};
+
+// ---- primitives ----
+void fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
+void fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
+void fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipStream_t s);
+void iota_u32(uint32_t *p, size_t n, hipStream_t s);
+// dst[idx[i]] = val[i]
+void scatter_u32(uint32_t *dst, const uint32_t *idx, const uint32_t *val, size_t n, hipStream_t s);
+// tot[c] = sum of kdeg over vertices with comm == c
+void community_totals(const uint32_t *comm, const uint32_t *kdeg, uint32_t n, uint32_t *tot,
+                      hipStream_t s);
+size_t scan_scratch_elems(size_t n);   // elements of T scratch for scan
+void scan_u32(const uint32_t *in, uint32_t *out, size_t n, bool inclusive, uint32_t *scratch,
+              hipStream_t s);
+void scan_u64(const uint64_t *in, uint64_t *out, size_t n, bool inclusive, uint64_t *scratch,
+              hipStream_t s);
+// LSD radix sort of (key, val) pairs over the low `bits` bits of key.  Result
+// lands in (k0, v0); (k1, v1) are same-sized buffers; hist needs
+// radix_hist_elems(n) u32.
+size_t radix_hist_elems(size_t n);
+void radix_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, size_t n, int bits,
+                      uint32_t *hist, uint32_t *scan_scratch, hipStream_t s);
+
+// ---- level setup ----
+void bucket_ids(const uint64_t *row, uint32_t n, uint8_t *bucket, hipStream_t s);
+// lists[b] receives the vertices of bucket b (all vertices); counts[b] their number.
+void build_lists_all(const uint8_t *bucket, uint32_t n, uint32_t *const *lists, uint32_t *counts,
+                     hipStream_t s);
+// sigma_in of partition comm from scratch, and sum of tot^2 (stats slots)
+void modularity_parts(const LevelView &g, const uint32_t *comm, const uint32_t *tot,
+                      unsigned long long *stats, hipStream_t s);
+
+// ---- sweep ----
+void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
+               const uint32_t *counts, hipStream_t s);
+void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *counts,
+                 hipStream_t s);
+// comm := target for movers, sum tot^2, compact marked vertices into next lists
+void finalize_pass(const SweepArgs &a, uint32_t *const *next_lists, uint32_t *next_counts,
+                   hipStream_t s);
+
+// ---- aggregation ----
+// new_id[c] = map[c]-1 for communities with tot>0; n_new via *map_last (device)
+void renumber_flags(const uint32_t *tot, uint32_t n, uint32_t *flags, hipStream_t s);
+void update_best(uint32_t *best, uint32_t n0, const uint32_t *comm, const uint32_t *map,
+                 hipStream_t s);
+void relabel_entries(const LevelView &g, const uint32_t *comm, const uint32_t *map, uint32_t n_new,
+                     uint64_t *keys, uint32_t *vals, hipStream_t s);
+void run_heads(const uint64_t *keys, size_t n, uint32_t *heads, hipStream_t s);
+void reduce_runs(const uint64_t *keys, const uint32_t *vals, const uint32_t *idx, size_t n,
+                 uint32_t *out_col, uint32_t *out_w, uint64_t *out_row, uint32_t n_new,
+                 hipStream_t s);
+void new_degrees(const uint32_t *tot, const uint32_t *map, uint32_t n, uint32_t *k_new,
+                 hipStream_t s);
+
+}  // namespace lv

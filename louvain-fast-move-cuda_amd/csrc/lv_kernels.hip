@@ -1,0 +1,1293 @@
+// lv_kernels.hip -- gfx950 kernels of the Louvain optimisation/aggregation loop.
+//
+// Best-move pass (reference: phaseOptimizationSort.cu:28-379,
+// phaseOptimizationHash.cu:23-162, hashmap.cuh:19-73): vertices are grouped by
+// degree; each degree bucket runs a kernel in which a group of G lanes (a
+// quarter wave up to a whole workgroup) owns one vertex, accumulates the
+// weights towards each neighbour community in an LDS open-addressing table
+// (2x the degree, so it never overflows), looks up its own community's weight,
+// scores every candidate and reduces to the best one.  Vertices above 4096
+// entries use pooled global-memory tables shared by 32 workgroups.  The first
+// sweep of every level runs at the identity partition and needs no table.
+//
+// Commit pass (phaseOptimizationUpdate.cu:14-132): movers update the exact
+// community totals with integer atomics, add their contribution to the change
+// of sigma_in, and mark neighbours left in another community (push form of
+// update_changed_kernel; equal to the pull form because the adjacency is
+// symmetric).  Finalize pass: comm := target, sum of tot^2, and compaction of
+// the marked vertices into per-bucket active lists for the next sweep.
+//
+// Aggregation (phaseAggregation.cuh:19-175): non-empty flags, inclusive scan
+// (renumbering), relabelled entries sorted by an LSD radix sort, run-length
+// reduction into the contracted CSR.
+#include "lv_kernels.h"
+
+#include "../../include/constants.h"
+#include "../../include/operators.h"
+
+#include <climits>
+#include <cmath>
+
+namespace lv {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ inline unsigned long long lanemask_lt() {
+    const int lane = threadIdx.x & 63;
+    return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+}
+
+__device__ inline int64_t shfl_xor_i64(int64_t v, int off) {
+    const int lo = __shfl_xor((int)(uint32_t)(uint64_t)v, off);
+    const int hi = __shfl_xor((int)(uint32_t)((uint64_t)v >> 32), off);
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+}
+
+__device__ inline uint64_t shfl_xor_u64(uint64_t v, int off) {
+    return (uint64_t)shfl_xor_i64((int64_t)v, off);
+}
+
+__device__ inline float shfl_xor_f32(float v, int off) { return __shfl_xor(v, off); }
+
+template <typename S>
+__device__ inline S shfl_xor_s(S v, int off);
+template <>
+__device__ inline int64_t shfl_xor_s<int64_t>(int64_t v, int off) { return shfl_xor_i64(v, off); }
+template <>
+__device__ inline float shfl_xor_s<float>(float v, int off) { return shfl_xor_f32(v, off); }
+
+// Butterfly reduction of (score, community) inside aligned groups of G lanes
+// (G <= 64); every lane of the group ends with the group's best.
+template <int G, typename S>
+__device__ inline void group_best(S &s, uint32_t &c, bool highest) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        const S os = shfl_xor_s<S>(s, off);
+        const uint32_t oc = (uint32_t)__shfl_xor((int)c, off);
+        if (better_candidate(os, oc, s, c, highest)) { s = os; c = oc; }
+    }
+}
+
+template <int G>
+__device__ inline uint64_t group_sum_u64(uint64_t x) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) x += shfl_xor_u64(x, off);
+    return x;
+}
+
+// Block-wide sum of up to 3 counters; thread 0 adds them to the spread slots.
+__device__ inline void block_add_stats(unsigned long long *stats, int st0, uint64_t a0, int st1,
+                                       uint64_t a1, int st2, uint64_t a2) {
+    __shared__ uint64_t red[3][kBlock / 64];
+    a0 = group_sum_u64<64>(a0);
+    a1 = group_sum_u64<64>(a1);
+    a2 = group_sum_u64<64>(a2);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][w] = a0; red[1][w] = a1; red[2][w] = a2; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t s0 = 0, s1 = 0, s2 = 0;
+        for (int i = 0; i < kBlock / 64; ++i) { s0 += red[0][i]; s1 += red[1][i]; s2 += red[2][i]; }
+        const int slot = blockIdx.x & (kStatSlots - 1);
+        if (st0 >= 0 && s0) atomicAdd(&stats[st0 * kStatSlots + slot], (unsigned long long)s0);
+        if (st1 >= 0 && s1) atomicAdd(&stats[st1 * kStatSlots + slot], (unsigned long long)s1);
+        if (st2 >= 0 && s2) atomicAdd(&stats[st2 * kStatSlots + slot], (unsigned long long)s2);
+    }
+    __syncthreads();
+}
+
+template <int LOG2T>
+__device__ inline uint32_t slot_hash(uint32_t c) {
+    return (c * 2654435761u) >> (32 - LOG2T);
+}
+
+__device__ inline uint32_t slot_hash_dyn(uint32_t c, uint32_t log2t) {
+    return (c * 2654435761u) >> (32 - log2t);
+}
+
+constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
+
+template <int GAIN>
+struct Score;
+template <>
+struct Score<0> {
+    using T = int64_t;
+    __device__ static T none() { return LLONG_MIN; }
+    __device__ static T of(uint32_t w, uint32_t self, uint32_t k, uint32_t tot_own, uint32_t tot_c,
+                           int64_t m2, double) {
+        (void)self; (void)tot_own;
+        return move_score_exact(w, k, tot_c, m2);
+    }
+    __device__ static bool positive(T best, uint32_t self, uint32_t k, uint32_t tot_own, int64_t m2) {
+        return delta_modularity_exact(best, self, k, tot_own, m2) > 0;
+    }
+};
+template <>
+struct Score<1> {
+    using T = float;
+    __device__ static T none() { return -INFINITY; }
+    __device__ static T of(uint32_t w, uint32_t self, uint32_t k, uint32_t tot_own, uint32_t tot_c,
+                           int64_t, double tw) {
+        return delta_modularity_ref_f32(w, self, k, tot_own, tot_c, tw);
+    }
+    __device__ static bool positive(T best, uint32_t, uint32_t, uint32_t, int64_t) {
+        return best > 0.0f;
+    }
+};
+
+// ---------------------------------------------------------------- utilities
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_u8(uint8_t *p, uint8_t v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_iota_u32(uint32_t *p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = (uint32_t)i;
+}
+
+__global__ void k_scatter_u32(uint32_t *dst, const uint32_t *idx, const uint32_t *val, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[idx[i]] = val[i];
+}
+
+__global__ void k_community_totals(const uint32_t *comm, const uint32_t *kdeg, uint32_t n,
+                                   uint32_t *tot) {
+    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n;
+         v += (size_t)gridDim.x * blockDim.x)
+        atomicAdd(&tot[comm[v]], kdeg[v]);
+}
+
+inline unsigned grid_for(size_t n, size_t per_block, unsigned cap = 65536u) {
+    size_t g = (n + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+// ------------------------------------------------------------------- scan
+
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+template <typename T>
+__device__ inline T wave_incl_scan(T x) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        T y;
+        if constexpr (sizeof(T) == 8) {
+            const int lo = __shfl_up((int)(uint32_t)(uint64_t)x, off);
+            const int hi = __shfl_up((int)(uint32_t)((uint64_t)x >> 32), off);
+            y = (T)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+        } else {
+            y = (T)__shfl_up((int)x, off);
+        }
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+// Exclusive block scan of one value per thread; returns the block total in *total.
+template <typename T>
+__device__ inline T block_excl_scan(T x, T *total) {
+    __shared__ T wsum[kBlock / 64];
+    const T inc = wave_incl_scan<T>(x);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) wsum[w] = inc;
+    __syncthreads();
+    T off = 0, tot = 0;
+    for (int i = 0; i < kBlock / 64; ++i) {
+        if (i < w) off += wsum[i];
+        tot += wsum[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - x;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scan_tile(const T *in, T *out, size_t n, T *block_sums,
+                                                      int inclusive) {
+    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    T v[kScanItems];
+    T run = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        const size_t e = base + i;
+        v[i] = e < n ? in[e] : (T)0;
+        run += v[i];
+    }
+    T total;
+    T off = block_excl_scan<T>(run, &total);
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        const size_t e = base + i;
+        const T x = v[i];
+        if (e < n) out[e] = inclusive ? off + x : off;
+        off += x;
+    }
+    if (threadIdx.x == 0 && block_sums) block_sums[blockIdx.x] = total;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scan_add(T *out, size_t n, const T *block_off) {
+    const T add = block_off[blockIdx.x];
+    const size_t base = (size_t)blockIdx.x * kScanTile;
+    for (int i = threadIdx.x; i < kScanTile; i += kBlock) {
+        const size_t e = base + i;
+        if (e < n) out[e] += add;
+    }
+}
+
+template <typename T>
+void scan_impl(const T *in, T *out, size_t n, bool inclusive, T *scratch, hipStream_t s) {
+    if (n == 0) return;
+    const size_t nb = (n + kScanTile - 1) / kScanTile;
+    if (nb == 1) {
+        hipLaunchKernelGGL(k_scan_tile<T>, dim3(1), dim3(kBlock), 0, s, in, out, n, (T *)nullptr,
+                           inclusive ? 1 : 0);
+        return;
+    }
+    T *sums = scratch;
+    T *rest = scratch + nb;
+    hipLaunchKernelGGL(k_scan_tile<T>, dim3((unsigned)nb), dim3(kBlock), 0, s, in, out, n, sums,
+                       inclusive ? 1 : 0);
+    scan_impl<T>(sums, sums, nb, false, rest, s);
+    hipLaunchKernelGGL(k_scan_add<T>, dim3((unsigned)nb), dim3(kBlock), 0, s, out, n,
+                       (const T *)sums);
+}
+
+// ------------------------------------------------------------- radix sort
+
+constexpr int kRsItems = 16;
+constexpr int kRsTile = kBlock * kRsItems;
+
+__global__ __launch_bounds__(kBlock) void k_rs_upsweep(const uint64_t *keys, size_t n, int shift,
+                                                       uint32_t *hist, uint32_t nblocks) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * kRsTile;
+    for (int i = 0; i < kRsItems; ++i) {
+        const size_t e = base + (size_t)i * kBlock + threadIdx.x;
+        if (e < n) atomicAdd(&h[(uint32_t)(keys[e] >> shift) & 0xFFu], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: tile element e = i*256 + tid keeps its order among equal digits.
+__global__ __launch_bounds__(kBlock) void k_rs_downsweep(const uint64_t *kin, const uint32_t *vin,
+                                                         uint64_t *kout, uint32_t *vout, size_t n,
+                                                         int shift, const uint32_t *hist_off,
+                                                         uint32_t nblocks) {
+    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_run[256];
+    __shared__ uint32_t s_w[kBlock / 64][256];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    s_base[tid] = hist_off[(size_t)tid * nblocks + blockIdx.x];
+    s_run[tid] = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) s_w[w][tid] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * kRsTile;
+    const unsigned long long lt = lanemask_lt();
+    for (int i = 0; i < kRsItems; ++i) {
+        const size_t e = base + (size_t)i * kBlock + tid;
+        const bool valid = e < n;
+        uint64_t key = 0;
+        uint32_t val = 0;
+        uint32_t d = 0;
+        if (valid) {
+            key = kin[e];
+            val = vin[e];
+            d = (uint32_t)(key >> shift) & 0xFFu;
+        }
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const unsigned long long m = __ballot(valid && bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t lower = (uint32_t)__popcll(peers & lt);
+        if (valid && lower == 0) s_w[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {
+            uint32_t r = s_run[tid];
+#pragma unroll
+            for (int w = 0; w < kBlock / 64; ++w) {
+                const uint32_t c = s_w[w][tid];
+                s_w[w][tid] = r;
+                r += c;
+            }
+            s_run[tid] = r;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t pos = s_base[d] + s_w[wave][d] + lower;
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) s_w[w][tid] = 0;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ level setup
+
+__device__ inline uint8_t bucket_of_degree(uint64_t d) {
+#pragma unroll
+    for (int b = 0; b < kNumBuckets - 1; ++b)
+        if (d <= kBucketMaxDeg[b]) return (uint8_t)b;
+    return (uint8_t)kHubBucket;
+}
+
+__global__ void k_bucket_ids(const uint64_t *row, uint32_t n, uint8_t *bucket) {
+    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n;
+         v += (size_t)gridDim.x * blockDim.x)
+        bucket[v] = bucket_of_degree(row[v + 1] - row[v]);
+}
+
+struct ListPtrs {
+    uint32_t *p[kNumBuckets];
+};
+
+// Append the vertex of each thread (if selected) to the list of its bucket,
+// with one global atomic per bucket per tile.
+__device__ inline void append_to_bucket(bool sel, uint32_t v, int b, const ListPtrs &lists,
+                                        uint32_t *counts) {
+    __shared__ uint32_t s_cnt[kNumBuckets];
+    __shared__ uint32_t s_base[kNumBuckets];
+    if (threadIdx.x < kNumBuckets) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t rank = 0;
+    if (sel) rank = atomicAdd(&s_cnt[b], 1u);
+    __syncthreads();
+    if (threadIdx.x < kNumBuckets) {
+        const uint32_t c = s_cnt[threadIdx.x];
+        s_base[threadIdx.x] = c ? atomicAdd(&counts[threadIdx.x], c) : 0u;
+    }
+    __syncthreads();
+    if (sel) lists.p[b][s_base[b] + rank] = v;
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBlock) void k_build_lists_all(const uint8_t *bucket, uint32_t n,
+                                                            ListPtrs lists, uint32_t *counts) {
+    const uint32_t ntiles = (n + kBlock - 1) / kBlock;
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t v = t * kBlock + threadIdx.x;
+        const bool sel = v < n;
+        append_to_bucket(sel, v, sel ? bucket[v] : 0, lists, counts);
+    }
+}
+
+// First entry index handled by a thread chunk, and its source vertex.
+__device__ inline uint32_t src_of_entry(const uint64_t *row, uint32_t n, uint64_t j) {
+    uint32_t lo = 0, hi = n;   // largest v with row[v] <= j
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (row[mid] <= j) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+constexpr int kChunk = 16;   // entries per thread in entry-parallel kernels
+
+// sigma_in = sum of w over entries with comm[src]==comm[dst]; sum of tot^2.
+__global__ __launch_bounds__(kBlock) void k_modularity_parts(LevelView g, const uint32_t *comm,
+                                                             const uint32_t *tot,
+                                                             unsigned long long *stats) {
+    uint64_t sig = 0, sq = 0;
+    const uint64_t nchunks = (g.E + kChunk - 1) / kChunk;
+    for (uint64_t ch = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; ch < nchunks;
+         ch += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t j0 = ch * kChunk;
+        const uint64_t j1 = j0 + kChunk < g.E ? j0 + kChunk : g.E;
+        uint32_t v = src_of_entry(g.row, g.n, j0);
+        uint64_t vend = g.row[v + 1];
+        uint32_t cv = comm[v];
+        for (uint64_t j = j0; j < j1; ++j) {
+            while (j >= vend) { ++v; vend = g.row[v + 1]; cv = comm[v]; }
+            if (internal_entry(cv, comm[g.col[j]])) sig += g.wt ? g.wt[j] : 1u;
+        }
+    }
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < g.n;
+         c += (uint64_t)gridDim.x * blockDim.x)
+        sq += square_u64(tot[c]);
+    block_add_stats(stats, ST_DSIGMA, sig, ST_SUMSQ, sq, -1, 0);
+}
+
+// ---------------------------------------------------------- best-move pass
+
+struct MoveCtx {
+    const uint64_t *row;
+    const uint32_t *col;
+    const uint32_t *wt;
+    const uint32_t *kdeg;
+    const uint32_t *comm;
+    const uint32_t *tot;
+    uint32_t *target;
+    unsigned long long *stats;
+    int64_t m2;
+    double tw;
+    uint32_t v_end;
+    bool highest;
+};
+
+// Group-per-vertex with an LDS table of T slots (T >= 2 * max degree).
+template <int G, int T, int GAIN, bool WEIGHTED>
+__global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t *__restrict__ list,
+                                                       uint32_t count) {
+    using SC = Score<GAIN>;
+    using S = typename SC::T;
+    constexpr int GPB = kBlock / G;
+    constexpr int LOG2T = ilog2(T);
+    __shared__ uint32_t s_key[GPB * T];
+    __shared__ uint32_t s_val[GPB * T];
+    const int lane = threadIdx.x & (G - 1);
+    const int gi = threadIdx.x / G;
+    const uint32_t idx = blockIdx.x * GPB + gi;
+    const bool valid = idx < count;
+    const uint32_t v = valid ? list[idx] : 0u;
+    const bool live = valid && v < a.v_end;
+    uint32_t *K = s_key + gi * T;
+    uint32_t *V = s_val + gi * T;
+    for (int s = lane; s < T; s += G) { K[s] = LV_EMPTY; V[s] = 0u; }
+    __syncthreads();
+    uint32_t own = 0;
+    uint64_t nself = 0;
+    if (live) {
+        own = a.comm[v];
+        const uint64_t beg = a.row[v], end = a.row[v + 1];
+        for (uint64_t j = beg + lane; j < end; j += G) {
+            const uint32_t x = a.col[j];
+            if (x == v) continue;
+            ++nself;
+            const uint32_t c = a.comm[x];
+            const uint32_t w = WEIGHTED ? a.wt[j] : 1u;
+            uint32_t h = slot_hash<LOG2T>(c);
+            for (;;) {
+                const uint32_t prev = atomicCAS(&K[h], LV_EMPTY, c);
+                if (prev == LV_EMPTY || prev == c) { atomicAdd(&V[h], w); break; }
+                h = (h + 1) & (T - 1);
+            }
+        }
+    }
+    __syncthreads();
+    S best = SC::none();
+    uint32_t best_c = a.highest ? 0u : LV_EMPTY;
+    uint64_t ncand = 0;
+    uint32_t self = 0, kv = 0, tot_own = 0;
+    if (live) {
+        uint32_t h = slot_hash<LOG2T>(own);
+        for (;;) {
+            const uint32_t c = K[h];
+            if (c == own) { self = V[h]; break; }
+            if (c == LV_EMPTY) break;
+            h = (h + 1) & (T - 1);
+        }
+        kv = a.kdeg[v];
+        tot_own = a.tot[own];
+        for (int s = lane; s < T; s += G) {
+            const uint32_t c = K[s];
+            if (c == LV_EMPTY || c == own) continue;
+            ++ncand;
+            const S sc = SC::of(V[s], self, kv, tot_own, a.tot[c], a.m2, a.tw);
+            if (better_candidate(sc, c, best, best_c, a.highest)) { best = sc; best_c = c; }
+        }
+    }
+    uint64_t considered = 0;
+    if constexpr (G <= 64) {
+        group_best<G, S>(best, best_c, a.highest);
+        const uint64_t gself = group_sum_u64<G>(nself);
+        if (live && lane == 0) {
+            considered = gself > 0;
+            if (best != SC::none() && SC::positive(best, self, kv, tot_own, a.m2))
+                a.target[v] = best_c;
+        }
+    } else {
+        __shared__ S r_s[kBlock / 64];
+        __shared__ uint32_t r_c[kBlock / 64];
+        __shared__ uint64_t r_n[kBlock / 64];
+        group_best<64, S>(best, best_c, a.highest);
+        const uint64_t wself = group_sum_u64<64>(nself);
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) { r_s[w] = best; r_c[w] = best_c; r_n[w] = wself; }
+        __syncthreads();
+        if (threadIdx.x == 0 && live) {
+            uint64_t tn = 0;
+            for (int i = 0; i < kBlock / 64; ++i) {
+                tn += r_n[i];
+                if (better_candidate(r_s[i], r_c[i], best, best_c, a.highest)) {
+                    best = r_s[i];
+                    best_c = r_c[i];
+                }
+            }
+            considered = tn > 0;
+            if (best != SC::none() && SC::positive(best, self, kv, tot_own, a.m2))
+                a.target[v] = best_c;
+        }
+    }
+    block_add_stats(a.stats, ST_E_ACTIVE, nself, ST_CANDIDATES, ncand, ST_CONSIDERED, considered);
+}
+
+// First sweep of a level (identity partition, phaseOptimizationSort.cu:28-158):
+// every non-self entry is its own candidate community, self = 0, and a move
+// also needs v < c (phaseOptimizationUpdate.cu:99).  Ties: lowest id.
+template <int G, int GAIN, bool WEIGHTED>
+__global__ __launch_bounds__(kBlock) void k_move_fast(MoveCtx a, const uint32_t *__restrict__ list,
+                                                      uint32_t count) {
+    using SC = Score<GAIN>;
+    using S = typename SC::T;
+    constexpr int GPB = kBlock / G;
+    const int lane = threadIdx.x & (G - 1);
+    const int gi = threadIdx.x / G;
+    const uint32_t idx = blockIdx.x * GPB + gi;
+    const bool valid = idx < count;
+    const uint32_t v = valid ? list[idx] : 0u;
+    const bool live = valid && v < a.v_end;
+    S best = SC::none();
+    uint32_t best_c = LV_EMPTY;
+    uint64_t nself = 0;
+    uint32_t kv = 0, tot_own = 0;
+    if (live) {
+        kv = a.kdeg[v];
+        tot_own = a.tot[v];
+        const uint64_t beg = a.row[v], end = a.row[v + 1];
+        for (uint64_t j = beg + lane; j < end; j += G) {
+            const uint32_t x = a.col[j];
+            if (x == v) continue;
+            ++nself;
+            const uint32_t w = WEIGHTED ? a.wt[j] : 1u;
+            const S sc = SC::of(w, 0u, kv, tot_own, a.tot[x], a.m2, a.tw);
+            if (better_candidate(sc, x, best, best_c, false)) { best = sc; best_c = x; }
+        }
+    }
+    uint64_t considered = 0;
+    if constexpr (G <= 64) {
+        group_best<G, S>(best, best_c, false);
+        const uint64_t gself = group_sum_u64<G>(nself);
+        if (live && lane == 0) {
+            considered = gself > 0;
+            if (best != SC::none() && SC::positive(best, 0u, kv, tot_own, a.m2) && v < best_c)
+                a.target[v] = best_c;
+        }
+    } else {
+        __shared__ S r_s[kBlock / 64];
+        __shared__ uint32_t r_c[kBlock / 64];
+        __shared__ uint64_t r_n[kBlock / 64];
+        group_best<64, S>(best, best_c, false);
+        const uint64_t wself = group_sum_u64<64>(nself);
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) { r_s[w] = best; r_c[w] = best_c; r_n[w] = wself; }
+        __syncthreads();
+        if (threadIdx.x == 0 && live) {
+            uint64_t tn = 0;
+            for (int i = 0; i < kBlock / 64; ++i) {
+                tn += r_n[i];
+                if (better_candidate(r_s[i], r_c[i], best, best_c, false)) {
+                    best = r_s[i];
+                    best_c = r_c[i];
+                }
+            }
+            considered = tn > 0;
+            if (best != SC::none() && SC::positive(best, 0u, kv, tot_own, a.m2) && v < best_c)
+                a.target[v] = best_c;
+        }
+    }
+    block_add_stats(a.stats, ST_E_ACTIVE, nself, ST_CANDIDATES, nself, ST_CONSIDERED, considered);
+}
+
+// ---- hubs: kHubParts workgroups per vertex, pooled global tables ----
+
+struct HubCtx {
+    MoveCtx m;
+    HubTables h;
+};
+
+template <bool WEIGHTED>
+__global__ __launch_bounds__(kBlock) void k_hub_insert(HubCtx a, const uint32_t *__restrict__ list,
+                                                       uint32_t count) {
+    const uint32_t hi = blockIdx.x / kHubParts;
+    const uint32_t part = blockIdx.x % kHubParts;
+    uint64_t nself = 0;
+    if (hi < count) {
+        const uint32_t v = list[hi];
+        if (v < a.m.v_end) {
+            const uint32_t slot = a.h.slot_of[v];
+            const uint64_t off = a.h.tab_off[slot];
+            const uint32_t lg = a.h.tab_log2[slot];
+            const uint32_t mask = (1u << lg) - 1u;
+            uint32_t *K = a.h.key + off;
+            uint32_t *V = a.h.val + off;
+            const uint64_t beg = a.m.row[v], end = a.m.row[v + 1];
+            for (uint64_t j = beg + (uint64_t)part * kBlock + threadIdx.x; j < end;
+                 j += (uint64_t)kHubParts * kBlock) {
+                const uint32_t x = a.m.col[j];
+                if (x == v) continue;
+                ++nself;
+                const uint32_t c = a.m.comm[x];
+                const uint32_t w = WEIGHTED ? a.m.wt[j] : 1u;
+                uint32_t h = slot_hash_dyn(c, lg);
+                for (;;) {
+                    const uint32_t prev = atomicCAS(&K[h], LV_EMPTY, c);
+                    if (prev == LV_EMPTY || prev == c) { atomicAdd(&V[h], w); break; }
+                    h = (h + 1) & mask;
+                }
+            }
+        }
+    }
+    block_add_stats(a.m.stats, ST_E_ACTIVE, nself, -1, 0, -1, 0);
+}
+
+__global__ void k_hub_self(HubCtx a, const uint32_t *__restrict__ list, uint32_t count) {
+    const uint32_t hi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (hi >= count) return;
+    const uint32_t v = list[hi];
+    const uint32_t slot = a.h.slot_of[v];
+    uint32_t self = 0;
+    if (v < a.m.v_end) {
+        const uint32_t own = a.m.comm[v];
+        const uint64_t off = a.h.tab_off[slot];
+        const uint32_t lg = a.h.tab_log2[slot];
+        const uint32_t mask = (1u << lg) - 1u;
+        uint32_t h = slot_hash_dyn(own, lg);
+        for (;;) {
+            const uint32_t c = a.h.key[off + h];
+            if (c == own) { self = a.h.val[off + h]; break; }
+            if (c == LV_EMPTY) break;
+            h = (h + 1) & mask;
+        }
+    }
+    a.h.self[slot] = self;
+}
+
+// Scan one part of the hub's table; record the part's best candidate; clear
+// the scanned slots so the pool stays empty between sweeps.
+template <int GAIN>
+__global__ __launch_bounds__(kBlock) void k_hub_scan(HubCtx a, const uint32_t *__restrict__ list,
+                                                     uint32_t count) {
+    using SC = Score<GAIN>;
+    using S = typename SC::T;
+    const uint32_t hi = blockIdx.x / kHubParts;
+    const uint32_t part = blockIdx.x % kHubParts;
+    S best = SC::none();
+    uint32_t best_c = a.m.highest ? 0u : LV_EMPTY;
+    uint64_t ncand = 0;
+    uint32_t slot = 0;
+    const bool ok = hi < count;
+    if (ok) {
+        const uint32_t v = list[hi];
+        slot = a.h.slot_of[v];
+        if (v < a.m.v_end) {
+            const uint32_t own = a.m.comm[v];
+            const uint64_t off = a.h.tab_off[slot];
+            const uint32_t size = 1u << a.h.tab_log2[slot];
+            const uint32_t per = size / kHubParts;
+            const uint32_t kv = a.m.kdeg[v];
+            const uint32_t tot_own = a.m.tot[own];
+            const uint32_t self = a.h.self[slot];
+            for (uint32_t s = part * per + threadIdx.x; s < (part + 1) * per; s += kBlock) {
+                const uint32_t c = a.h.key[off + s];
+                if (c == LV_EMPTY) continue;
+                const uint32_t W = a.h.val[off + s];
+                a.h.key[off + s] = LV_EMPTY;
+                a.h.val[off + s] = 0u;
+                if (c == own) continue;
+                ++ncand;
+                const S sc = SC::of(W, self, kv, tot_own, a.m.tot[c], a.m.m2, a.m.tw);
+                if (better_candidate(sc, c, best, best_c, a.m.highest)) { best = sc; best_c = c; }
+            }
+        }
+    }
+    __shared__ S r_s[kBlock / 64];
+    __shared__ uint32_t r_c[kBlock / 64];
+    group_best<64, S>(best, best_c, a.m.highest);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { r_s[w] = best; r_c[w] = best_c; }
+    __syncthreads();
+    if (threadIdx.x == 0 && ok) {
+        for (int i = 0; i < kBlock / 64; ++i)
+            if (better_candidate(r_s[i], r_c[i], best, best_c, a.m.highest)) {
+                best = r_s[i];
+                best_c = r_c[i];
+            }
+        if constexpr (GAIN == 0) {
+            a.h.part_s[(size_t)slot * kHubParts + part] = best;
+        } else {
+            a.h.part_s[(size_t)slot * kHubParts + part] = (int64_t)__float_as_int(best);
+        }
+        a.h.part_c[(size_t)slot * kHubParts + part] = best_c;
+    }
+    block_add_stats(a.m.stats, ST_CANDIDATES, ncand, -1, 0, -1, 0);
+}
+
+// Fast sweep for hubs: per-part best over entries (no table).
+template <int GAIN, bool WEIGHTED>
+__global__ __launch_bounds__(kBlock) void k_hub_fast(HubCtx a, const uint32_t *__restrict__ list,
+                                                     uint32_t count) {
+    using SC = Score<GAIN>;
+    using S = typename SC::T;
+    const uint32_t hi = blockIdx.x / kHubParts;
+    const uint32_t part = blockIdx.x % kHubParts;
+    S best = SC::none();
+    uint32_t best_c = LV_EMPTY;
+    uint64_t nself = 0;
+    uint32_t slot = 0;
+    const bool ok = hi < count;
+    if (ok) {
+        const uint32_t v = list[hi];
+        slot = a.h.slot_of[v];
+        if (v < a.m.v_end) {
+            const uint32_t kv = a.m.kdeg[v];
+            const uint32_t tot_own = a.m.tot[v];
+            const uint64_t beg = a.m.row[v], end = a.m.row[v + 1];
+            for (uint64_t j = beg + (uint64_t)part * kBlock + threadIdx.x; j < end;
+                 j += (uint64_t)kHubParts * kBlock) {
+                const uint32_t x = a.m.col[j];
+                if (x == v) continue;
+                ++nself;
+                const uint32_t wgt = WEIGHTED ? a.m.wt[j] : 1u;
+                const S sc = SC::of(wgt, 0u, kv, tot_own, a.m.tot[x], a.m.m2, a.m.tw);
+                if (better_candidate(sc, x, best, best_c, false)) { best = sc; best_c = x; }
+            }
+        }
+    }
+    __shared__ S r_s[kBlock / 64];
+    __shared__ uint32_t r_c[kBlock / 64];
+    group_best<64, S>(best, best_c, false);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { r_s[w] = best; r_c[w] = best_c; }
+    __syncthreads();
+    if (threadIdx.x == 0 && ok) {
+        for (int i = 0; i < kBlock / 64; ++i)
+            if (better_candidate(r_s[i], r_c[i], best, best_c, false)) {
+                best = r_s[i];
+                best_c = r_c[i];
+            }
+        if constexpr (GAIN == 0) {
+            a.h.part_s[(size_t)slot * kHubParts + part] = best;
+        } else {
+            a.h.part_s[(size_t)slot * kHubParts + part] = (int64_t)__float_as_int(best);
+        }
+        a.h.part_c[(size_t)slot * kHubParts + part] = best_c;
+        if (part == 0) a.h.self[slot] = 0u;
+    }
+    block_add_stats(a.m.stats, ST_E_ACTIVE, nself, ST_CANDIDATES, nself, -1, 0);
+}
+
+// One wave per hub: reduce the parts, apply the move rule.
+template <int GAIN>
+__global__ void k_hub_decide(HubCtx a, const uint32_t *__restrict__ list, uint32_t count,
+                             int fast) {
+    using SC = Score<GAIN>;
+    using S = typename SC::T;
+    const uint32_t hi = blockIdx.x;
+    if (hi >= count) return;
+    const uint32_t v = list[hi];
+    const uint32_t slot = a.h.slot_of[v];
+    const int lane = threadIdx.x;
+    const bool highest = fast ? false : a.m.highest;
+    S best = SC::none();
+    uint32_t best_c = highest ? 0u : LV_EMPTY;
+    if (lane < kHubParts) {
+        const int64_t raw = a.h.part_s[(size_t)slot * kHubParts + lane];
+        if constexpr (GAIN == 0) best = raw;
+        else best = __int_as_float((int)raw);
+        best_c = a.h.part_c[(size_t)slot * kHubParts + lane];
+    }
+    group_best<64, S>(best, best_c, highest);
+    if (lane == 0 && v < a.m.v_end) {
+        const uint32_t own = a.m.comm[v];
+        const uint32_t kv = a.m.kdeg[v];
+        const uint32_t tot_own = a.m.tot[own];
+        const uint32_t self = a.h.self[slot];
+        const bool has = best != SC::none();
+        if (has && SC::positive(best, self, kv, tot_own, a.m.m2) && (!fast || v < best_c))
+            a.m.target[v] = best_c;
+        // a hub row holds > 4096 entries and at most two self-loop entries, so
+        // it always has candidates
+        atomicAdd(&a.m.stats[ST_CONSIDERED * kStatSlots + (hi & (kStatSlots - 1))], 1ull);
+    }
+}
+
+// ------------------------------------------------------------- commit pass
+
+struct CommitCtx {
+    const uint64_t *row;
+    const uint32_t *col;
+    const uint32_t *wt;
+    const uint32_t *kdeg;
+    const uint32_t *comm;
+    const uint32_t *target;
+    uint32_t *tot;
+    uint8_t *mark;
+    unsigned long long *stats;
+};
+
+// Group (G <= 64) or `parts` workgroups (G == 256) per listed vertex.
+template <int G, bool WEIGHTED>
+__global__ __launch_bounds__(kBlock) void k_commit(CommitCtx a, const uint32_t *__restrict__ list,
+                                                   uint32_t count, uint32_t parts) {
+    constexpr int GPB = kBlock / G;
+    const int lane = threadIdx.x & (G - 1);
+    uint32_t idx, part;
+    if constexpr (G == kBlock) {
+        idx = blockIdx.x / parts;
+        part = blockIdx.x % parts;
+    } else {
+        idx = blockIdx.x * GPB + threadIdx.x / G;
+        part = 0;
+    }
+    int64_t dsig = 0;
+    if (idx < count) {
+        const uint32_t v = list[idx];
+        const uint32_t own = a.comm[v];
+        const uint32_t c = a.target[v];
+        if (c != own) {
+            if (lane == 0 && part == 0) {
+                const uint32_t kv = a.kdeg[v];
+                atomicAdd(&a.tot[c], kv);
+                atomicSub(&a.tot[own], kv);
+            }
+            const uint64_t beg = a.row[v], end = a.row[v + 1];
+            const uint64_t stride = (uint64_t)G * (G == kBlock ? parts : 1u);
+            for (uint64_t j = beg + (uint64_t)part * G + lane; j < end; j += stride) {
+                const uint32_t x = a.col[j];
+                if (x == v) continue;
+                const uint32_t ox = a.comm[x];
+                const uint32_t nx = a.target[x];
+                const int d = (int)(nx == c) - (int)(ox == own);
+                if (d) {
+                    const int64_t w = WEIGHTED ? (int64_t)a.wt[j] : 1;
+                    dsig += w * d * (ox != nx ? 1 : 2);
+                }
+                if (nx != c) a.mark[x] = 1;
+            }
+        }
+    }
+    block_add_stats(a.stats, ST_DSIGMA, (uint64_t)dsig, -1, 0, -1, 0);
+}
+
+// ----------------------------------------------------------- finalize pass
+
+__global__ __launch_bounds__(kBlock) void k_finalize(uint32_t *comm, const uint32_t *target,
+                                                     const uint32_t *tot, uint8_t *mark,
+                                                     const uint8_t *bucket, uint32_t n,
+                                                     ListPtrs next, uint32_t *next_counts,
+                                                     unsigned long long *stats) {
+    uint64_t moved = 0, sq = 0;
+    const uint32_t ntiles = (n + kBlock - 1) / kBlock;
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t v = t * kBlock + threadIdx.x;
+        bool sel = false;
+        int b = 0;
+        if (v < n) {
+            const uint32_t tv = target[v];
+            if (tv != comm[v]) { comm[v] = tv; ++moved; }
+            sq += square_u64(tot[v]);
+            if (mark[v]) {
+                mark[v] = 0;
+                sel = true;
+                b = bucket[v];
+            }
+        }
+        append_to_bucket(sel, v, b, next, next_counts);
+    }
+    block_add_stats(stats, ST_MOVED, moved, ST_SUMSQ, sq, -1, 0);
+}
+
+// ------------------------------------------------------------- aggregation
+
+__global__ void k_renumber_flags(const uint32_t *tot, uint32_t n, uint32_t *flags) {
+    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+         c += (size_t)gridDim.x * blockDim.x)
+        flags[c] = count_not_zero(tot[c]);
+}
+
+__global__ void k_update_best(uint32_t *best, uint32_t n0, const uint32_t *comm,
+                              const uint32_t *map) {
+    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n0;
+         v += (size_t)gridDim.x * blockDim.x)
+        best[v] = renumbered(map, comm[best[v]]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_relabel(LevelView g, const uint32_t *comm,
+                                                    const uint32_t *map, uint32_t n_new,
+                                                    uint64_t *keys, uint32_t *vals) {
+    const uint64_t nchunks = (g.E + kChunk - 1) / kChunk;
+    for (uint64_t ch = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; ch < nchunks;
+         ch += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t j0 = ch * kChunk;
+        const uint64_t j1 = j0 + kChunk < g.E ? j0 + kChunk : g.E;
+        uint32_t v = src_of_entry(g.row, g.n, j0);
+        uint64_t vend = g.row[v + 1];
+        uint64_t cs = renumbered(map, comm[v]);
+        for (uint64_t j = j0; j < j1; ++j) {
+            while (j >= vend) { ++v; vend = g.row[v + 1]; cs = renumbered(map, comm[v]); }
+            keys[j] = cs * (uint64_t)n_new + renumbered(map, comm[g.col[j]]);
+            vals[j] = g.wt ? g.wt[j] : 1u;
+        }
+    }
+}
+
+__global__ void k_run_heads(const uint64_t *keys, size_t n, uint32_t *heads) {
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < n;
+         j += (size_t)gridDim.x * blockDim.x)
+        heads[j] = (j == 0 || keys[j] != keys[j - 1]) ? 1u : 0u;
+}
+
+// idx = inclusive scan of heads (1-based output index of each entry's run).
+__global__ __launch_bounds__(kBlock) void k_reduce_runs(const uint64_t *keys, const uint32_t *vals,
+                                                        const uint32_t *idx, size_t n,
+                                                        uint32_t *out_col, uint32_t *out_w,
+                                                        uint64_t *out_row, uint32_t n_new) {
+    const uint64_t nchunks = (n + kChunk - 1) / kChunk;
+    for (uint64_t ch = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; ch < nchunks;
+         ch += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t j0 = ch * kChunk;
+        const uint64_t j1 = j0 + kChunk < n ? j0 + kChunk : n;
+        uint32_t cur = idx[j0];
+        uint32_t acc = 0;
+        for (uint64_t j = j0; j < j1; ++j) {
+            const uint32_t o = idx[j];
+            if (o != cur) {
+                atomicAdd(&out_w[cur - 1], acc);
+                acc = 0;
+                cur = o;
+            }
+            acc += vals[j];
+            if (j == 0 || keys[j] != keys[j - 1]) {
+                const uint64_t key = keys[j];
+                const uint32_t src = (uint32_t)(key / n_new);
+                out_col[o - 1] = (uint32_t)(key % n_new);
+                if (j == 0 || (uint32_t)(keys[j - 1] / n_new) != src) out_row[src] = o - 1;
+            }
+        }
+        atomicAdd(&out_w[cur - 1], acc);
+    }
+}
+
+__global__ void k_new_degrees(const uint32_t *tot, const uint32_t *map, uint32_t n,
+                              uint32_t *k_new) {
+    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+         c += (size_t)gridDim.x * blockDim.x)
+        if (tot[c] > 0) k_new[renumbered(map, (uint32_t)c)] = tot[c];
+}
+
+}  // namespace
+
+// ====================================================================== host
+
+void fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, p, v, n);
+}
+void fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n, kBlock * 16)), dim3(kBlock), 0, s, p, v, n);
+}
+void fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, p, v, n);
+}
+void iota_u32(uint32_t *p, size_t n, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_iota_u32, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, p, n);
+}
+
+void scatter_u32(uint32_t *dst, const uint32_t *idx, const uint32_t *val, size_t n, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_scatter_u32, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, dst, idx,
+                       val, n);
+}
+
+void community_totals(const uint32_t *comm, const uint32_t *kdeg, uint32_t n, uint32_t *tot,
+                      hipStream_t s) {
+    fill_u32(tot, 0u, n, s);
+    if (!n) return;
+    hipLaunchKernelGGL(k_community_totals, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, comm,
+                       kdeg, n, tot);
+}
+
+size_t scan_scratch_elems(size_t n) {
+    size_t total = 0;
+    size_t m = n;
+    while (m > (size_t)kScanTile) {
+        m = (m + kScanTile - 1) / kScanTile;
+        total += m;
+    }
+    return total + 1;
+}
+
+void scan_u32(const uint32_t *in, uint32_t *out, size_t n, bool inclusive, uint32_t *scratch,
+              hipStream_t s) {
+    scan_impl<uint32_t>(in, out, n, inclusive, scratch, s);
+}
+void scan_u64(const uint64_t *in, uint64_t *out, size_t n, bool inclusive, uint64_t *scratch,
+              hipStream_t s) {
+    scan_impl<uint64_t>(in, out, n, inclusive, scratch, s);
+}
+
+size_t radix_hist_elems(size_t n) {
+    const size_t nb = (n + kRsTile - 1) / kRsTile;
+    return 256 * (nb ? nb : 1);
+}
+
+void radix_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, size_t n, int bits,
+                      uint32_t *hist, uint32_t *scan_scratch, hipStream_t s) {
+    if (n < 2 || bits <= 0) return;
+    const uint32_t nb = (uint32_t)((n + kRsTile - 1) / kRsTile);
+    const size_t hn = (size_t)256 * nb;
+    uint64_t *ki = k0, *ko = k1;
+    uint32_t *vi = v0, *vo = v1;
+    int passes = 0;
+    for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(k_rs_upsweep, dim3(nb), dim3(kBlock), 0, s, (const uint64_t *)ki, n,
+                           shift, hist, nb);
+        scan_u32(hist, hist, hn, false, scan_scratch, s);
+        hipLaunchKernelGGL(k_rs_downsweep, dim3(nb), dim3(kBlock), 0, s, (const uint64_t *)ki,
+                           (const uint32_t *)vi, ko, vo, n, shift, (const uint32_t *)hist, nb);
+        uint64_t *tk = ki; ki = ko; ko = tk;
+        uint32_t *tv = vi; vi = vo; vo = tv;
+        ++passes;
+    }
+    if (passes & 1) {
+        (void)hipMemcpyAsync(k0, ki, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s);
+        (void)hipMemcpyAsync(v0, vi, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+    }
+}
+
+void bucket_ids(const uint64_t *row, uint32_t n, uint8_t *bucket, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_bucket_ids, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, row, n,
+                       bucket);
+}
+
+void build_lists_all(const uint8_t *bucket, uint32_t n, uint32_t *const *lists, uint32_t *counts,
+                     hipStream_t s) {
+    ListPtrs lp;
+    for (int b = 0; b < kNumBuckets; ++b) lp.p[b] = lists[b];
+    fill_u32(counts, 0u, kNumBuckets, s);
+    if (!n) return;
+    hipLaunchKernelGGL(k_build_lists_all, dim3(grid_for(n, kBlock * 4, 16384u)), dim3(kBlock), 0,
+                       s, bucket, n, lp, counts);
+}
+
+void modularity_parts(const LevelView &g, const uint32_t *comm, const uint32_t *tot,
+                      unsigned long long *stats, hipStream_t s) {
+    const uint64_t work = (g.E + kChunk - 1) / kChunk > g.n ? (g.E + kChunk - 1) / kChunk : g.n;
+    hipLaunchKernelGGL(k_modularity_parts, dim3(grid_for(work, kBlock, 8192u)), dim3(kBlock), 0, s,
+                       g, comm, tot, stats);
+}
+
+namespace {
+
+template <int G, int T>
+void launch_move_table(const MoveCtx &m, bool weighted, int gain, const uint32_t *list,
+                       uint32_t count, hipStream_t s) {
+    constexpr int GPB = kBlock / G;
+    const dim3 grid((count + GPB - 1) / GPB), blk(kBlock);
+    if (gain == 0) {
+        if (weighted) hipLaunchKernelGGL((k_move_table<G, T, 0, true>), grid, blk, 0, s, m, list, count);
+        else hipLaunchKernelGGL((k_move_table<G, T, 0, false>), grid, blk, 0, s, m, list, count);
+    } else {
+        if (weighted) hipLaunchKernelGGL((k_move_table<G, T, 1, true>), grid, blk, 0, s, m, list, count);
+        else hipLaunchKernelGGL((k_move_table<G, T, 1, false>), grid, blk, 0, s, m, list, count);
+    }
+}
+
+template <int G>
+void launch_move_fast(const MoveCtx &m, bool weighted, int gain, const uint32_t *list,
+                      uint32_t count, hipStream_t s) {
+    constexpr int GPB = kBlock / G;
+    const dim3 grid((count + GPB - 1) / GPB), blk(kBlock);
+    if (gain == 0) {
+        if (weighted) hipLaunchKernelGGL((k_move_fast<G, 0, true>), grid, blk, 0, s, m, list, count);
+        else hipLaunchKernelGGL((k_move_fast<G, 0, false>), grid, blk, 0, s, m, list, count);
+    } else {
+        if (weighted) hipLaunchKernelGGL((k_move_fast<G, 1, true>), grid, blk, 0, s, m, list, count);
+        else hipLaunchKernelGGL((k_move_fast<G, 1, false>), grid, blk, 0, s, m, list, count);
+    }
+}
+
+}  // namespace
+
+void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
+               const uint32_t *counts, hipStream_t s) {
+    MoveCtx m;
+    m.row = a.g.row; m.col = a.g.col; m.wt = a.g.wt; m.kdeg = a.g.kdeg;
+    m.comm = a.comm; m.tot = a.tot; m.target = a.target; m.stats = a.stats;
+    m.m2 = a.m2; m.tw = a.tw; m.v_end = a.v_end; m.highest = a.highest;
+    const bool wtd = a.g.wt != nullptr;
+    const bool fast = a.kind == 0;
+    for (int b = 0; b < kNumBuckets; ++b) {
+        const uint32_t cnt = counts[b];
+        if (!cnt) continue;
+        const uint32_t *L = lists[b];
+        if (fast) {
+            switch (b) {
+                case 0: launch_move_fast<4>(m, wtd, a.gain, L, cnt, s); break;
+                case 1: launch_move_fast<8>(m, wtd, a.gain, L, cnt, s); break;
+                case 2: launch_move_fast<16>(m, wtd, a.gain, L, cnt, s); break;
+                case 3: launch_move_fast<32>(m, wtd, a.gain, L, cnt, s); break;
+                case 4: launch_move_fast<64>(m, wtd, a.gain, L, cnt, s); break;
+                case 5: launch_move_fast<64>(m, wtd, a.gain, L, cnt, s); break;
+                case 6: launch_move_fast<256>(m, wtd, a.gain, L, cnt, s); break;
+                case 7: launch_move_fast<256>(m, wtd, a.gain, L, cnt, s); break;
+                default: {
+                    HubCtx h{m, hub};
+                    const dim3 grid(cnt * kHubParts), blk(kBlock);
+                    if (a.gain == 0) {
+                        if (wtd) hipLaunchKernelGGL((k_hub_fast<0, true>), grid, blk, 0, s, h, L, cnt);
+                        else hipLaunchKernelGGL((k_hub_fast<0, false>), grid, blk, 0, s, h, L, cnt);
+                        hipLaunchKernelGGL(k_hub_decide<0>, dim3(cnt), dim3(64), 0, s, h, L, cnt, 1);
+                    } else {
+                        if (wtd) hipLaunchKernelGGL((k_hub_fast<1, true>), grid, blk, 0, s, h, L, cnt);
+                        else hipLaunchKernelGGL((k_hub_fast<1, false>), grid, blk, 0, s, h, L, cnt);
+                        hipLaunchKernelGGL(k_hub_decide<1>, dim3(cnt), dim3(64), 0, s, h, L, cnt, 1);
+                    }
+                    // stats: considered hubs counted host-side from the list size
+                    break;
+                }
+            }
+        } else {
+            switch (b) {
+                case 0: launch_move_table<4, 8>(m, wtd, a.gain, L, cnt, s); break;
+                case 1: launch_move_table<8, 16>(m, wtd, a.gain, L, cnt, s); break;
+                case 2: launch_move_table<16, 32>(m, wtd, a.gain, L, cnt, s); break;
+                case 3: launch_move_table<32, 64>(m, wtd, a.gain, L, cnt, s); break;
+                case 4: launch_move_table<64, 128>(m, wtd, a.gain, L, cnt, s); break;
+                case 5: launch_move_table<64, 512>(m, wtd, a.gain, L, cnt, s); break;
+                case 6: launch_move_table<256, 2048>(m, wtd, a.gain, L, cnt, s); break;
+                case 7: launch_move_table<256, 8192>(m, wtd, a.gain, L, cnt, s); break;
+                default: {
+                    HubCtx h{m, hub};
+                    const dim3 grid(cnt * kHubParts), blk(kBlock);
+                    if (wtd) hipLaunchKernelGGL(k_hub_insert<true>, grid, blk, 0, s, h, L, cnt);
+                    else hipLaunchKernelGGL(k_hub_insert<false>, grid, blk, 0, s, h, L, cnt);
+                    hipLaunchKernelGGL(k_hub_self, dim3((cnt + 63) / 64), dim3(64), 0, s, h, L, cnt);
+                    if (a.gain == 0) {
+                        hipLaunchKernelGGL(k_hub_scan<0>, grid, blk, 0, s, h, L, cnt);
+                        hipLaunchKernelGGL(k_hub_decide<0>, dim3(cnt), dim3(64), 0, s, h, L, cnt, 0);
+                    } else {
+                        hipLaunchKernelGGL(k_hub_scan<1>, grid, blk, 0, s, h, L, cnt);
+                        hipLaunchKernelGGL(k_hub_decide<1>, dim3(cnt), dim3(64), 0, s, h, L, cnt, 0);
+                    }
+                    break;
+                }
+            }
+        }
+    }
+}
+
+void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *counts,
+                 hipStream_t s) {
+    CommitCtx c;
+    c.row = a.g.row; c.col = a.g.col; c.wt = a.g.wt; c.kdeg = a.g.kdeg;
+    c.comm = a.comm; c.target = a.target; c.tot = a.tot; c.mark = a.mark; c.stats = a.stats;
+    const bool wtd = a.g.wt != nullptr;
+    for (int b = 0; b < kNumBuckets; ++b) {
+        const uint32_t cnt = counts[b];
+        if (!cnt) continue;
+        const uint32_t *L = lists[b];
+        const dim3 blk(kBlock);
+#define LV_COMMIT(GG, PARTS)                                                                   \
+    do {                                                                                       \
+        const uint32_t gpb = kBlock / (GG);                                                    \
+        const dim3 grid((GG) == kBlock ? cnt * (PARTS) : (cnt + gpb - 1) / gpb);               \
+        if (wtd) hipLaunchKernelGGL((k_commit<GG, true>), grid, blk, 0, s, c, L, cnt, (uint32_t)(PARTS)); \
+        else hipLaunchKernelGGL((k_commit<GG, false>), grid, blk, 0, s, c, L, cnt, (uint32_t)(PARTS)); \
+    } while (0)
+        switch (b) {
+            case 0: LV_COMMIT(4, 1); break;
+            case 1: LV_COMMIT(8, 1); break;
+            case 2: LV_COMMIT(16, 1); break;
+            case 3: LV_COMMIT(32, 1); break;
+            case 4: LV_COMMIT(64, 1); break;
+            case 5: LV_COMMIT(64, 1); break;
+            case 6: LV_COMMIT(256, 1); break;
+            case 7: LV_COMMIT(256, 4); break;
+            default: LV_COMMIT(256, kHubParts); break;
+        }
+#undef LV_COMMIT
+    }
+}
+
+void finalize_pass(const SweepArgs &a, uint32_t *const *next_lists, uint32_t *next_counts,
+                   hipStream_t s) {
+    ListPtrs lp;
+    for (int b = 0; b < kNumBuckets; ++b) lp.p[b] = next_lists[b];
+    fill_u32(next_counts, 0u, kNumBuckets, s);
+    const uint32_t n = a.g.n;
+    if (!n) return;
+    hipLaunchKernelGGL(k_finalize, dim3(grid_for(n, kBlock * 4, 16384u)), dim3(kBlock), 0, s,
+                       const_cast<uint32_t *>(a.comm), (const uint32_t *)a.target,
+                       (const uint32_t *)a.tot, a.mark, a.bucket, n, lp, next_counts, a.stats);
+}
+
+void renumber_flags(const uint32_t *tot, uint32_t n, uint32_t *flags, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_renumber_flags, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, tot, n,
+                       flags);
+}
+
+void update_best(uint32_t *best, uint32_t n0, const uint32_t *comm, const uint32_t *map,
+                 hipStream_t s) {
+    if (!n0) return;
+    hipLaunchKernelGGL(k_update_best, dim3(grid_for(n0, kBlock * 4)), dim3(kBlock), 0, s, best, n0,
+                       comm, map);
+}
+
+void relabel_entries(const LevelView &g, const uint32_t *comm, const uint32_t *map, uint32_t n_new,
+                     uint64_t *keys, uint32_t *vals, hipStream_t s) {
+    if (!g.E) return;
+    const uint64_t nchunks = (g.E + kChunk - 1) / kChunk;
+    hipLaunchKernelGGL(k_relabel, dim3(grid_for(nchunks, kBlock, 16384u)), dim3(kBlock), 0, s, g,
+                       comm, map, n_new, keys, vals);
+}
+
+void run_heads(const uint64_t *keys, size_t n, uint32_t *heads, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_run_heads, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, keys, n,
+                       heads);
+}
+
+void reduce_runs(const uint64_t *keys, const uint32_t *vals, const uint32_t *idx, size_t n,
+                 uint32_t *out_col, uint32_t *out_w, uint64_t *out_row, uint32_t n_new,
+                 hipStream_t s) {
+    if (!n) return;
+    const uint64_t nchunks = (n + kChunk - 1) / kChunk;
+    hipLaunchKernelGGL(k_reduce_runs, dim3(grid_for(nchunks, kBlock, 16384u)), dim3(kBlock), 0, s,
+                       keys, vals, idx, n, out_col, out_w, out_row, n_new);
+}
+
+void new_degrees(const uint32_t *tot, const uint32_t *map, uint32_t n, uint32_t *k_new,
+                 hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_new_degrees, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, tot, map,
+                       n, k_new);
+}
+
+}  // namespace lv

@@ -1,0 +1,215 @@
+"""GPU parity: the HIP library (through the C ABI) against the CPU oracle.
+
+Bar: bit-exact per-vertex best moves, partitions (FNV hash per sweep), exact
+modularity parts (sigma_in, sum of tot^2), renumbered ids and contracted
+graphs; modularity (f64 combine of exact integers) identical.
+"""
+import itertools
+import sys
+import zlib
+
+import numpy as np
+import pytest
+
+import louvain_amd as L
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+MODES = [("hash", L.HASH, O.HASH), ("sort", L.SORT, O.SORT),
+         ("adaptive_speed", L.ADAPTIVE_SPEED, O.ADAPTIVE_SPEED),
+         ("adaptive_memory", L.ADAPTIVE_MEMORY, O.ADAPTIVE_MEMORY)]
+GAINS = [("exact", L.GAIN_EXACT, O.GAIN_EXACT), ("f32", L.GAIN_REF_F32, O.GAIN_REF_F32)]
+TIES = [("lowest", L.TIE_LOWEST, O.TIE_LOWEST), ("reference", L.TIE_REFERENCE, O.TIE_REFERENCE)]
+REC_KEYS = ["level", "sweep", "kind", "n", "n_active", "n_moved", "n_considered", "E",
+            "e_active", "sigma_in", "sum_tot_sq", "q", "comm_hash"]
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if L.device_count() < 1:
+        pytest.skip("no GPU visible")
+
+
+def compare_runs(a, b, label):
+    sa = [{k: s[k] for k in REC_KEYS} for s in a["sweeps"]]
+    sb = [{k: s[k] for k in REC_KEYS} for s in b["sweeps"]]
+    for i, (x, y) in enumerate(zip(sa, sb)):
+        assert x == y, f"{label}: sweep record {i} differs\nGPU   {x}\noracle {y}"
+    assert len(sa) == len(sb), f"{label}: {len(sa)} vs {len(sb)} sweeps"
+    assert a["modularity"] == b["modularity"], label
+    assert a["n_comm"] == b["n_comm"], label
+    assert np.array_equal(a["best"], b["best"]), label
+
+
+def run_both(edges, gprm, oprm, w=None):
+    ea, eb = edges
+    g = L.Graph.from_edges(ea, eb, w)
+    og = O.Graph.from_edges(ea, eb, w)
+    return g.louvain(gprm, hash_trace=True), og.louvain(oprm)
+
+
+@pytest.mark.parametrize("mode,gain,tie", list(itertools.product(MODES, GAINS, TIES)),
+                         ids=lambda x: x[0])
+def test_fixture_full_run(fixture_path, mode, gain, tie):
+    for thr in (1e-3, 1e-6):
+        label = f"fixture {mode[0]} {gain[0]} {tie[0]} thr={thr}"
+        say(label)
+        g = L.Graph.load_edge(fixture_path)
+        og = O.Graph.load_edge(fixture_path)
+        a = g.louvain(L.params(mode=mode[1], gain=gain[1], tie=tie[1], sweep_threshold=thr,
+                               level_threshold=thr), hash_trace=True)
+        b = og.louvain(O.params(mode=mode[2], gain=gain[2], tie=tie[2], sweep_threshold=thr,
+                                level_threshold=thr))
+        compare_runs(a, b, label)
+
+
+def _random_graph(rng, n, m, hubs=0, hub_deg=0, selfloops=0):
+    a = rng.integers(0, n, m)
+    b = rng.integers(0, n, m)
+    ea, eb = [a], [b]
+    for h in range(hubs):
+        nb = rng.choice(n, size=min(hub_deg, n - 1), replace=False)
+        ea.append(np.full(len(nb), h)); eb.append(nb)
+    if selfloops:
+        s = rng.integers(0, n, selfloops)
+        ea.append(s); eb.append(s)
+    # ring keeps ids dense
+    ea.append(np.arange(n)); eb.append((np.arange(n) + 1) % n)
+    return np.concatenate(ea), np.concatenate(eb)
+
+
+GRAPHS = {
+    "er_small": dict(n=300, m=900),
+    "er_selfloops": dict(n=500, m=2000, selfloops=40),
+    "hubs": dict(n=20000, m=40000, hubs=3, hub_deg=9000),
+    "mid": dict(n=5000, m=60000),
+}
+
+
+@pytest.mark.parametrize("name", list(GRAPHS))
+@pytest.mark.parametrize("weighted", [False, True])
+def test_random_graph_full_run(name, weighted):
+    rng = np.random.default_rng(zlib.crc32(name.encode()) + weighted)
+    ea, eb = _random_graph(rng, **GRAPHS[name])
+    w = rng.integers(1, 11, len(ea)).astype(np.uint32) if weighted else None
+    for (mn, gm, om), (gn, gg, og_) in itertools.product(MODES[:2], GAINS):
+        label = f"{name} w={weighted} {mn} {gn}"
+        say(label)
+        a, b = run_both((ea, eb), L.params(mode=gm, gain=gg, sweep_threshold=1e-6,
+                                           level_threshold=1e-6),
+                        O.params(mode=om, gain=og_, sweep_threshold=1e-6,
+                                 level_threshold=1e-6), w)
+        compare_runs(a, b, label)
+
+
+@pytest.mark.parametrize("kind", [L.SWEEP_SORT, L.SWEEP_HASH])
+def test_best_moves_snapshots(kind):
+    rng = np.random.default_rng(7 + kind)
+    ea, eb = _random_graph(rng, n=12000, m=50000, hubs=2, hub_deg=6000, selfloops=10)
+    w = rng.integers(1, 6, len(ea)).astype(np.uint32)
+    g = L.Graph.from_edges(ea, eb, w)
+    og = O.Graph.from_edges(ea, eb, w)
+    n = g.n
+    c = L.Community(g)
+    for trial in range(6):
+        ncomm = [n, 5000, 300, 20, 3, 1][trial]
+        comm = rng.integers(0, ncomm, n).astype(np.uint32)
+        active = (rng.random(n) < [1.0, 0.5, 0.9, 0.1, 1.0, 1.0][trial]).astype(np.uint8)
+        c.set_partition(comm)
+        for (gn, gg, og_), (tn, gt, ot) in itertools.product(GAINS, TIES):
+            say(f"snapshot kind={kind} trial={trial} {gn} {tn}")
+            tg, ea_g = c.best_moves(kind, L.params(gain=gg, tie=gt), active)
+            to, moved, ea_o, _ = og.best_moves(comm, active, kind, O.params(gain=og_, tie=ot))
+            assert np.array_equal(tg, to), f"trial {trial} {gn} {tn}: " \
+                f"{np.flatnonzero(tg != to)[:10]}"
+            assert ea_g == ea_o
+        q, si, s = c.modularity_parts()
+        qo, sio, so = og.modularity(comm)
+        assert (si, s) == (sio, so) and q == qo
+
+
+def test_fast_sweep_identity():
+    rng = np.random.default_rng(11)
+    ea, eb = _random_graph(rng, n=15000, m=60000, hubs=2, hub_deg=7000)
+    g = L.Graph.from_edges(ea, eb)
+    og = O.Graph.from_edges(ea, eb)
+    c = L.Community(g)
+    n = g.n
+    ident = np.arange(n, dtype=np.uint32)
+    for gn, gg, og_ in GAINS:
+        say(f"fast identity {gn}")
+        tg, _ = c.best_moves(L.SWEEP_FAST, L.params(gain=gg))
+        to, _, _, _ = og.best_moves(ident, np.ones(n, np.uint8), O.SWEEP_FAST, O.params(gain=og_))
+        assert np.array_equal(tg, to)
+
+
+def test_aggregation_matches_oracle():
+    rng = np.random.default_rng(5)
+    ea, eb = _random_graph(rng, n=8000, m=40000, hubs=1, hub_deg=5000, selfloops=20)
+    w = rng.integers(1, 9, len(ea)).astype(np.uint32)
+    g = L.Graph.from_edges(ea, eb, w)
+    og = O.Graph.from_edges(ea, eb, w)
+    n = g.n
+    for ncomm in (n, 3000, 100, 1):
+        say(f"aggregate ncomm={ncomm}")
+        comm = rng.integers(0, ncomm, n).astype(np.uint32)
+        c = L.Community(g)
+        c.set_partition(comm)
+        c.aggregation_phase(True)
+        new_id, n_new, h = og.aggregate(comm)
+        row, col, ww, k = c.graph_csr()
+        r2, c2, w2, k2 = h.arrays()
+        assert len(row) - 1 == n_new
+        assert np.array_equal(row, r2) and np.array_equal(col, c2) and np.array_equal(ww, w2)
+        assert np.array_equal(k.astype(np.uint64), k2)
+        best, nb = c.best_communities(n)
+        assert nb == n_new
+        assert np.array_equal(best, new_id[comm])
+        # Q is preserved by contraction
+        q_new, _, _ = c.modularity_parts()
+        assert q_new == og.modularity(comm)[0]
+        c.close()
+
+
+def test_rmat_scale16_parity():
+    ea, eb, n = L.gen_rmat(16, seed=2)
+    for (mn, gm, om) in MODES[:2]:
+        say(f"rmat16 {mn}")
+        a, b = run_both((ea, eb), L.params(mode=gm, sweep_threshold=1e-6, level_threshold=1e-6),
+                        O.params(mode=om, sweep_threshold=1e-6, level_threshold=1e-6))
+        compare_runs(a, b, f"rmat16 {mn}")
+
+
+def test_chunk_emulation_parity(fixture_path):
+    for mode in (MODES[0], MODES[1]):
+        say(f"chunk emulation {mode[0]}")
+        g = L.Graph.load_edge(fixture_path)
+        og = O.Graph.load_edge(fixture_path)
+        a = g.louvain(L.params(mode=mode[1], chunk_emulation=1, chunk_entries=3000),
+                      hash_trace=True)
+        b = og.louvain(O.params(mode=mode[2], chunk_emulation=1, chunk_entries=3000))
+        compare_runs(a, b, f"chunk {mode[0]}")
+
+
+def test_phase_api_matches_louvain(fixture_path):
+    """OptimizationPhase / AggregationPhase driven one call at a time."""
+    g = L.Graph.load_edge(fixture_path)
+    ref = g.louvain(L.params())
+    c = L.Community(g)
+    prm = L.params()
+    while True:
+        old = c.modularity
+        c.optimization_phase(prm)
+        c.refine_phase()
+        cont = c.modularity - old > prm.level_threshold
+        c.aggregation_phase(cont, prm)
+        if not cont:
+            break
+    best, nc = c.best_communities(g.n)
+    assert np.array_equal(best, ref["best"]) and nc == ref["n_comm"]
+    assert c.modularity == ref["modularity"]
