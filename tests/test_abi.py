@@ -1,0 +1,134 @@
+"""CPU tests of the C ABI library: it loads, exports every declared entry
+point, and its host-side parts (.edge parsing, CSR construction, generators,
+error mapping) agree with the oracle.  No kernel is launched here."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import louvain_amd as L
+from oracle import oracle as O
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    txt = open(os.path.join(REPO, "include", "lv_api.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = set(re.findall(r"\b(lv_[a-z0-9_]+)\s*\(", txt))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 30
+    for name in names:
+        assert hasattr(L.lib, name), name
+    assert set(names) == set(L.SIGNATURES), set(names) ^ set(L.SIGNATURES)
+
+
+def test_default_params_mirror_reference_constants():
+    p = L.params()
+    assert (p.mode, p.gain, p.tie, p.chunk_emulation) == (L.HASH, L.GAIN_EXACT, L.TIE_LOWEST, 0)
+    assert p.chunk_entries == 50_000_000          # STEP_ROUND (constants.h:18)
+    assert p.sweep_threshold == p.level_threshold == 0.001   # constants.h:15
+    assert p.max_sweeps == 1_000_000_000          # EARLY_STOP_LIMIT (constants.h:16)
+
+
+def csr_equal(g, og):
+    row, col, w = g.csr()
+    r2, c2, w2, _ = og.arrays()
+    return np.array_equal(row, r2) and np.array_equal(col, c2) and np.array_equal(w, w2)
+
+
+def test_fixture_csr_matches_oracle(fixture_path):
+    g = L.Graph.load_edge(fixture_path)
+    og = O.Graph.load_edge(fixture_path)
+    assert g.info() == {"n": 10000, "E": 39992, "total_weight": 19996.0, "min_id": 0}
+    assert csr_equal(g, og)
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_random_edge_lists_match_oracle(seed):
+    rng = np.random.default_rng(seed)
+    n = 300
+    m = 2000
+    off = int(rng.integers(-50, 50))
+    a = np.concatenate([rng.integers(0, n, m), np.arange(n), rng.integers(0, n, 7)]) + off
+    b = np.concatenate([rng.integers(0, n, m), (np.arange(n) + 1) % n,
+                        rng.integers(0, n, 7)]) + off
+    b[-3:] = a[-3:]                                   # self-loops
+    w = rng.integers(1, 20, len(a)).astype(np.uint32) if seed % 2 else None
+    g = L.Graph.from_edges(a, b, w)
+    og = O.Graph.from_edges(a, b, w)
+    assert csr_equal(g, og)
+    i = g.info()
+    assert i["min_id"] == og.min_id and i["total_weight"] == og.total_weight
+
+
+def test_weighted_file_and_errors(tmp_path):
+    p = tmp_path / "w.edge"
+    p.write_text("c comment\n0 1 4\n1 2 3\n1 0 9\n2 2 5\n")
+    g = L.Graph.load_edge(str(p), weighted=True, skip=1)
+    og = O.Graph.load_edge(str(p), weighted=True, skip=1)
+    assert csr_equal(g, og)
+    with pytest.raises(L.LouvainError) as e:
+        L.Graph.from_edges([0, 5], [1, 6])
+    assert e.value.status == 5                        # LV_ERR_NOT_DENSE
+    empty = tmp_path / "e.edge"
+    empty.write_text("nothing here\n")
+    with pytest.raises(L.LouvainError) as e:
+        L.Graph.load_edge(str(empty))
+    assert e.value.status == 3                        # LV_ERR_PARSE
+    with pytest.raises(L.LouvainError) as e:
+        L.Graph.load_edge(str(tmp_path / "missing.edge"))
+    assert e.value.status == 4                        # LV_ERR_IO
+
+
+def test_community_without_device_reports_no_device(fixture_path):
+    if L.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    g = L.Graph.load_edge(fixture_path)
+    with pytest.raises(L.LouvainError) as e:
+        L.Community(g)
+    assert e.value.status == 8                        # LV_ERR_NO_DEVICE
+    with pytest.raises(L.LouvainError) as e:
+        g.louvain()
+    assert e.value.status == 8
+
+
+def test_rmat_generator_properties():
+    ea, eb, n = L.gen_rmat(12, seed=2)
+    ea2, eb2, n2 = L.gen_rmat(12, seed=2)
+    assert np.array_equal(ea, ea2) and np.array_equal(eb, eb2) and n == n2
+    assert np.all(ea < eb)                            # normalised, no self-loops
+    key = ea * (1 << 32) + eb
+    assert np.all(np.diff(key) > 0)                   # sorted, no duplicates
+    used = np.zeros(n, bool); used[ea] = True; used[eb] = True
+    assert used.all() and ea.max() < n and eb.max() < n   # dense compacted ids
+    ea3, _, _ = L.gen_rmat(12, seed=3)
+    assert not np.array_equal(ea, ea3[:len(ea)]) or len(ea) != len(ea3)
+
+
+def test_ba_generator_properties():
+    n, m = 2000, 3
+    ea, eb = L.gen_ba(n, m, seed=1)
+    assert len(ea) == (n - m) * m
+    pairs = set(zip(np.minimum(ea, eb).tolist(), np.maximum(ea, eb).tolist()))
+    assert len(pairs) == len(ea)                      # simple graph
+    deg = np.bincount(np.concatenate([ea, eb]), minlength=n)
+    assert deg[m:].min() >= m and deg.max() > 10 * m   # preferential attachment hubs
+
+
+def test_planted_generator_properties():
+    ea, eb, ew, planted = L.gen_planted(n=20000, n_comm=40, size_lo=50, size_hi=2000,
+                                        max_deg=500, seed=3)
+    assert ew.min() >= 1 and ew.max() <= 10
+    n = int(max(ea.max(), eb.max())) + 1
+    assert len(planted) == n
+    og = O.Graph.from_edges(ea, eb, ew)
+    q, _, _ = og.modularity(planted)
+    inside = np.mean(planted[ea] == planted[eb])
+    assert 0.55 < inside < 0.85                       # mixing mu = 0.3
+    assert q > 0.5
