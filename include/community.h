@@ -53,6 +53,8 @@ struct Community {
     void update_best(const unsigned int *map);
     // identity partition on the current graph, totals = degrees (community.h:57-65)
     void update();
+    // the same after a contraction, with the modularity parts carried over
+    void update(int64_t sigma_in_carry, uint64_t sum_tot_sq_carry);
     // from-scratch modularity of the current partition (community.h:68-106)
     void compute_modularity();
 
@@ -64,6 +66,7 @@ struct Community {
 
   private:
     void init(const GraphHost &g, hipStream_t stream);
+    void update_level(bool carry, int64_t sigma_in_carry, uint64_t sum_tot_sq_carry);
     bool own_stream_ = false;
 };
 

@@ -13,6 +13,8 @@
 
 struct AggregationPhase {
     static void run(Community &c, bool aggregate, MODE mode);
+    // contract the current level with the renumbering map (device, 1-based)
+    static void contract(Community &c, const uint32_t *map, uint32_t n_new);
 };
 
 #endif

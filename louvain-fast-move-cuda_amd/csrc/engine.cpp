@@ -70,8 +70,7 @@ void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
     hub_slot_of.release();
     hub_tab_off.release();
     hub_tab_log2.release();
-    hub_key.release();
-    hub_val.release();
+    hub_slot.release();
     hub_part_s.release();
     hub_part_c.release();
     hub_self.release();
@@ -100,13 +99,11 @@ void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
         fill_u32(hub_slot_of.data(), LV_EMPTY, n, s);
         hub_tab_off.resize(n_hubs);
         hub_tab_log2.resize(n_hubs);
-        hub_key.resize(pool);
-        hub_val.resize(pool);
+        hub_slot.resize(pool);
         hub_part_s.resize((size_t)n_hubs * kHubParts);
         hub_part_c.resize((size_t)n_hubs * kHubParts);
         hub_self.resize(n_hubs);
-        fill_u32(hub_key.data(), LV_EMPTY, pool, s);
-        fill_u32(hub_val.data(), 0u, pool, s);
+        fill_u64(hub_slot.data(), kHubEmpty, pool, s);
         LV_HIP(hipMemcpyAsync(hub_tab_off.data(), toff.data(), n_hubs * sizeof(uint64_t),
                               hipMemcpyHostToDevice, s));
         LV_HIP(hipMemcpyAsync(hub_tab_log2.data(), tlog.data(), n_hubs * sizeof(uint32_t),
@@ -142,8 +139,7 @@ HubTables SweepState::hub_tables() {
     h.slot_of = hub_slot_of.data();
     h.tab_off = hub_tab_off.data();
     h.tab_log2 = hub_tab_log2.data();
-    h.key = hub_key.data();
-    h.val = hub_val.data();
+    h.slot = hub_slot.data();
     h.part_s = hub_part_s.data();
     h.part_c = hub_part_c.data();
     h.self = hub_self.data();
@@ -316,7 +312,13 @@ void Community::update_best(const unsigned int *map) {
     lv::update_best(best_communities.data(), n_original, communities.data(), map, stream());
 }
 
-void Community::update() {
+void Community::update() { update_level(false, 0, 0); }
+
+void Community::update(int64_t sigma_in_carry, uint64_t sum_tot_sq_carry) {
+    update_level(true, sigma_in_carry, sum_tot_sq_carry);
+}
+
+void Community::update_level(bool carry, int64_t sigma_in_carry, uint64_t sum_tot_sq_carry) {
     const uint32_t n = graph.n_nodes;
     hipStream_t s = stream();
     communities.resize(std::max<size_t>(n, 1));
@@ -327,7 +329,23 @@ void Community::update() {
                               n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
     state->setup_level(graph, s);
     iota_u32(state->target.data(), n, s);
-    compute_modularity();
+    if (carry) {
+        sigma_in = sigma_in_carry;
+        sum_tot_sq = sum_tot_sq_carry;
+        modularity = q_from_parts();
+        return;
+    }
+    // identity partition of a level-0 graph (rows sorted by destination):
+    // sigma_in = self-loop weight, sum of tot^2 = sum of k^2
+    fill_u64(state->stats.data(), 0ull, state->stats.size(), s);
+    identity_parts(level_view(graph), state->stats.data(), s);
+    LV_HIP(hipMemcpyAsync(state->host_stats, state->stats.data(),
+                          (size_t)kNumStats * kStatSlots * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, s));
+    LV_HIP(hipStreamSynchronize(s));
+    sigma_in = (int64_t)sum_slots(state->host_stats, ST_DSIGMA);
+    sum_tot_sq = sum_slots(state->host_stats, ST_SUMSQ);
+    modularity = q_from_parts();
 }
 
 double Community::q_from_parts() const {
@@ -551,24 +569,53 @@ uint32_t chunk_vertex_end(Community &c, const lv_params &p, int kind, const uint
 
 // -------------------------------------------------------- AggregationPhase
 
+// Workspace slots of the aggregation phase.
+namespace lv {
+enum AggSlot : size_t {
+    W_MAP, W_SCAN32, W_NEWC, W_VK0, W_VK1, W_MEMBERS, W_VV1, W_HIST, W_HIST_SCAN, W_MSTART, W_P,
+    W_SCAN64, W_CROW, W_CBUCKET, W_CCOUNT, W_CLISTS, W_TCOL, W_TW, W_RCOUNT, W_HOFF, W_HK0, W_HK1,
+    W_HV0, W_HV1, W_HIDX, W_HS, W_UKEY, W_WS, W_WE, W_USTART
+};
+
+static int bits_for(uint64_t x) {   // smallest b with 2^b >= x (x >= 1)
+    int b = 0;
+    while (b < 64 && (1ull << b) < x) ++b;
+    return b;
+}
+}  // namespace lv
+
 void AggregationPhase::run(Community &c, bool aggregate, MODE mode) {
-    (void)mode;  // sort and hash contraction give the same exact graph
+    (void)mode;  // the sort and hash contractions of the reference give the same exact graph
     hipStream_t s = c.stream();
+    Workspace &ws = c.state->ws;
     const uint32_t n = c.graph.n_nodes;
     // community_map = inclusive_scan(tot > 0) (phaseAggregation.cuh:27-43)
-    DeviceArray<uint32_t> map(std::max<size_t>(n, 1));
-    DeviceArray<uint32_t> scratch(scan_scratch_elems(n));
-    renumber_flags(c.communities_weight.data(), n, map.data(), s);
-    scan_u32(map.data(), map.data(), n, true, scratch.data(), s);
-    c.update_best(map.data());                                        // :45
+    uint32_t *map = ws.get<uint32_t>(W_MAP, n);
+    renumber_flags(c.communities_weight.data(), n, map, s);
+    scan_u32(map, map, n, true, ws.get<uint32_t>(W_SCAN32, scan_scratch_elems(n)), s);
+    c.update_best(map);                                               // :45
     uint32_t n_new = 0;
-    if (n) LV_HIP(hipMemcpyAsync(&n_new, map.data() + (n - 1), sizeof(uint32_t),
-                                 hipMemcpyDeviceToHost, s));
+    if (n) LV_HIP(hipMemcpyAsync(&n_new, map + (n - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     LV_HIP(hipStreamSynchronize(s));
     c.n_of_best_communities = n_new;                                  // :46
     if (!aggregate) return;
-    // contraction (phaseAggregation.cuh:51-105): relabel, sort, reduce
+    const int64_t sigma_in = c.sigma_in;
+    const uint64_t sum_tot_sq = c.sum_tot_sq;
+    contract(c, map, n_new);
+    c.level += 1;
+    // community.update() (:174): identity partition on the contracted graph.
+    // Its modularity parts equal the ones just reached (self-loop weight of a
+    // community = its internal weight, new degree = old total), so they carry.
+    c.update(sigma_in, sum_tot_sq);
+}
+
+// Contraction by community hashing (phaseAggregation.cuh:51-170).
+void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_new) {
+    hipStream_t s = c.stream();
+    Workspace &ws = c.state->ws;
+    const uint32_t n = c.graph.n_nodes;
     const uint64_t E = c.graph.n_entries;
+    const LevelView g = level_view(c.graph);
     GraphDevice ng;
     ng.n_nodes = n_new;
     ng.total_weight = c.graph.total_weight;                           // :172
@@ -576,44 +623,108 @@ void AggregationPhase::run(Community &c, bool aggregate, MODE mode) {
     ng.unit_weights = false;
     ng.neighboorhood_sum.resize((size_t)n_new + 1);
     ng.tot_weight_per_nodes.resize(std::max<size_t>(n_new, 1));
-    uint64_t e_new = 0;
+    // 1. members of each new community: vertices sorted by new id
+    uint32_t *newc = ws.get<uint32_t>(W_NEWC, n);
+    uint64_t *vk0 = ws.get<uint64_t>(W_VK0, n), *vk1 = ws.get<uint64_t>(W_VK1, n);
+    uint32_t *members = ws.get<uint32_t>(W_MEMBERS, n), *vv1 = ws.get<uint32_t>(W_VV1, n);
+    agg_newc(c.communities.data(), map, n, newc, vk0, members, s);
+    radix_sort_pairs(vk0, members, vk1, vv1, n, bits_for(n_new),
+                     ws.get<uint32_t>(W_HIST, radix_hist_elems(n)),
+                     ws.get<uint32_t>(W_HIST_SCAN, scan_scratch_elems(radix_hist_elems(n))), s);
+    uint32_t *mstart = ws.get<uint32_t>(W_MSTART, (size_t)n_new + 1);
+    uint64_t *P = ws.get<uint64_t>(W_P, (size_t)n + 1);
+    agg_members(vk0, members, g.row, n, n_new, mstart, P, s);
+    scan_u64(P, P, (size_t)n + 1, false, ws.get<uint64_t>(W_SCAN64, scan_scratch_elems((size_t)n + 1)), s);
+    uint64_t *crow = ws.get<uint64_t>(W_CROW, (size_t)n_new + 1);
+    agg_comm_rows(P, mstart, n_new, crow, s);
+    // 2. bucket communities by their number of virtual entries
+    uint8_t *cb = ws.get<uint8_t>(W_CBUCKET, n_new);
+    uint32_t *cnt_dev = ws.get<uint32_t>(W_CCOUNT, kNumBuckets);
+    bucket_ids(crow, n_new, cb, s);
+    bucket_counts(cb, n_new, cnt_dev, s);
+    uint32_t cnt[kNumBuckets];
+    LV_HIP(hipMemcpyAsync(cnt, cnt_dev, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    LV_HIP(hipStreamSynchronize(s));
+    uint32_t *clists = ws.get<uint32_t>(W_CLISTS, n_new);
+    uint32_t *lp[kNumBuckets];
     {
-        DeviceArray<uint64_t> k0(std::max<size_t>(E, 1)), k1(std::max<size_t>(E, 1));
-        DeviceArray<uint32_t> v0(std::max<size_t>(E, 1)), v1(std::max<size_t>(E, 1));
-        relabel_entries(level_view(c.graph), c.communities.data(), map.data(), n_new, k0.data(),
-                        v0.data(), s);
-        int bits = 0;
-        const uint64_t maxkey = (uint64_t)n_new * (uint64_t)n_new;
-        while (bits < 64 && (1ull << bits) < maxkey) ++bits;
-        DeviceArray<uint32_t> hist(radix_hist_elems(E));
-        DeviceArray<uint32_t> hs(scan_scratch_elems(radix_hist_elems(E)));
-        radix_sort_pairs(k0.data(), v0.data(), k1.data(), v1.data(), E, bits, hist.data(), hs.data(),
-                         s);
-        // run heads -> inclusive scan -> 1-based output index per entry
-        uint32_t *heads = v1.data();
-        run_heads(k0.data(), E, heads, s);
-        DeviceArray<uint32_t> es(scan_scratch_elems(E));
-        scan_u32(heads, heads, E, true, es.data(), s);
-        if (E) LV_HIP(hipMemcpyAsync(&e_new, heads + (E - 1), sizeof(uint32_t),
-                                     hipMemcpyDeviceToHost, s));
-        LV_HIP(hipStreamSynchronize(s));
-        uint32_t e_new32 = (uint32_t)e_new;
-        (void)e_new32;
-        ng.edge_destination.resize(std::max<size_t>(e_new, 1));
-        ng.weights.resize(std::max<size_t>(e_new, 1));
-        fill_u32(ng.weights.data(), 0u, e_new, s);
-        reduce_runs(k0.data(), v0.data(), heads, E, ng.edge_destination.data(), ng.weights.data(),
-                    ng.neighboorhood_sum.data(), n_new, s);
-        LV_HIP(hipMemcpyAsync(ng.neighboorhood_sum.data() + n_new, &e_new, sizeof(uint64_t),
-                              hipMemcpyHostToDevice, s));
-        new_degrees(c.communities_weight.data(), map.data(), n, ng.tot_weight_per_nodes.data(), s);
-        LV_HIP(hipStreamSynchronize(s));
+        uint64_t off = 0;
+        for (int b = 0; b < kNumBuckets; ++b) { lp[b] = clists + off; off += cnt[b]; }
     }
+    build_lists_all(cb, n_new, lp, cnt_dev, s);
+    // 3. sorted distinct rows of the LDS-table communities
+    AggArgs x;
+    x.g = g;
+    x.newc = newc;
+    x.members = members;
+    x.mstart = mstart;
+    x.P = P;
+    x.crow = crow;
+    x.tcol = ws.get<uint32_t>(W_TCOL, E);
+    x.tw = ws.get<uint32_t>(W_TW, E);
+    x.rcount = ws.get<uint32_t>(W_RCOUNT, n_new);
+    agg_tables(x, lp, cnt, s);
+    // 4. giant communities: gather their entries, radix sort, reduce runs
+    const uint32_t nh = cnt[kHubBucket];
+    const uint32_t hub_parts = 64;
+    const int bd = std::max(bits_for(n_new), 1);
+    uint32_t *ustart = nullptr;
+    uint64_t *ukey = nullptr, *wsum0 = nullptr, *wsum1 = nullptr;
+    if (nh) {
+        std::vector<uint32_t> hubs(nh);
+        LV_HIP(hipMemcpyAsync(hubs.data(), lp[kHubBucket], nh * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, s));
+        std::vector<uint64_t> crow_h((size_t)n_new + 1);
+        LV_HIP(hipMemcpyAsync(crow_h.data(), crow, ((size_t)n_new + 1) * sizeof(uint64_t),
+                              hipMemcpyDeviceToHost, s));
+        LV_HIP(hipStreamSynchronize(s));
+        std::vector<uint64_t> hoff_h((size_t)nh + 1, 0);
+        for (uint32_t i = 0; i < nh; ++i)
+            hoff_h[i + 1] = hoff_h[i] + (crow_h[hubs[i] + 1] - crow_h[hubs[i]]);
+        const uint64_t H = hoff_h[nh];
+        uint64_t *hoff = ws.get<uint64_t>(W_HOFF, (size_t)nh + 1);
+        LV_HIP(hipMemcpyAsync(hoff, hoff_h.data(), ((size_t)nh + 1) * sizeof(uint64_t),
+                              hipMemcpyHostToDevice, s));
+        uint64_t *hk0 = ws.get<uint64_t>(W_HK0, H), *hk1 = ws.get<uint64_t>(W_HK1, H);
+        uint32_t *hv0 = ws.get<uint32_t>(W_HV0, H), *hv1 = ws.get<uint32_t>(W_HV1, H);
+        const int hb = std::max(bits_for(nh), 1);
+        agg_hub_gather(x, lp[kHubBucket], nh, hoff, hub_parts, bd, hk0, hv0, s);
+        radix_sort_pairs(hk0, hv0, hk1, hv1, H, bd + hb, ws.get<uint32_t>(W_HIST, radix_hist_elems(H)),
+                         ws.get<uint32_t>(W_HIST_SCAN, scan_scratch_elems(radix_hist_elems(H))), s);
+        uint32_t *idx = ws.get<uint32_t>(W_HIDX, H);
+        uint64_t *S = ws.get<uint64_t>(W_HS, H);
+        heads_u64(hk0, H, idx, s);
+        scan_u32(idx, idx, H, true, ws.get<uint32_t>(W_SCAN32, scan_scratch_elems(H)), s);
+        widen_u32(hv0, (uint32_t)H, S, s);
+        scan_u64(S, S, H, true, ws.get<uint64_t>(W_SCAN64, scan_scratch_elems(H)), s);
+        uint32_t nu = 0;
+        LV_HIP(hipMemcpyAsync(&nu, idx + (H - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        LV_HIP(hipStreamSynchronize(s));
+        ukey = ws.get<uint64_t>(W_UKEY, nu);
+        wsum0 = ws.get<uint64_t>(W_WS, nu);
+        wsum1 = ws.get<uint64_t>(W_WE, nu);
+        agg_runs(hk0, hv0, idx, S, H, ukey, wsum0, wsum1, s);
+        ustart = ws.get<uint32_t>(W_USTART, (size_t)nh + 1);
+        agg_hub_rows(ukey, nu, bd, lp[kHubBucket], nh, ustart, x.rcount, s);
+    }
+    // 5. new offsets, then every row copied to place
+    uint64_t *orow = ng.neighboorhood_sum.data();
+    widen_u32(x.rcount, n_new, orow, s);
+    set_u64(orow + n_new, 0ull, s);
+    scan_u64(orow, orow, (size_t)n_new + 1, false,
+             ws.get<uint64_t>(W_SCAN64, scan_scratch_elems((size_t)n_new + 1)), s);
+    uint64_t e_new = 0;
+    LV_HIP(hipMemcpyAsync(&e_new, orow + n_new, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    LV_HIP(hipStreamSynchronize(s));
+    ng.edge_destination.resize(std::max<size_t>(e_new, 1));
+    ng.weights.resize(std::max<size_t>(e_new, 1));
+    agg_copy(x, lp, cnt, hub_parts, orow, ng.edge_destination.data(), ng.weights.data(), ustart,
+             ukey, wsum0, wsum1, (uint32_t)((1ull << bd) - 1), s);
+    new_degrees(c.communities_weight.data(), map, n, ng.tot_weight_per_nodes.data(), s);
+    LV_HIP(hipStreamSynchronize(s));
     ng.n_entries = e_new;
     ng.n_links = e_new / 2;                                           // :97
     c.graph = std::move(ng);
-    c.level += 1;
-    c.update();                                                       // :174
 }
 
 // ---------------------------------------------------- ModularityAlgorithms

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "../../include/device_array.h"
 #include "../../include/graph.h"
@@ -28,6 +29,23 @@ extern KernelStatsAcc g_last_run_stats;
 constexpr size_t kStatWordsOffset = (size_t)kNumStats * kStatSlots;
 constexpr size_t kHostWords = kStatWordsOffset + kNumBuckets;
 
+// Grow-only device scratch: one buffer per slot, reused across levels and
+// runs so that the level loop does not call hipMalloc/hipFree (which
+// synchronise the device) once the largest level has been seen.
+struct Workspace {
+    std::vector<DeviceArray<uint8_t>> slots;
+    template <typename T>
+    T *get(size_t slot, size_t n) {
+        if (slots.size() <= slot) slots.resize(slot + 1);
+        const size_t bytes = (n ? n : 1) * sizeof(T);
+        if (slots[slot].size() < bytes) {
+            slots[slot].release();
+            slots[slot].resize(bytes);
+        }
+        return reinterpret_cast<T *>(slots[slot].data());
+    }
+};
+
 struct SweepState {
     uint32_t n = 0;
     DeviceArray<uint8_t> bucket;
@@ -47,10 +65,11 @@ struct SweepState {
     DeviceArray<uint32_t> hub_slot_of;
     DeviceArray<uint64_t> hub_tab_off;
     DeviceArray<uint32_t> hub_tab_log2;
-    DeviceArray<uint32_t> hub_key, hub_val;
+    DeviceArray<unsigned long long> hub_slot;
     DeviceArray<int64_t> hub_part_s;
     DeviceArray<uint32_t> hub_part_c;
     DeviceArray<uint32_t> hub_self;
+    Workspace ws;
     // profiling
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     KernelStatsAcc acc;

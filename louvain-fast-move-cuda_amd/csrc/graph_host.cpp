@@ -3,6 +3,7 @@
 // (src,dst), the min_id shift and the degree computation).
 #include "../../include/graphHost.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -154,6 +155,23 @@ GraphHost GraphHost::from_csr(uint32_t n, uint64_t E, const uint64_t *row, const
         for (uint64_t j = row[v]; j < row[v + 1]; ++j) {
             if (col[j] >= n) throw lv::Error(LV_ERR_ARG, "column id out of range");
             wsum += w ? w[j] : 1.0;
+        }
+    }
+    // rows in destination order, as the reference's sort by (src, dst) leaves them
+    std::vector<std::pair<uint32_t, uint32_t>> tmp;
+    for (uint32_t v = 0; v < n; ++v) {
+        const uint64_t b = g.row[v], e = g.row[v + 1];
+        bool sorted = true;
+        for (uint64_t j = b + 1; j < e; ++j)
+            if (g.col[j] < g.col[j - 1]) { sorted = false; break; }
+        if (sorted) continue;
+        tmp.clear();
+        for (uint64_t j = b; j < e; ++j) tmp.emplace_back(g.col[j], g.unit_weights ? 1u : g.weights[j]);
+        std::stable_sort(tmp.begin(), tmp.end(),
+                         [](const auto &x, const auto &y) { return x.first < y.first; });
+        for (uint64_t j = b; j < e; ++j) {
+            g.col[j] = tmp[j - b].first;
+            if (!g.unit_weights) g.weights[j] = tmp[j - b].second;
         }
     }
     g.total_weight = total_weight > 0 ? total_weight : wsum / 2;

@@ -3,6 +3,7 @@
 // std::bad_alloc to an lv_status.
 #include "../../include/lv_api.h"
 
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -239,13 +240,34 @@ lv_status lv_community_copy_graph(const lv_community *c, uint64_t *row, uint32_t
     if (!c) return LV_ERR_ARG;
     return guarded([&] {
         const GraphDevice &g = c->c.graph;
-        if (row) LV_HIP(hipMemcpy(row, g.neighboorhood_sum.data(), ((size_t)g.n_nodes + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
-        if (col && g.n_entries) LV_HIP(hipMemcpy(col, g.edge_destination.data(), g.n_entries * sizeof(uint32_t), hipMemcpyDeviceToHost));
-        if (w) {
-            if (g.unit_weights) std::fill(w, w + g.n_entries, 1u);
-            else if (g.n_entries) LV_HIP(hipMemcpy(w, g.weights.data(), g.n_entries * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        std::vector<uint64_t> r((size_t)g.n_nodes + 1);
+        std::vector<uint32_t> cc(g.n_entries), ww(g.n_entries, 1u);
+        LV_HIP(hipMemcpy(r.data(), g.neighboorhood_sum.data(), r.size() * sizeof(uint64_t),
+                         hipMemcpyDeviceToHost));
+        if (g.n_entries)
+            LV_HIP(hipMemcpy(cc.data(), g.edge_destination.data(), g.n_entries * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost));
+        if (!g.unit_weights && g.n_entries)
+            LV_HIP(hipMemcpy(ww.data(), g.weights.data(), g.n_entries * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost));
+        // rows of contracted levels are kept in hash-table order on the device;
+        // exported rows are put in destination order (the reference's CSR order)
+        std::vector<std::pair<uint32_t, uint32_t>> tmp;
+        for (uint32_t v = 0; v < g.n_nodes; ++v) {
+            tmp.clear();
+            for (uint64_t j = r[v]; j < r[v + 1]; ++j) tmp.emplace_back(cc[j], ww[j]);
+            std::sort(tmp.begin(), tmp.end());
+            for (uint64_t j = r[v]; j < r[v + 1]; ++j) {
+                cc[j] = tmp[j - r[v]].first;
+                ww[j] = tmp[j - r[v]].second;
+            }
         }
-        if (k && g.n_nodes) LV_HIP(hipMemcpy(k, g.tot_weight_per_nodes.data(), g.n_nodes * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (row) std::memcpy(row, r.data(), r.size() * sizeof(uint64_t));
+        if (col) std::memcpy(col, cc.data(), cc.size() * sizeof(uint32_t));
+        if (w) std::memcpy(w, ww.data(), ww.size() * sizeof(uint32_t));
+        if (k && g.n_nodes)
+            LV_HIP(hipMemcpy(k, g.tot_weight_per_nodes.data(), g.n_nodes * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost));
     });
 }
 

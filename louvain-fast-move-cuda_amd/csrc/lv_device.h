@@ -1,0 +1,162 @@
+// lv_device.h -- device helpers shared by the gfx950 kernels: wave64
+// shuffles and reductions, LDS hash slots, per-block statistics, degree
+// buckets and bucket-list appends.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/constants.h"
+#include "../../include/operators.h"
+#include "lv_kernels.h"
+
+namespace lv {
+
+constexpr int kBlock = 256;
+
+__device__ inline unsigned long long lanemask_lt() {
+    const int lane = threadIdx.x & 63;
+    return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+}
+
+__device__ inline int64_t shfl_xor_i64(int64_t v, int off) {
+    const int lo = __shfl_xor((int)(uint32_t)(uint64_t)v, off);
+    const int hi = __shfl_xor((int)(uint32_t)((uint64_t)v >> 32), off);
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+}
+
+__device__ inline uint64_t shfl_xor_u64(uint64_t v, int off) {
+    return (uint64_t)shfl_xor_i64((int64_t)v, off);
+}
+
+__device__ inline float shfl_xor_f32(float v, int off) { return __shfl_xor(v, off); }
+
+template <typename S>
+__device__ inline S shfl_xor_s(S v, int off);
+template <>
+__device__ inline int64_t shfl_xor_s<int64_t>(int64_t v, int off) { return shfl_xor_i64(v, off); }
+template <>
+__device__ inline float shfl_xor_s<float>(float v, int off) { return shfl_xor_f32(v, off); }
+
+// Butterfly reduction of (score, community) inside aligned groups of G lanes
+// (G <= 64); every lane of the group ends with the group's best.
+template <int G, typename S>
+__device__ inline void group_best(S &s, uint32_t &c, bool highest) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        const S os = shfl_xor_s<S>(s, off);
+        const uint32_t oc = (uint32_t)__shfl_xor((int)c, off);
+        if (better_candidate(os, oc, s, c, highest)) { s = os; c = oc; }
+    }
+}
+
+template <int G>
+__device__ inline uint64_t group_sum_u64(uint64_t x) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) x += shfl_xor_u64(x, off);
+    return x;
+}
+
+// Block-wide sum of up to 3 counters; thread 0 adds them to the spread slots.
+__device__ inline void block_add_stats(unsigned long long *stats, int st0, uint64_t a0, int st1,
+                                       uint64_t a1, int st2, uint64_t a2) {
+    __shared__ uint64_t red[3][kBlock / 64];
+    a0 = group_sum_u64<64>(a0);
+    a1 = group_sum_u64<64>(a1);
+    a2 = group_sum_u64<64>(a2);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][w] = a0; red[1][w] = a1; red[2][w] = a2; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t s0 = 0, s1 = 0, s2 = 0;
+        for (int i = 0; i < kBlock / 64; ++i) { s0 += red[0][i]; s1 += red[1][i]; s2 += red[2][i]; }
+        const int slot = blockIdx.x & (kStatSlots - 1);
+        if (st0 >= 0 && s0) atomicAdd(&stats[st0 * kStatSlots + slot], (unsigned long long)s0);
+        if (st1 >= 0 && s1) atomicAdd(&stats[st1 * kStatSlots + slot], (unsigned long long)s1);
+        if (st2 >= 0 && s2) atomicAdd(&stats[st2 * kStatSlots + slot], (unsigned long long)s2);
+    }
+    __syncthreads();
+}
+
+template <int LOG2T>
+__device__ inline uint32_t slot_hash(uint32_t c) {
+    return (c * 2654435761u) >> (32 - LOG2T);
+}
+
+__device__ inline uint32_t slot_hash_dyn(uint32_t c, uint32_t log2t) {
+    return (c * 2654435761u) >> (32 - log2t);
+}
+
+constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
+
+template <int GAIN>
+struct Score;
+template <>
+struct Score<0> {
+    using T = int64_t;
+    __device__ static T none() { return LLONG_MIN; }
+    __device__ static T of(uint32_t w, uint32_t self, uint32_t k, uint32_t tot_own, uint32_t tot_c,
+                           int64_t m2, double) {
+        (void)self; (void)tot_own;
+        return move_score_exact(w, k, tot_c, m2);
+    }
+    __device__ static bool positive(T best, uint32_t self, uint32_t k, uint32_t tot_own, int64_t m2) {
+        return delta_modularity_exact(best, self, k, tot_own, m2) > 0;
+    }
+};
+template <>
+struct Score<1> {
+    using T = float;
+    __device__ static T none() { return -INFINITY; }
+    __device__ static T of(uint32_t w, uint32_t self, uint32_t k, uint32_t tot_own, uint32_t tot_c,
+                           int64_t, double tw) {
+        return delta_modularity_ref_f32(w, self, k, tot_own, tot_c, tw);
+    }
+    __device__ static bool positive(T best, uint32_t, uint32_t, uint32_t, int64_t) {
+        return best > 0.0f;
+    }
+};
+
+
+inline unsigned grid_for(size_t n, size_t per_block, unsigned cap = 65536u) {
+    size_t g = (n + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+__device__ inline uint8_t bucket_of_degree(uint64_t d) {
+#pragma unroll
+    for (int b = 0; b < kNumBuckets - 1; ++b)
+        if (d <= kBucketMaxDeg[b]) return (uint8_t)b;
+    return (uint8_t)kHubBucket;
+}
+
+struct ListPtrs {
+    uint32_t *p[kNumBuckets];
+};
+
+// Append the vertex of each thread (if selected) to the list of its bucket,
+// with one global atomic per bucket per tile.
+__device__ inline void append_to_bucket(bool sel, uint32_t v, int b, const ListPtrs &lists,
+                                        uint32_t *counts) {
+    __shared__ uint32_t s_cnt[kNumBuckets];
+    __shared__ uint32_t s_base[kNumBuckets];
+    if (threadIdx.x < kNumBuckets) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t rank = 0;
+    if (sel) rank = atomicAdd(&s_cnt[b], 1u);
+    __syncthreads();
+    if (threadIdx.x < kNumBuckets) {
+        const uint32_t c = s_cnt[threadIdx.x];
+        s_base[threadIdx.x] = c ? atomicAdd(&counts[threadIdx.x], c) : 0u;
+    }
+    __syncthreads();
+    if (sel) lists.p[b][s_base[b] + rank] = v;
+    __syncthreads();
+}
+
+}  // namespace lv

@@ -21,6 +21,7 @@ constexpr int kNumBuckets = 9;        // 8 LDS-table buckets + global-table hubs
 constexpr int kHubBucket = 8;
 constexpr int kStatSlots = 64;        // spread of the per-block statistic atomics
 constexpr int kHubParts = 32;         // workgroups per hub vertex
+constexpr unsigned long long kHubEmpty = 0xFFFFFFFF00000000ull;   // empty packed slot
 
 // Maximum degree handled by each LDS bucket; larger goes to the hub bucket.
 constexpr uint32_t kBucketMaxDeg[kNumBuckets - 1] = {4, 8, 16, 32, 64, 256, 1024, 4096};
@@ -49,8 +50,7 @@ struct HubTables {
     uint32_t *slot_of;        // [n] hub slot of a vertex (LV_EMPTY if not a hub)
     uint64_t *tab_off;        // [n_hubs] offset of the hub's table
     uint32_t *tab_log2;       // [n_hubs] log2 of its table size
-    uint32_t *key;            // pooled table keys (kept LV_EMPTY between uses)
-    uint32_t *val;            // pooled table weights (kept 0 between uses)
+    unsigned long long *slot; // pooled tables: (community << 32) | weight, kept kHubEmpty between uses
     int64_t *part_s;          // [n_hubs * kHubParts] partial best scores
     uint32_t *part_c;         // [n_hubs * kHubParts] partial best communities
     uint32_t *self;           // [n_hubs] weight to own community
@@ -103,6 +103,9 @@ void build_lists_all(const uint8_t *bucket, uint32_t n, uint32_t *const *lists, 
 void modularity_parts(const LevelView &g, const uint32_t *comm, const uint32_t *tot,
                       unsigned long long *stats, hipStream_t s);
 
+// sigma_in and sum of k^2 of the identity partition (self-loop weights)
+void identity_parts(const LevelView &g, unsigned long long *stats, hipStream_t s);
+
 // ---- sweep ----
 void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
                const uint32_t *counts, hipStream_t s);
@@ -117,13 +120,47 @@ void finalize_pass(const SweepArgs &a, uint32_t *const *next_lists, uint32_t *ne
 void renumber_flags(const uint32_t *tot, uint32_t n, uint32_t *flags, hipStream_t s);
 void update_best(uint32_t *best, uint32_t n0, const uint32_t *comm, const uint32_t *map,
                  hipStream_t s);
-void relabel_entries(const LevelView &g, const uint32_t *comm, const uint32_t *map, uint32_t n_new,
-                     uint64_t *keys, uint32_t *vals, hipStream_t s);
-void run_heads(const uint64_t *keys, size_t n, uint32_t *heads, hipStream_t s);
-void reduce_runs(const uint64_t *keys, const uint32_t *vals, const uint32_t *idx, size_t n,
-                 uint32_t *out_col, uint32_t *out_w, uint64_t *out_row, uint32_t n_new,
-                 hipStream_t s);
 void new_degrees(const uint32_t *tot, const uint32_t *map, uint32_t n, uint32_t *k_new,
                  hipStream_t s);
+
+// ---- aggregation by community hashing (lv_aggregate.hip) ----
+struct AggArgs {
+    LevelView g;
+    const uint32_t *newc;      // [n]
+    const uint32_t *members;   // [n] vertices sorted by new community
+    const uint32_t *mstart;    // [n_new + 1]
+    const uint64_t *P;         // [n + 1] prefix of member degrees
+    const uint64_t *crow;      // [n_new + 1] virtual entry offsets of communities
+    uint32_t *tcol;            // [E] staging
+    uint32_t *tw;              // [E] staging
+    uint32_t *rcount;          // [n_new], zeroed
+};
+// newc[v] = map[comm[v]]-1; key/val prepared for the vertex sort (key = newc, val = v)
+void agg_newc(const uint32_t *comm, const uint32_t *map, uint32_t n, uint32_t *newc, uint64_t *key,
+              uint32_t *val, hipStream_t s);
+// member starts (mstart[n_new] = n) and member degrees (mdeg[n] = 0)
+void agg_members(const uint64_t *skey, const uint32_t *members, const uint64_t *row, uint32_t n,
+                 uint32_t n_new, uint32_t *mstart, uint64_t *mdeg, hipStream_t s);
+void agg_comm_rows(const uint64_t *P, const uint32_t *mstart, uint32_t n_new, uint64_t *crow,
+                   hipStream_t s);
+// LDS-table communities (buckets 0-7): sorted distinct rows into the staging area
+void agg_tables(const AggArgs &x, uint32_t *const *lists, const uint32_t *counts, hipStream_t s);
+// giant communities: their entries as keys (hub index << bits | destination)
+void agg_hub_gather(const AggArgs &x, const uint32_t *hub_list, uint32_t nh, const uint64_t *hoff,
+                    uint32_t parts, int bits, uint64_t *hkey, uint32_t *hval, hipStream_t s);
+void heads_u64(const uint64_t *keys, uint64_t n, uint32_t *heads, hipStream_t s);
+// runs of sorted keys: idx = inclusive scan of heads, S = inclusive scan of weights
+void agg_runs(const uint64_t *keys, const uint32_t *vals, const uint32_t *idx, const uint64_t *S,
+              uint64_t n, uint64_t *ukey, uint64_t *ws, uint64_t *we, hipStream_t s);
+void agg_hub_rows(const uint64_t *ukey, uint64_t nu, int bits, const uint32_t *hub_list,
+                  uint32_t nh, uint32_t *ustart, uint32_t *rcount, hipStream_t s);
+void agg_copy(const AggArgs &x, uint32_t *const *lists, const uint32_t *counts, uint32_t hub_parts,
+              const uint64_t *out_row, uint32_t *col, uint32_t *w, const uint32_t *ustart,
+              const uint64_t *ukey, const uint64_t *ws, const uint64_t *we, uint32_t mask,
+              hipStream_t s);
+void set_u64(uint64_t *p, uint64_t v, hipStream_t s);
+void widen_u32(const uint32_t *in, uint32_t n, uint64_t *out, hipStream_t s);
+// per-bucket sizes of a degree array given by offsets row[n+1]
+void bucket_counts(const uint8_t *bucket, uint32_t n, uint32_t *counts, hipStream_t s);
 
 }  // namespace lv

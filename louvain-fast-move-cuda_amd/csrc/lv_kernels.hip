@@ -17,9 +17,8 @@
 // symmetric).  Finalize pass: comm := target, sum of tot^2, and compaction of
 // the marked vertices into per-bucket active lists for the next sweep.
 //
-// Aggregation (phaseAggregation.cuh:19-175): non-empty flags, inclusive scan
-// (renumbering), relabelled entries sorted by an LSD radix sort, run-length
-// reduction into the contracted CSR.
+// Aggregation (phaseAggregation.cuh:19-175): non-empty flags and inclusive
+// scan (renumbering) here; the contraction itself is in lv_aggregate.hip.
 #include "lv_kernels.h"
 
 #include "../../include/constants.h"
@@ -28,114 +27,11 @@
 #include <climits>
 #include <cmath>
 
+#include "lv_device.h"
+
 namespace lv {
 
 namespace {
-
-constexpr int kBlock = 256;
-
-__device__ inline unsigned long long lanemask_lt() {
-    const int lane = threadIdx.x & 63;
-    return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
-}
-
-__device__ inline int64_t shfl_xor_i64(int64_t v, int off) {
-    const int lo = __shfl_xor((int)(uint32_t)(uint64_t)v, off);
-    const int hi = __shfl_xor((int)(uint32_t)((uint64_t)v >> 32), off);
-    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
-}
-
-__device__ inline uint64_t shfl_xor_u64(uint64_t v, int off) {
-    return (uint64_t)shfl_xor_i64((int64_t)v, off);
-}
-
-__device__ inline float shfl_xor_f32(float v, int off) { return __shfl_xor(v, off); }
-
-template <typename S>
-__device__ inline S shfl_xor_s(S v, int off);
-template <>
-__device__ inline int64_t shfl_xor_s<int64_t>(int64_t v, int off) { return shfl_xor_i64(v, off); }
-template <>
-__device__ inline float shfl_xor_s<float>(float v, int off) { return shfl_xor_f32(v, off); }
-
-// Butterfly reduction of (score, community) inside aligned groups of G lanes
-// (G <= 64); every lane of the group ends with the group's best.
-template <int G, typename S>
-__device__ inline void group_best(S &s, uint32_t &c, bool highest) {
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) {
-        const S os = shfl_xor_s<S>(s, off);
-        const uint32_t oc = (uint32_t)__shfl_xor((int)c, off);
-        if (better_candidate(os, oc, s, c, highest)) { s = os; c = oc; }
-    }
-}
-
-template <int G>
-__device__ inline uint64_t group_sum_u64(uint64_t x) {
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) x += shfl_xor_u64(x, off);
-    return x;
-}
-
-// Block-wide sum of up to 3 counters; thread 0 adds them to the spread slots.
-__device__ inline void block_add_stats(unsigned long long *stats, int st0, uint64_t a0, int st1,
-                                       uint64_t a1, int st2, uint64_t a2) {
-    __shared__ uint64_t red[3][kBlock / 64];
-    a0 = group_sum_u64<64>(a0);
-    a1 = group_sum_u64<64>(a1);
-    a2 = group_sum_u64<64>(a2);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { red[0][w] = a0; red[1][w] = a1; red[2][w] = a2; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t s0 = 0, s1 = 0, s2 = 0;
-        for (int i = 0; i < kBlock / 64; ++i) { s0 += red[0][i]; s1 += red[1][i]; s2 += red[2][i]; }
-        const int slot = blockIdx.x & (kStatSlots - 1);
-        if (st0 >= 0 && s0) atomicAdd(&stats[st0 * kStatSlots + slot], (unsigned long long)s0);
-        if (st1 >= 0 && s1) atomicAdd(&stats[st1 * kStatSlots + slot], (unsigned long long)s1);
-        if (st2 >= 0 && s2) atomicAdd(&stats[st2 * kStatSlots + slot], (unsigned long long)s2);
-    }
-    __syncthreads();
-}
-
-template <int LOG2T>
-__device__ inline uint32_t slot_hash(uint32_t c) {
-    return (c * 2654435761u) >> (32 - LOG2T);
-}
-
-__device__ inline uint32_t slot_hash_dyn(uint32_t c, uint32_t log2t) {
-    return (c * 2654435761u) >> (32 - log2t);
-}
-
-constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
-
-template <int GAIN>
-struct Score;
-template <>
-struct Score<0> {
-    using T = int64_t;
-    __device__ static T none() { return LLONG_MIN; }
-    __device__ static T of(uint32_t w, uint32_t self, uint32_t k, uint32_t tot_own, uint32_t tot_c,
-                           int64_t m2, double) {
-        (void)self; (void)tot_own;
-        return move_score_exact(w, k, tot_c, m2);
-    }
-    __device__ static bool positive(T best, uint32_t self, uint32_t k, uint32_t tot_own, int64_t m2) {
-        return delta_modularity_exact(best, self, k, tot_own, m2) > 0;
-    }
-};
-template <>
-struct Score<1> {
-    using T = float;
-    __device__ static T none() { return -INFINITY; }
-    __device__ static T of(uint32_t w, uint32_t self, uint32_t k, uint32_t tot_own, uint32_t tot_c,
-                           int64_t, double tw) {
-        return delta_modularity_ref_f32(w, self, k, tot_own, tot_c, tw);
-    }
-    __device__ static bool positive(T best, uint32_t, uint32_t, uint32_t, int64_t) {
-        return best > 0.0f;
-    }
-};
 
 // ---------------------------------------------------------------- utilities
 
@@ -173,12 +69,6 @@ __global__ void k_community_totals(const uint32_t *comm, const uint32_t *kdeg, u
         atomicAdd(&tot[comm[v]], kdeg[v]);
 }
 
-inline unsigned grid_for(size_t n, size_t per_block, unsigned cap = 65536u) {
-    size_t g = (n + per_block - 1) / per_block;
-    if (g < 1) g = 1;
-    if (g > cap) g = cap;
-    return (unsigned)g;
-}
 
 // ------------------------------------------------------------------- scan
 
@@ -355,41 +245,23 @@ __global__ __launch_bounds__(kBlock) void k_rs_downsweep(const uint64_t *kin, co
 
 // ------------------------------------------------------------ level setup
 
-__device__ inline uint8_t bucket_of_degree(uint64_t d) {
-#pragma unroll
-    for (int b = 0; b < kNumBuckets - 1; ++b)
-        if (d <= kBucketMaxDeg[b]) return (uint8_t)b;
-    return (uint8_t)kHubBucket;
-}
-
 __global__ void k_bucket_ids(const uint64_t *row, uint32_t n, uint8_t *bucket) {
     for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n;
          v += (size_t)gridDim.x * blockDim.x)
         bucket[v] = bucket_of_degree(row[v + 1] - row[v]);
 }
 
-struct ListPtrs {
-    uint32_t *p[kNumBuckets];
-};
 
-// Append the vertex of each thread (if selected) to the list of its bucket,
-// with one global atomic per bucket per tile.
-__device__ inline void append_to_bucket(bool sel, uint32_t v, int b, const ListPtrs &lists,
-                                        uint32_t *counts) {
-    __shared__ uint32_t s_cnt[kNumBuckets];
-    __shared__ uint32_t s_base[kNumBuckets];
-    if (threadIdx.x < kNumBuckets) s_cnt[threadIdx.x] = 0;
+__global__ __launch_bounds__(kBlock) void k_bucket_counts(const uint8_t *bucket, uint32_t n,
+                                                          uint32_t *counts) {
+    __shared__ uint32_t h[kNumBuckets];
+    if (threadIdx.x < kNumBuckets) h[threadIdx.x] = 0;
     __syncthreads();
-    uint32_t rank = 0;
-    if (sel) rank = atomicAdd(&s_cnt[b], 1u);
+    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n;
+         v += (size_t)gridDim.x * blockDim.x)
+        atomicAdd(&h[bucket[v]], 1u);
     __syncthreads();
-    if (threadIdx.x < kNumBuckets) {
-        const uint32_t c = s_cnt[threadIdx.x];
-        s_base[threadIdx.x] = c ? atomicAdd(&counts[threadIdx.x], c) : 0u;
-    }
-    __syncthreads();
-    if (sel) lists.p[b][s_base[b] + rank] = v;
-    __syncthreads();
+    if (threadIdx.x < kNumBuckets && h[threadIdx.x]) atomicAdd(&counts[threadIdx.x], h[threadIdx.x]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_build_lists_all(const uint8_t *bucket, uint32_t n,
@@ -435,6 +307,25 @@ __global__ __launch_bounds__(kBlock) void k_modularity_parts(LevelView g, const 
     for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < g.n;
          c += (uint64_t)gridDim.x * blockDim.x)
         sq += square_u64(tot[c]);
+    block_add_stats(stats, ST_DSIGMA, sig, ST_SUMSQ, sq, -1, 0);
+}
+
+// Identity partition: sigma_in = weight of self-loop entries (rows are sorted
+// by destination, so v's own entries are found by binary search), sum of k^2.
+__global__ __launch_bounds__(kBlock) void k_identity_parts(LevelView g,
+                                                           unsigned long long *stats) {
+    uint64_t sig = 0, sq = 0;
+    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < g.n;
+         v += (size_t)gridDim.x * blockDim.x) {
+        uint64_t lo = g.row[v], hi = g.row[v + 1];
+        const uint64_t end = hi;
+        while (lo < hi) {
+            const uint64_t mid = lo + ((hi - lo) >> 1);
+            if (g.col[mid] < v) lo = mid + 1; else hi = mid;
+        }
+        for (uint64_t j = lo; j < end && g.col[j] == v; ++j) sig += g.wt ? g.wt[j] : 1u;
+        sq += square_u64(g.kdeg[v]);
+    }
     block_add_stats(stats, ST_DSIGMA, sig, ST_SUMSQ, sq, -1, 0);
 }
 
@@ -639,8 +530,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_insert(HubCtx a, const uint32_t 
             const uint64_t off = a.h.tab_off[slot];
             const uint32_t lg = a.h.tab_log2[slot];
             const uint32_t mask = (1u << lg) - 1u;
-            uint32_t *K = a.h.key + off;
-            uint32_t *V = a.h.val + off;
+            unsigned long long *S = a.h.slot + off;
             const uint64_t beg = a.m.row[v], end = a.m.row[v + 1];
             for (uint64_t j = beg + (uint64_t)part * kBlock + threadIdx.x; j < end;
                  j += (uint64_t)kHubParts * kBlock) {
@@ -649,10 +539,23 @@ __global__ __launch_bounds__(kBlock) void k_hub_insert(HubCtx a, const uint32_t 
                 ++nself;
                 const uint32_t c = a.m.comm[x];
                 const uint32_t w = WEIGHTED ? a.m.wt[j] : 1u;
+                // one atomic per entry: add to the packed slot holding c, or
+                // claim an empty slot with (c << 32 | w)
                 uint32_t h = slot_hash_dyn(c, lg);
                 for (;;) {
-                    const uint32_t prev = atomicCAS(&K[h], LV_EMPTY, c);
-                    if (prev == LV_EMPTY || prev == c) { atomicAdd(&V[h], w); break; }
+                    const unsigned long long cur =
+                        __hip_atomic_load(&S[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const uint32_t kk = (uint32_t)(cur >> 32);
+                    if (kk == c) { atomicAdd(&S[h], (unsigned long long)w); break; }
+                    if (kk == LV_EMPTY) {
+                        const unsigned long long prev =
+                            atomicCAS(&S[h], kHubEmpty, ((unsigned long long)c << 32) | w);
+                        if (prev == kHubEmpty) break;
+                        if ((uint32_t)(prev >> 32) == c) {
+                            atomicAdd(&S[h], (unsigned long long)w);
+                            break;
+                        }
+                    }
                     h = (h + 1) & mask;
                 }
             }
@@ -674,8 +577,9 @@ __global__ void k_hub_self(HubCtx a, const uint32_t *__restrict__ list, uint32_t
         const uint32_t mask = (1u << lg) - 1u;
         uint32_t h = slot_hash_dyn(own, lg);
         for (;;) {
-            const uint32_t c = a.h.key[off + h];
-            if (c == own) { self = a.h.val[off + h]; break; }
+            const unsigned long long cur = a.h.slot[off + h];
+            const uint32_t c = (uint32_t)(cur >> 32);
+            if (c == own) { self = (uint32_t)cur; break; }
             if (c == LV_EMPTY) break;
             h = (h + 1) & mask;
         }
@@ -709,11 +613,11 @@ __global__ __launch_bounds__(kBlock) void k_hub_scan(HubCtx a, const uint32_t *_
             const uint32_t tot_own = a.m.tot[own];
             const uint32_t self = a.h.self[slot];
             for (uint32_t s = part * per + threadIdx.x; s < (part + 1) * per; s += kBlock) {
-                const uint32_t c = a.h.key[off + s];
+                const unsigned long long cur = a.h.slot[off + s];
+                const uint32_t c = (uint32_t)(cur >> 32);
                 if (c == LV_EMPTY) continue;
-                const uint32_t W = a.h.val[off + s];
-                a.h.key[off + s] = LV_EMPTY;
-                a.h.val[off + s] = 0u;
+                const uint32_t W = (uint32_t)cur;
+                a.h.slot[off + s] = kHubEmpty;
                 if (c == own) continue;
                 ++ncand;
                 const S sc = SC::of(W, self, kv, tot_own, a.m.tot[c], a.m.m2, a.m.tw);
@@ -933,62 +837,6 @@ __global__ void k_update_best(uint32_t *best, uint32_t n0, const uint32_t *comm,
         best[v] = renumbered(map, comm[best[v]]);
 }
 
-__global__ __launch_bounds__(kBlock) void k_relabel(LevelView g, const uint32_t *comm,
-                                                    const uint32_t *map, uint32_t n_new,
-                                                    uint64_t *keys, uint32_t *vals) {
-    const uint64_t nchunks = (g.E + kChunk - 1) / kChunk;
-    for (uint64_t ch = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; ch < nchunks;
-         ch += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t j0 = ch * kChunk;
-        const uint64_t j1 = j0 + kChunk < g.E ? j0 + kChunk : g.E;
-        uint32_t v = src_of_entry(g.row, g.n, j0);
-        uint64_t vend = g.row[v + 1];
-        uint64_t cs = renumbered(map, comm[v]);
-        for (uint64_t j = j0; j < j1; ++j) {
-            while (j >= vend) { ++v; vend = g.row[v + 1]; cs = renumbered(map, comm[v]); }
-            keys[j] = cs * (uint64_t)n_new + renumbered(map, comm[g.col[j]]);
-            vals[j] = g.wt ? g.wt[j] : 1u;
-        }
-    }
-}
-
-__global__ void k_run_heads(const uint64_t *keys, size_t n, uint32_t *heads) {
-    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < n;
-         j += (size_t)gridDim.x * blockDim.x)
-        heads[j] = (j == 0 || keys[j] != keys[j - 1]) ? 1u : 0u;
-}
-
-// idx = inclusive scan of heads (1-based output index of each entry's run).
-__global__ __launch_bounds__(kBlock) void k_reduce_runs(const uint64_t *keys, const uint32_t *vals,
-                                                        const uint32_t *idx, size_t n,
-                                                        uint32_t *out_col, uint32_t *out_w,
-                                                        uint64_t *out_row, uint32_t n_new) {
-    const uint64_t nchunks = (n + kChunk - 1) / kChunk;
-    for (uint64_t ch = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; ch < nchunks;
-         ch += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t j0 = ch * kChunk;
-        const uint64_t j1 = j0 + kChunk < n ? j0 + kChunk : n;
-        uint32_t cur = idx[j0];
-        uint32_t acc = 0;
-        for (uint64_t j = j0; j < j1; ++j) {
-            const uint32_t o = idx[j];
-            if (o != cur) {
-                atomicAdd(&out_w[cur - 1], acc);
-                acc = 0;
-                cur = o;
-            }
-            acc += vals[j];
-            if (j == 0 || keys[j] != keys[j - 1]) {
-                const uint64_t key = keys[j];
-                const uint32_t src = (uint32_t)(key / n_new);
-                out_col[o - 1] = (uint32_t)(key % n_new);
-                if (j == 0 || (uint32_t)(keys[j - 1] / n_new) != src) out_row[src] = o - 1;
-            }
-        }
-        atomicAdd(&out_w[cur - 1], acc);
-    }
-}
-
 __global__ void k_new_degrees(const uint32_t *tot, const uint32_t *map, uint32_t n,
                               uint32_t *k_new) {
     for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
@@ -1085,6 +933,13 @@ void bucket_ids(const uint64_t *row, uint32_t n, uint8_t *bucket, hipStream_t s)
                        bucket);
 }
 
+void bucket_counts(const uint8_t *bucket, uint32_t n, uint32_t *counts, hipStream_t s) {
+    fill_u32(counts, 0u, kNumBuckets, s);
+    if (!n) return;
+    hipLaunchKernelGGL(k_bucket_counts, dim3(grid_for(n, kBlock * 16, 4096u)), dim3(kBlock), 0, s,
+                       bucket, n, counts);
+}
+
 void build_lists_all(const uint8_t *bucket, uint32_t n, uint32_t *const *lists, uint32_t *counts,
                      hipStream_t s) {
     ListPtrs lp;
@@ -1093,6 +948,12 @@ void build_lists_all(const uint8_t *bucket, uint32_t n, uint32_t *const *lists, 
     if (!n) return;
     hipLaunchKernelGGL(k_build_lists_all, dim3(grid_for(n, kBlock * 4, 16384u)), dim3(kBlock), 0,
                        s, bucket, n, lp, counts);
+}
+
+void identity_parts(const LevelView &g, unsigned long long *stats, hipStream_t s) {
+    if (!g.n) return;
+    hipLaunchKernelGGL(k_identity_parts, dim3(grid_for(g.n, kBlock, 8192u)), dim3(kBlock), 0, s, g,
+                       stats);
 }
 
 void modularity_parts(const LevelView &g, const uint32_t *comm, const uint32_t *tot,
@@ -1258,29 +1119,6 @@ void update_best(uint32_t *best, uint32_t n0, const uint32_t *comm, const uint32
     if (!n0) return;
     hipLaunchKernelGGL(k_update_best, dim3(grid_for(n0, kBlock * 4)), dim3(kBlock), 0, s, best, n0,
                        comm, map);
-}
-
-void relabel_entries(const LevelView &g, const uint32_t *comm, const uint32_t *map, uint32_t n_new,
-                     uint64_t *keys, uint32_t *vals, hipStream_t s) {
-    if (!g.E) return;
-    const uint64_t nchunks = (g.E + kChunk - 1) / kChunk;
-    hipLaunchKernelGGL(k_relabel, dim3(grid_for(nchunks, kBlock, 16384u)), dim3(kBlock), 0, s, g,
-                       comm, map, n_new, keys, vals);
-}
-
-void run_heads(const uint64_t *keys, size_t n, uint32_t *heads, hipStream_t s) {
-    if (!n) return;
-    hipLaunchKernelGGL(k_run_heads, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, keys, n,
-                       heads);
-}
-
-void reduce_runs(const uint64_t *keys, const uint32_t *vals, const uint32_t *idx, size_t n,
-                 uint32_t *out_col, uint32_t *out_w, uint64_t *out_row, uint32_t n_new,
-                 hipStream_t s) {
-    if (!n) return;
-    const uint64_t nchunks = (n + kChunk - 1) / kChunk;
-    hipLaunchKernelGGL(k_reduce_runs, dim3(grid_for(nchunks, kBlock, 16384u)), dim3(kBlock), 0, s,
-                       keys, vals, idx, n, out_col, out_w, out_row, n_new);
 }
 
 void new_degrees(const uint32_t *tot, const uint32_t *map, uint32_t n, uint32_t *k_new,
