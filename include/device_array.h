@@ -77,6 +77,11 @@ class DeviceArray {
         n_ = n;
         owned_ = true;
     }
+    // Keep the storage when it already holds n elements (grow-only reuse).
+    void ensure(size_t n) {
+        if (n <= n_ && owned_) return;
+        resize(n);
+    }
     void release() {
         if (p_ && owned_) (void)hipFree(p_);
         p_ = nullptr;

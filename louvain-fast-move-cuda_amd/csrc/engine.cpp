@@ -35,13 +35,13 @@ KernelStatsAcc g_last_run_stats;
 
 void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
     n = g.n_nodes;
-    bucket.resize(std::max<size_t>(n, 1));
-    mark.resize(std::max<size_t>(n, 1));
-    target.resize(std::max<size_t>(n, 1));
-    list_buf[0].resize(std::max<size_t>(n, 1));
-    list_buf[1].resize(std::max<size_t>(n, 1));
-    counts_dev.resize(2 * kNumBuckets);
-    stats.resize((size_t)kNumStats * kStatSlots);
+    bucket.ensure(std::max<size_t>(n, 1));
+    mark.ensure(std::max<size_t>(n, 1));
+    target.ensure(std::max<size_t>(n, 1));
+    list_buf[0].ensure(std::max<size_t>(n, 1));
+    list_buf[1].ensure(std::max<size_t>(n, 1));
+    counts_dev.ensure(2 * kNumBuckets);
+    stats.ensure((size_t)kNumStats * kStatSlots);
     if (!host_stats) {
         LV_HIP(hipHostMalloc((void **)&host_stats, kHostWords * sizeof(unsigned long long),
                              hipHostMallocDefault));
@@ -49,15 +49,11 @@ void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
     fill_u8(mark.data(), 0, n, s);
     // bucket ids and per-bucket sizes (static: degrees do not change within a level)
     bucket_ids(g.neighboorhood_sum.data(), n, bucket.data(), s);
-    uint32_t *tmp_lists[kNumBuckets];
-    for (int b = 0; b < kNumBuckets; ++b) tmp_lists[b] = list_buf[1].data();  // counting pass only
-    // count per bucket by building into a scratch list region per bucket: first a
-    // histogram (lists all aliased to one buffer, then rebuilt with true offsets)
-    build_lists_all(bucket.data(), n, tmp_lists, counts_dev.data(), s);
+    bucket_counts(bucket.data(), n, counts_dev.data(), s);
+    uint32_t cnt[kNumBuckets];
     LV_HIP(hipMemcpyAsync(host_stats, counts_dev.data(), kNumBuckets * sizeof(uint32_t),
                           hipMemcpyDeviceToHost, s));
     LV_HIP(hipStreamSynchronize(s));
-    uint32_t cnt[kNumBuckets];
     std::memcpy(cnt, host_stats, sizeof(cnt));
     uint64_t off = 0;
     for (int b = 0; b < kNumBuckets; ++b) {
@@ -67,42 +63,38 @@ void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
     }
     // hub tables
     n_hubs = bucket_size[kHubBucket];
-    hub_slot_of.release();
-    hub_tab_off.release();
-    hub_tab_log2.release();
-    hub_slot.release();
-    hub_part_s.release();
-    hub_part_c.release();
-    hub_self.release();
     if (n_hubs > 0) {
         build_static_lists(s);
         std::vector<uint32_t> hubs(n_hubs);
+        std::vector<uint64_t> hdeg(n_hubs);
+        uint64_t *hdeg_dev = ws.get<uint64_t>(kSetupSlot, n_hubs);
+        gather_degrees(g.neighboorhood_sum.data(), list_buf[0].data() + bucket_off[kHubBucket],
+                       n_hubs, hdeg_dev, s);
         LV_HIP(hipMemcpyAsync(hubs.data(), list_buf[0].data() + bucket_off[kHubBucket],
                               n_hubs * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        std::vector<uint64_t> rowh(g.n_nodes + 1);
-        LV_HIP(hipMemcpyAsync(rowh.data(), g.neighboorhood_sum.data(),
-                              (g.n_nodes + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        LV_HIP(hipMemcpyAsync(hdeg.data(), hdeg_dev, n_hubs * sizeof(uint64_t),
+                              hipMemcpyDeviceToHost, s));
         LV_HIP(hipStreamSynchronize(s));
         std::vector<uint64_t> toff(n_hubs);
         std::vector<uint32_t> tlog(n_hubs);
         uint64_t pool = 0;
         std::vector<uint32_t> slot_pairs;  // (vertex, slot)
         for (uint32_t i = 0; i < n_hubs; ++i) {
-            const uint64_t d = rowh[hubs[i] + 1] - rowh[hubs[i]];
+            const uint64_t d = hdeg[i];
             uint32_t lg = 1;
             while ((1ull << lg) < 2 * d) ++lg;
             tlog[i] = lg;
             toff[i] = pool;
             pool += 1ull << lg;
         }
-        hub_slot_of.resize(n);
+        hub_slot_of.ensure(n);
         fill_u32(hub_slot_of.data(), LV_EMPTY, n, s);
-        hub_tab_off.resize(n_hubs);
-        hub_tab_log2.resize(n_hubs);
-        hub_slot.resize(pool);
-        hub_part_s.resize((size_t)n_hubs * kHubParts);
-        hub_part_c.resize((size_t)n_hubs * kHubParts);
-        hub_self.resize(n_hubs);
+        hub_tab_off.ensure(n_hubs);
+        hub_tab_log2.ensure(n_hubs);
+        hub_slot.ensure(pool);
+        hub_part_s.ensure((size_t)n_hubs * kHubParts);
+        hub_part_c.ensure((size_t)n_hubs * kHubParts);
+        hub_self.ensure(n_hubs);
         fill_u64(hub_slot.data(), kHubEmpty, pool, s);
         LV_HIP(hipMemcpyAsync(hub_tab_off.data(), toff.data(), n_hubs * sizeof(uint64_t),
                               hipMemcpyHostToDevice, s));
@@ -111,12 +103,13 @@ void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
         std::vector<uint32_t> slots_host(n_hubs);
         for (uint32_t i = 0; i < n_hubs; ++i) slots_host[i] = i;
         // slot_of[hubs[i]] = i
-        DeviceArray<uint32_t> hub_ids(n_hubs), hub_slots(n_hubs);
-        LV_HIP(hipMemcpyAsync(hub_ids.data(), hubs.data(), n_hubs * sizeof(uint32_t),
+        uint32_t *hub_ids = ws.get<uint32_t>(kSetupSlot + 1, n_hubs);
+        uint32_t *hub_slots = ws.get<uint32_t>(kSetupSlot + 2, n_hubs);
+        LV_HIP(hipMemcpyAsync(hub_ids, hubs.data(), n_hubs * sizeof(uint32_t),
                               hipMemcpyHostToDevice, s));
-        LV_HIP(hipMemcpyAsync(hub_slots.data(), slots_host.data(), n_hubs * sizeof(uint32_t),
+        LV_HIP(hipMemcpyAsync(hub_slots, slots_host.data(), n_hubs * sizeof(uint32_t),
                               hipMemcpyHostToDevice, s));
-        scatter_u32(hub_slot_of.data(), hub_ids.data(), hub_slots.data(), n_hubs, s);
+        scatter_u32(hub_slot_of.data(), hub_ids, hub_slots, n_hubs, s);
         LV_HIP(hipStreamSynchronize(s));
     }
 }
@@ -252,6 +245,7 @@ GraphDevice GraphDevice::view_of(const GraphDevice &g) {
 
 void Community::reset() {
     graph = GraphDevice::view_of(graph0);
+    state->next_level_buf = 0;
     level = 0;
     n_of_best_communities = graph.n_nodes;
     iota_u32(best_communities.data(), graph.n_nodes, stream());
@@ -321,8 +315,8 @@ void Community::update(int64_t sigma_in_carry, uint64_t sum_tot_sq_carry) {
 void Community::update_level(bool carry, int64_t sigma_in_carry, uint64_t sum_tot_sq_carry) {
     const uint32_t n = graph.n_nodes;
     hipStream_t s = stream();
-    communities.resize(std::max<size_t>(n, 1));
-    communities_weight.resize(std::max<size_t>(n, 1));
+    communities.ensure(std::max<size_t>(n, 1));
+    communities_weight.ensure(std::max<size_t>(n, 1));
     iota_u32(communities.data(), n, s);                               // community.h:58-60
     if (n)
         LV_HIP(hipMemcpyAsync(communities_weight.data(), graph.tot_weight_per_nodes.data(),
@@ -616,13 +610,15 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
     const uint32_t n = c.graph.n_nodes;
     const uint64_t E = c.graph.n_entries;
     const LevelView g = level_view(c.graph);
-    GraphDevice ng;
+    // the contracted graph goes to the level buffer not holding the current level
+    GraphDevice &ng = c.state->level_buf[c.state->next_level_buf];
+    c.state->next_level_buf ^= 1;
     ng.n_nodes = n_new;
     ng.total_weight = c.graph.total_weight;                           // :172
     ng.m2 = c.graph.m2;
     ng.unit_weights = false;
-    ng.neighboorhood_sum.resize((size_t)n_new + 1);
-    ng.tot_weight_per_nodes.resize(std::max<size_t>(n_new, 1));
+    ng.neighboorhood_sum.ensure((size_t)n_new + 1);
+    ng.tot_weight_per_nodes.ensure(std::max<size_t>(n_new, 1));
     // 1. members of each new community: vertices sorted by new id
     uint32_t *newc = ws.get<uint32_t>(W_NEWC, n);
     uint64_t *vk0 = ws.get<uint64_t>(W_VK0, n), *vk1 = ws.get<uint64_t>(W_VK1, n);
@@ -674,13 +670,13 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
         std::vector<uint32_t> hubs(nh);
         LV_HIP(hipMemcpyAsync(hubs.data(), lp[kHubBucket], nh * sizeof(uint32_t),
                               hipMemcpyDeviceToHost, s));
-        std::vector<uint64_t> crow_h((size_t)n_new + 1);
-        LV_HIP(hipMemcpyAsync(crow_h.data(), crow, ((size_t)n_new + 1) * sizeof(uint64_t),
-                              hipMemcpyDeviceToHost, s));
+        std::vector<uint64_t> hsize(nh);
+        uint64_t *hsize_dev = ws.get<uint64_t>(W_HOFF, (size_t)nh + 1);
+        gather_degrees(crow, lp[kHubBucket], nh, hsize_dev, s);
+        LV_HIP(hipMemcpyAsync(hsize.data(), hsize_dev, nh * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
         LV_HIP(hipStreamSynchronize(s));
         std::vector<uint64_t> hoff_h((size_t)nh + 1, 0);
-        for (uint32_t i = 0; i < nh; ++i)
-            hoff_h[i + 1] = hoff_h[i] + (crow_h[hubs[i] + 1] - crow_h[hubs[i]]);
+        for (uint32_t i = 0; i < nh; ++i) hoff_h[i + 1] = hoff_h[i] + hsize[i];
         const uint64_t H = hoff_h[nh];
         uint64_t *hoff = ws.get<uint64_t>(W_HOFF, (size_t)nh + 1);
         LV_HIP(hipMemcpyAsync(hoff, hoff_h.data(), ((size_t)nh + 1) * sizeof(uint64_t),
@@ -716,15 +712,15 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
     uint64_t e_new = 0;
     LV_HIP(hipMemcpyAsync(&e_new, orow + n_new, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     LV_HIP(hipStreamSynchronize(s));
-    ng.edge_destination.resize(std::max<size_t>(e_new, 1));
-    ng.weights.resize(std::max<size_t>(e_new, 1));
+    ng.edge_destination.ensure(std::max<size_t>(e_new, 1));
+    ng.weights.ensure(std::max<size_t>(e_new, 1));
     agg_copy(x, lp, cnt, hub_parts, orow, ng.edge_destination.data(), ng.weights.data(), ustart,
              ukey, wsum0, wsum1, (uint32_t)((1ull << bd) - 1), s);
     new_degrees(c.communities_weight.data(), map, n, ng.tot_weight_per_nodes.data(), s);
     LV_HIP(hipStreamSynchronize(s));
     ng.n_entries = e_new;
     ng.n_links = e_new / 2;                                           // :97
-    c.graph = std::move(ng);
+    c.graph = GraphDevice::view_of(ng);
 }
 
 // ---------------------------------------------------- ModularityAlgorithms

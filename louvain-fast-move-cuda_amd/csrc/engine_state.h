@@ -46,6 +46,8 @@ struct Workspace {
     }
 };
 
+constexpr size_t kSetupSlot = 40;   // workspace slots used by level setup
+
 struct SweepState {
     uint32_t n = 0;
     DeviceArray<uint8_t> bucket;
@@ -70,6 +72,10 @@ struct SweepState {
     DeviceArray<uint32_t> hub_part_c;
     DeviceArray<uint32_t> hub_self;
     Workspace ws;
+    // two reusable storages for contracted levels (level l+1 is built while
+    // level l, held in the other one or in the resident level-0 graph, is read)
+    GraphDevice level_buf[2];
+    int next_level_buf = 0;
     // profiling
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     KernelStatsAcc acc;

@@ -79,6 +79,9 @@ void fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipStream_t
 void iota_u32(uint32_t *p, size_t n, hipStream_t s);
 // dst[idx[i]] = val[i]
 void scatter_u32(uint32_t *dst, const uint32_t *idx, const uint32_t *val, size_t n, hipStream_t s);
+// deg[i] = row[ids[i]+1] - row[ids[i]]
+void gather_degrees(const uint64_t *row, const uint32_t *ids, uint32_t n, uint64_t *deg,
+                    hipStream_t s);
 // tot[c] = sum of kdeg over vertices with comm == c
 void community_totals(const uint32_t *comm, const uint32_t *kdeg, uint32_t n, uint32_t *tot,
                       hipStream_t s);

@@ -62,6 +62,13 @@ __global__ void k_scatter_u32(uint32_t *dst, const uint32_t *idx, const uint32_t
         dst[idx[i]] = val[i];
 }
 
+__global__ void k_gather_degrees(const uint64_t *row, const uint32_t *ids, uint32_t n,
+                                 uint64_t *deg) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        deg[i] = row[ids[i] + 1] - row[ids[i]];
+}
+
 __global__ void k_community_totals(const uint32_t *comm, const uint32_t *kdeg, uint32_t n,
                                    uint32_t *tot) {
     for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n;
@@ -869,6 +876,13 @@ void scatter_u32(uint32_t *dst, const uint32_t *idx, const uint32_t *val, size_t
     if (!n) return;
     hipLaunchKernelGGL(k_scatter_u32, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, dst, idx,
                        val, n);
+}
+
+void gather_degrees(const uint64_t *row, const uint32_t *ids, uint32_t n, uint64_t *deg,
+                    hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_gather_degrees, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, row, ids, n,
+                       deg);
 }
 
 void community_totals(const uint32_t *comm, const uint32_t *kdeg, uint32_t n, uint32_t *tot,
