@@ -66,6 +66,8 @@ def main():
     ap.add_argument("--threshold", type=float, default=0.001)
     ap.add_argument("--cpu-sweeps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe-bytes", type=int, default=1 << 30,
+                    help="size of the plain copy kernel run as the bandwidth reference")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -75,6 +77,8 @@ def main():
     import louvain_amd as L
     if L.device_count() < 1:
         raise SystemExit("bench.py: no GPU visible")
+    copy_gbs = L.bandwidth_probe(args.probe_bytes, 5) if args.probe_bytes > 0 else None
+    log(f"plain copy kernel: {copy_gbs} GB/s")
     t = time.perf_counter()
     ea, eb, n = L.gen_rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=2)
     log(f"rmat{args.scale}: {len(ea)} undirected edges, n={n} ({time.perf_counter() - t:.1f}s)")
@@ -107,10 +111,26 @@ def main():
     total = sum(times)
     traversed = sum(r["traversed_entries"] for r in results)
     last = results[-1]
-    ks = {k: sum(s[k] for s in kstats) for k in kstats[0]}
+    ks = {k: (sum(s[k] for s in kstats) if not isinstance(kstats[0][k], list)
+              else [sum(s[k][i] for s in kstats) for i in range(len(kstats[0][k]))])
+          for k in kstats[0]}
     move_s = ks["move_ms"] / 1e3
-    achieved = ks["move_bytes"] / move_s / 1e9 if move_s > 0 else 0.0
+    pass_gbs = ks["move_bytes"] / move_s / 1e9 if move_s > 0 else 0.0
+    # dominant best-move kernel: the variant (one kernel launch per sweep) with
+    # the most event-timed time over the timed runs
+    vi = max(range(len(ks["variant_ms"])), key=lambda i: ks["variant_ms"][i])
+    v_launch = ks["variant_launches"][vi]
+    v_ms = ks["variant_ms"][vi] / max(1, v_launch)
+    v_bytes = ks["variant_bytes"][vi] / max(1, v_launch)
+    achieved = v_bytes / (v_ms / 1e3) / 1e9 if v_ms > 0 else 0.0
     sw0 = last["sweeps"][0]
+    # hardware traffic of the dominant kernel from the committed counter runs
+    traffic = None
+    tf = os.path.join(REPO, "profiles", "k_move_table_256_2048_rmat22_traffic.json")
+    if os.path.exists(tf) and args.scale == 22:
+        t = json.load(open(tf))
+        if t.get("kernel") == L.kernel_variant_name(vi):
+            traffic = t["traffic_bytes_per_launch"]
     out = {
         "metric": METRIC,
         "value": traversed / total,
@@ -135,11 +155,14 @@ def main():
         "sweeps": last["total_sweeps"],
         "sweep0_entries": sw0["e_active"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "best-move pass (all degree-bucket kernels of a sweep)",
-                     "launches": ks["move_launches"],
-                     "avg_launch_ms": ks["move_ms"] / max(1, ks["move_launches"]),
-                     "avg_bytes": ks["move_bytes"] / max(1, ks["move_launches"])},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": L.kernel_variant_name(vi),
+                     "launches": v_launch, "avg_launch_ms": v_ms, "avg_bytes": v_bytes,
+                     "copy_kernel_gbs": copy_gbs,
+                     "best_move_pass": {"achieved_gbs": pass_gbs,
+                                        "frac": pass_gbs / HBM_PEAK_GBS,
+                                        "ms_per_run": ks["move_ms"] / args.steps,
+                                        "sweeps_per_run": ks["move_launches"] / args.steps}},
     }
     if not args.no_cpu:
         row, col, _ = g.csr()

@@ -99,6 +99,12 @@ typedef struct lv_kernel_stats {
     uint64_t move_entries;      /* entries traversed by those launches */
     double sweep_ms;            /* whole sweeps (move + commit + modularity), event timed */
     uint64_t sweeps;
+    /* per best-move kernel variant v = (bucket*2 + fast)*2 + weighted, bucket 0-7
+     * (one kernel launch each; lv_kernel_variant_name gives the kernel name) */
+    uint64_t variant_launches[32];
+    double variant_ms[32];
+    uint64_t variant_bytes[32];
+    uint64_t variant_entries[32];
 } lv_kernel_stats;
 
 typedef struct lv_graph lv_graph;          /* GraphHost (graphHost.cuh:7-18): host CSR */
@@ -164,6 +170,13 @@ lv_status lv_community_modularity_parts(lv_community *c, int64_t *sigma_in, uint
                                         double *q);
 lv_status lv_kernel_stats_get(const lv_community *c, lv_kernel_stats *s);
 lv_status lv_last_run_kernel_stats(lv_kernel_stats *s);
+/* Device copy bandwidth (read + write bytes per second / 1e9) of `reps`
+ * launches of the plain copy kernel k_stream_copy over `bytes`: the measured
+ * ceiling next to the nominal 8 TB/s, and the known-size reference for
+ * FETCH_SIZE/WRITE_SIZE counters. */
+lv_status lv_bandwidth_probe(uint64_t bytes, int reps, double *gbs);
+/* Kernel name of best-move variant v for gain mode `gain` (as rocprofv3 lists it). */
+lv_status lv_kernel_variant_name(int variant, int gain, char *buf, size_t len);
 
 /* Seeded synthetic graph generators (host, multithreaded; counter-based PRNG,
  * so the output is independent of the thread count).  Edges are written as

@@ -41,7 +41,7 @@ void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
     list_buf[0].ensure(std::max<size_t>(n, 1));
     list_buf[1].ensure(std::max<size_t>(n, 1));
     counts_dev.ensure(2 * kNumBuckets);
-    stats.ensure((size_t)kNumStats * kStatSlots);
+    stats.ensure((size_t)(kNumBuckets + 1) * kSection);
     if (!host_stats) {
         LV_HIP(hipHostMalloc((void **)&host_stats, kHostWords * sizeof(unsigned long long),
                              hipHostMallocDefault));
@@ -156,7 +156,19 @@ LevelView level_view(const GraphDevice &g) {
 
 uint64_t sum_slots(const unsigned long long *h, int stat) {
     uint64_t s = 0;
-    for (int i = 0; i < kStatSlots; ++i) s += h[stat * kStatSlots + i];
+    for (int i = 0; i < kStatSlots; ++i) s += h[kGlobalSection + stat * kStatSlots + i];
+    return s;
+}
+
+uint64_t bucket_slots(const unsigned long long *h, int bucket, int stat) {
+    uint64_t s = 0;
+    for (int i = 0; i < kStatSlots; ++i) s += h[(size_t)bucket * kSection + stat * kStatSlots + i];
+    return s;
+}
+
+uint64_t all_bucket_slots(const unsigned long long *h, int stat) {
+    uint64_t s = 0;
+    for (int b = 0; b < kNumBuckets; ++b) s += bucket_slots(h, b, stat);
     return s;
 }
 
@@ -332,10 +344,9 @@ void Community::update_level(bool carry, int64_t sigma_in_carry, uint64_t sum_to
     // identity partition of a level-0 graph (rows sorted by destination):
     // sigma_in = self-loop weight, sum of tot^2 = sum of k^2
     fill_u64(state->stats.data(), 0ull, state->stats.size(), s);
-    identity_parts(level_view(graph), state->stats.data(), s);
+    identity_parts(level_view(graph), state->stats.data() + kGlobalSection, s);
     LV_HIP(hipMemcpyAsync(state->host_stats, state->stats.data(),
-                          (size_t)kNumStats * kStatSlots * sizeof(unsigned long long),
-                          hipMemcpyDeviceToHost, s));
+                          kStatWordsOffset * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     LV_HIP(hipStreamSynchronize(s));
     sigma_in = (int64_t)sum_slots(state->host_stats, ST_DSIGMA);
     sum_tot_sq = sum_slots(state->host_stats, ST_SUMSQ);
@@ -351,10 +362,9 @@ void Community::compute_modularity() {
     hipStream_t s = stream();
     fill_u64(state->stats.data(), 0ull, state->stats.size(), s);
     modularity_parts(level_view(graph), communities.data(), communities_weight.data(),
-                     state->stats.data(), s);
+                     state->stats.data() + kGlobalSection, s);
     LV_HIP(hipMemcpyAsync(state->host_stats, state->stats.data(),
-                          (size_t)kNumStats * kStatSlots * sizeof(unsigned long long),
-                          hipMemcpyDeviceToHost, s));
+                          kStatWordsOffset * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     LV_HIP(hipStreamSynchronize(s));
     sigma_in = (int64_t)sum_slots(state->host_stats, ST_DSIGMA);
     sum_tot_sq = sum_slots(state->host_stats, ST_SUMSQ);
@@ -396,7 +406,9 @@ void launch_sweep(Community &c, const lv_params &p, int kind, bool apply, uint32
     a.tot = c.communities_weight.data();
     a.mark = st.mark.data();
     a.bucket = st.bucket.data();
-    a.stats = st.stats.data();
+    a.stats = st.stats.data() + kGlobalSection;
+    a.bstats = st.stats.data();
+    a.bev = p.profile ? st.bev : nullptr;
     a.m2 = (int64_t)(2.0 * c.graph.total_weight);
     a.tw = c.graph.total_weight;
     a.v_end = v_end;
@@ -417,8 +429,7 @@ void launch_sweep(Community &c, const lv_params &p, int kind, bool apply, uint32
     }
     if (p.profile) LV_HIP(hipEventRecord(st.ev[2], s));
     LV_HIP(hipMemcpyAsync(st.host_stats, st.stats.data(),
-                          (size_t)kNumStats * kStatSlots * sizeof(unsigned long long),
-                          hipMemcpyDeviceToHost, s));
+                          kStatWordsOffset * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
 }
 
 }  // namespace lv
@@ -444,9 +455,8 @@ int OptimizationPhase::run(Community &c, const lv_params &p, lv_sweep_rec *rec, 
         launch_sweep(c, p, kind, true, v_end);
         LV_HIP(hipStreamSynchronize(s));
         const unsigned long long *h = st.host_stats;
-        const uint64_t e_active = sum_slots(h, ST_E_ACTIVE);
-        const uint64_t considered = sum_slots(h, ST_CONSIDERED);
-        const uint64_t cand = sum_slots(h, ST_CANDIDATES);
+        const uint64_t e_active = all_bucket_slots(h, ST_E_ACTIVE);
+        const uint64_t considered = all_bucket_slots(h, ST_CONSIDERED);
         const uint64_t moved = sum_slots(h, ST_MOVED);
         const int64_t dsig = (int64_t)sum_slots(h, ST_DSIGMA);
         const uint64_t sq = sum_slots(h, ST_SUMSQ);
@@ -457,9 +467,32 @@ int OptimizationPhase::run(Community &c, const lv_params &p, lv_sweep_rec *rec, 
             LV_HIP(hipEventElapsedTime(&ms_move, st.ev[0], st.ev[1]));
             LV_HIP(hipEventElapsedTime(&ms_sweep, st.ev[0], st.ev[2]));
             const bool fast = kind == LV_SWEEP_FAST;
-            const uint64_t per_entry = (fast ? 4u : 8u) + (c.graph.unit_weights ? 0u : 4u);
-            const uint64_t bytes = per_entry * e_active + 4u * cand +
-                                   (fast ? 28u : 32u) * (uint64_t)n_active + 4u * moved;
+            const bool wtd = !c.graph.unit_weights;
+            // algorithmic bytes of a best-move launch: per entry col (+comm[x]
+            // unless fast, +w if weighted), per candidate tot[c], per listed
+            // vertex list+row+k+tot_own (+comm[v] unless fast), per mover target
+            auto bytes_of = [&](uint64_t ent, uint64_t cnd, uint64_t verts, uint64_t tw) {
+                const uint64_t per_entry = (fast ? 4u : 8u) + (wtd ? 4u : 0u);
+                return per_entry * ent + 4u * cnd + (fast ? 28u : 32u) * verts + 4u * tw;
+            };
+            uint64_t bytes = 0;
+            for (int b = 0; b < kNumBuckets; ++b) {
+                const uint32_t verts = st.count[b];
+                if (!verts) continue;
+                const uint64_t be = bucket_slots(h, b, ST_E_ACTIVE);
+                const uint64_t bc = bucket_slots(h, b, ST_CANDIDATES);
+                const uint64_t bt = bucket_slots(h, b, ST_TWRITES);
+                const uint64_t bb = bytes_of(be, bc, verts, bt);
+                bytes += bb;
+                if (b == kHubBucket) continue;
+                float ms_b = 0;
+                LV_HIP(hipEventElapsedTime(&ms_b, st.bev[2 * b], st.bev[2 * b + 1]));
+                const int vi = move_variant(b, fast, wtd);
+                st.acc.var_launches[vi] += 1;
+                st.acc.var_ms[vi] += ms_b;
+                st.acc.var_bytes[vi] += bb;
+                st.acc.var_entries[vi] += be;
+            }
             st.acc.move_launches += 1;
             st.acc.move_ms += ms_move;
             st.acc.move_bytes += bytes;

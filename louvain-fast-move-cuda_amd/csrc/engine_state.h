@@ -13,6 +13,12 @@
 
 namespace lv {
 
+// best-move kernel variants timed one by one: (bucket 0-7, fast, weighted)
+constexpr int kMoveVariants = 32;
+inline int move_variant(int bucket, bool fast, bool weighted) {
+    return (bucket * 2 + (fast ? 1 : 0)) * 2 + (weighted ? 1 : 0);
+}
+
 struct KernelStatsAcc {
     uint64_t move_launches = 0;
     double move_ms = 0;
@@ -20,13 +26,20 @@ struct KernelStatsAcc {
     uint64_t move_entries = 0;
     double sweep_ms = 0;
     uint64_t sweeps = 0;
+    uint64_t var_launches[kMoveVariants] = {};
+    double var_ms[kMoveVariants] = {};
+    uint64_t var_bytes[kMoveVariants] = {};
+    uint64_t var_entries[kMoveVariants] = {};
 };
 
 extern KernelStatsAcc g_last_run_stats;
 
-// host_stats layout: kNumStats*kStatSlots u64 counters, then kNumBuckets u32
-// next-list sizes
-constexpr size_t kStatWordsOffset = (size_t)kNumStats * kStatSlots;
+// stats layout: one section of kNumStats*kStatSlots counters per degree bucket
+// (best-move kernels) and a last, global section (commit, finalize,
+// modularity); host_stats mirrors it and then holds kNumBuckets u32 list sizes.
+constexpr size_t kSection = (size_t)kNumStats * kStatSlots;
+constexpr size_t kGlobalSection = (size_t)kNumBuckets * kSection;
+constexpr size_t kStatWordsOffset = (size_t)(kNumBuckets + 1) * kSection;
 constexpr size_t kHostWords = kStatWordsOffset + kNumBuckets;
 
 // Grow-only device scratch: one buffer per slot, reused across levels and
@@ -78,10 +91,12 @@ struct SweepState {
     int next_level_buf = 0;
     // profiling
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t bev[2 * kNumBuckets] = {};
     KernelStatsAcc acc;
 
     SweepState() {
         for (auto &e : ev) LV_HIP(hipEventCreate(&e));
+        for (auto &e : bev) LV_HIP(hipEventCreate(&e));
     }
     ~SweepState();
     SweepState(const SweepState &) = delete;
@@ -94,7 +109,9 @@ struct SweepState {
 };
 
 LevelView level_view(const GraphDevice &g);
-uint64_t sum_slots(const unsigned long long *h, int stat);
+uint64_t sum_slots(const unsigned long long *h, int stat);            // global section
+uint64_t bucket_slots(const unsigned long long *h, int bucket, int stat);
+uint64_t all_bucket_slots(const unsigned long long *h, int stat);
 void launch_sweep(struct ::Community &c, const lv_params &p, int kind, bool apply, uint32_t v_end);
 
 }  // namespace lv

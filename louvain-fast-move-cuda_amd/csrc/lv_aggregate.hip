@@ -25,6 +25,8 @@ namespace lv {
 
 namespace {
 
+constexpr int kAggIlp = 4;   // entries per lane with loads in flight together
+
 struct AggCtx {
     const uint64_t *row;
     const uint32_t *col;
@@ -75,7 +77,6 @@ template <int G, int T, bool WEIGHTED>
 __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *__restrict__ list,
                                                       uint32_t count) {
     constexpr int GPB = kBlock / G;
-    constexpr int LOG2T = ilog2(T);
     __shared__ uint32_t s_key[GPB * T];
     __shared__ uint32_t s_val[GPB * T];
     __shared__ uint32_t s_cnt[GPB];
@@ -87,11 +88,16 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
     const uint32_t c = valid ? list[idx] : 0u;
     uint32_t *K = s_key + gi * T;
     uint32_t *V = s_val + gi * T;
-    for (int s = lane; s < T; s += G) { K[s] = LV_EMPTY; V[s] = 0u; }
+    uint64_t k0 = 0;
+    uint32_t lg = 1;
+    if (valid) {
+        k0 = a.crow[c];
+        lg = table_log2(a.crow[c + 1] - k0);   // at most 2x the virtual entries, <= T
+    }
+    const uint32_t tv = 1u << lg, mask = tv - 1u;
+    for (uint32_t s = lane; s < tv; s += G) { K[s] = LV_EMPTY; V[s] = 0u; }
     if (lane == 0) { s_cnt[gi] = 0u; s_pos[gi] = 0u; }
     __syncthreads();
-    uint64_t k0 = 0;
-    if (valid) k0 = a.crow[c];
     // members staged G at a time: row start and first virtual entry in LDS
     __shared__ uint64_t s_mrow[GPB][G];
     __shared__ uint64_t s_mp[GPB][G + 1];
@@ -107,17 +113,33 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
         if (lane == 0) s_mp[gi][nm] = a.P[mb + nm];
         group_barrier<G>();
         const uint64_t e0 = s_mp[gi][0], e1 = s_mp[gi][nm];
-        for (uint64_t k = e0 + lane; k < e1; k += G) {
-            const uint32_t i = lds_member(s_mp[gi], nm, k);
-            const uint64_t j = s_mrow[gi][i] + (k - s_mp[gi][i]);
-            const uint32_t d = a.newc[a.col[j]];
-            const uint32_t w = WEIGHTED ? a.wt[j] : 1u;
-            uint32_t h = slot_hash<LOG2T>(d);
-            for (;;) {
-                const uint32_t prev = atomicCAS(&K[h], LV_EMPTY, d);
-                if (prev == LV_EMPTY) { atomicAdd(&s_cnt[gi], 1u); atomicAdd(&V[h], w); break; }
-                if (prev == d) { atomicAdd(&V[h], w); break; }
-                h = (h + 1) & (T - 1);
+        for (uint64_t kb = e0 + lane; kb < e1; kb += (uint64_t)G * kAggIlp) {
+            uint32_t xs[kAggIlp], ws[kAggIlp], ds[kAggIlp];
+#pragma unroll
+            for (int u = 0; u < kAggIlp; ++u) {
+                const uint64_t k = kb + (uint64_t)u * G;
+                xs[u] = LV_EMPTY;
+                ws[u] = 1u;
+                if (k < e1) {
+                    const uint32_t i = lds_member(s_mp[gi], nm, k);
+                    const uint64_t j = s_mrow[gi][i] + (k - s_mp[gi][i]);
+                    xs[u] = stream_load(&a.col[j]);
+                    if (WEIGHTED) ws[u] = stream_load(&a.wt[j]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kAggIlp; ++u) ds[u] = xs[u] != LV_EMPTY ? a.newc[xs[u]] : LV_EMPTY;
+#pragma unroll
+            for (int u = 0; u < kAggIlp; ++u) {
+                const uint32_t d = ds[u];
+                if (d == LV_EMPTY) continue;
+                uint32_t h = slot_hash_dyn(d, lg);
+                for (;;) {
+                    const uint32_t prev = atomicCAS(&K[h], LV_EMPTY, d);
+                    if (prev == LV_EMPTY) { atomicAdd(&s_cnt[gi], 1u); atomicAdd(&V[h], ws[u]); break; }
+                    if (prev == d) { atomicAdd(&V[h], ws[u]); break; }
+                    h = (h + 1) & mask;
+                }
             }
         }
         group_barrier<G>();
@@ -128,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
     // are put in destination order on the way out)
     if (valid) {
         if (lane == 0) a.rcount[c] = s_cnt[gi];
-        for (int s = lane; s < T; s += G) {
+        for (uint32_t s = lane; s < tv; s += G) {
             const uint32_t d = K[s];
             if (d == LV_EMPTY) continue;
             const uint32_t pos = atomicAdd(&s_pos[gi], 1u);

@@ -4,6 +4,7 @@
 #include "../../include/lv_api.h"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -294,7 +295,8 @@ lv_status lv_best_moves(lv_community *c, int kind, const lv_params *p, const uin
             a.tot = C.communities_weight.data();
             a.mark = st.mark.data();
             a.bucket = st.bucket.data();
-            a.stats = st.stats.data();
+            a.stats = st.stats.data() + lv::kGlobalSection;
+            a.bstats = st.stats.data();
             lv::fill_u64(st.stats.data(), 0ull, st.stats.size(), s);
             lv::finalize_pass(a, st.lists(st.cur ^ 1), st.counts_dev.data() + lv::kNumBuckets, s);
             uint32_t cnt[lv::kNumBuckets];
@@ -306,7 +308,7 @@ lv_status lv_best_moves(lv_community *c, int kind, const lv_params *p, const uin
         }
         lv::launch_sweep(C, P, kind, false, n);
         LV_HIP(hipStreamSynchronize(s));
-        if (e_active) *e_active = lv::sum_slots(st.host_stats, lv::ST_E_ACTIVE);
+        if (e_active) *e_active = lv::all_bucket_slots(st.host_stats, lv::ST_E_ACTIVE);
         LV_HIP(hipMemcpy(target, st.target.data(), n * sizeof(uint32_t), hipMemcpyDeviceToHost));
         // restore target == comm
         LV_HIP(hipMemcpyAsync(st.target.data(), C.communities.data(), n * sizeof(uint32_t),
@@ -333,6 +335,50 @@ static void fill_stats(lv_kernel_stats *s, const lv::KernelStatsAcc &a) {
     s->move_entries = a.move_entries;
     s->sweep_ms = a.sweep_ms;
     s->sweeps = a.sweeps;
+    for (int i = 0; i < lv::kMoveVariants; ++i) {
+        s->variant_launches[i] = a.var_launches[i];
+        s->variant_ms[i] = a.var_ms[i];
+        s->variant_bytes[i] = a.var_bytes[i];
+        s->variant_entries[i] = a.var_entries[i];
+    }
+}
+
+lv_status lv_bandwidth_probe(uint64_t bytes, int reps, double *gbs) {
+    if (bytes < 16 || reps < 1 || !gbs) return LV_ERR_ARG;
+    const lv_status d = need_device();
+    if (d != LV_OK) return d;
+    return guarded([&] {
+        bytes &= ~(uint64_t)15;
+        lv::DeviceArray<uint8_t> a(bytes), b(bytes);
+        hipStream_t s;
+        LV_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        hipEvent_t e0, e1;
+        LV_HIP(hipEventCreate(&e0));
+        LV_HIP(hipEventCreate(&e1));
+        lv::fill_u32(reinterpret_cast<uint32_t *>(a.data()), 1u, bytes / 4, s);
+        lv::stream_copy(a.data(), b.data(), bytes, s);   // warm
+        LV_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r) lv::stream_copy(a.data(), b.data(), bytes, s);
+        LV_HIP(hipEventRecord(e1, s));
+        LV_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        LV_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;   // read + write
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+    });
+}
+
+lv_status lv_kernel_variant_name(int variant, int gain, char *buf, size_t len) {
+    if (variant < 0 || variant >= lv::kMoveVariants || !buf || len == 0) return LV_ERR_ARG;
+    static const int G[8] = {4, 8, 16, 32, 64, 64, 256, 256};
+    static const int T[8] = {8, 16, 32, 64, 128, 512, 2048, 8192};
+    const int b = variant / 4, fast = (variant / 2) % 2, wtd = variant % 2;
+    const char *w = wtd ? "true" : "false";
+    if (fast) std::snprintf(buf, len, "k_move_fast<%d, %d, %s>", G[b], gain, w);
+    else std::snprintf(buf, len, "k_move_table<%d, %d, %d, %s>", G[b], T[b], gain, w);
+    return LV_OK;
 }
 
 lv_status lv_kernel_stats_get(const lv_community *c, lv_kernel_stats *s) {

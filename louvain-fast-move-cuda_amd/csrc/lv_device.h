@@ -60,30 +60,45 @@ __device__ inline uint64_t group_sum_u64(uint64_t x) {
     return x;
 }
 
-// Block-wide sum of up to 3 counters; thread 0 adds them to the spread slots.
+// Block-wide sum of up to 4 counters; thread 0 adds them to the spread slots
+// (a negative stat index skips a counter).
 __device__ inline void block_add_stats(unsigned long long *stats, int st0, uint64_t a0, int st1,
-                                       uint64_t a1, int st2, uint64_t a2) {
-    __shared__ uint64_t red[3][kBlock / 64];
+                                       uint64_t a1, int st2, uint64_t a2, int st3 = -1,
+                                       uint64_t a3 = 0) {
+    __shared__ uint64_t red[4][kBlock / 64];
     a0 = group_sum_u64<64>(a0);
     a1 = group_sum_u64<64>(a1);
     a2 = group_sum_u64<64>(a2);
+    a3 = group_sum_u64<64>(a3);
     const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { red[0][w] = a0; red[1][w] = a1; red[2][w] = a2; }
+    if ((threadIdx.x & 63) == 0) { red[0][w] = a0; red[1][w] = a1; red[2][w] = a2; red[3][w] = a3; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint64_t s0 = 0, s1 = 0, s2 = 0;
-        for (int i = 0; i < kBlock / 64; ++i) { s0 += red[0][i]; s1 += red[1][i]; s2 += red[2][i]; }
+        uint64_t t[4] = {0, 0, 0, 0};
+        for (int i = 0; i < kBlock / 64; ++i)
+            for (int k = 0; k < 4; ++k) t[k] += red[k][i];
+        const int st[4] = {st0, st1, st2, st3};
         const int slot = blockIdx.x & (kStatSlots - 1);
-        if (st0 >= 0 && s0) atomicAdd(&stats[st0 * kStatSlots + slot], (unsigned long long)s0);
-        if (st1 >= 0 && s1) atomicAdd(&stats[st1 * kStatSlots + slot], (unsigned long long)s1);
-        if (st2 >= 0 && s2) atomicAdd(&stats[st2 * kStatSlots + slot], (unsigned long long)s2);
+        for (int k = 0; k < 4; ++k)
+            if (st[k] >= 0 && t[k]) atomicAdd(&stats[st[k] * kStatSlots + slot], (unsigned long long)t[k]);
     }
     __syncthreads();
 }
 
+// Adjacency streams are read once per pass: non-temporal loads keep them from
+// evicting the gathered comm/tot lines from L2.
+__device__ inline uint32_t stream_load(const uint32_t *p) { return __builtin_nontemporal_load(p); }
+
 template <int LOG2T>
 __device__ inline uint32_t slot_hash(uint32_t c) {
     return (c * 2654435761u) >> (32 - LOG2T);
+}
+
+// log2 of the smallest power of two >= 2*d (at least 2 slots): an
+// open-addressing table for d keys then stays at most half full.
+__device__ inline uint32_t table_log2(uint64_t d) {
+    const uint64_t need = 2 * d;
+    return need <= 2 ? 1u : (uint32_t)(64 - __builtin_clzll(need - 1));
 }
 
 __device__ inline uint32_t slot_hash_dyn(uint32_t c, uint32_t log2t) {

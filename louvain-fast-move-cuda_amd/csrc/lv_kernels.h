@@ -33,7 +33,7 @@ enum Stat : int {
     ST_MOVED = 3,        // vertices that changed community
     ST_DSIGMA = 4,       // change of sigma_in (two's complement)
     ST_SUMSQ = 5,        // sum over communities of tot^2
-    ST_VERTS = 6,        // examined vertices
+    ST_TWRITES = 6,      // target writes of the best-move pass (movers)
     kNumStats = 7
 };
 
@@ -63,7 +63,9 @@ struct SweepArgs {
     uint32_t *tot;
     uint8_t *mark;
     const uint8_t *bucket;
-    unsigned long long *stats;   // [kNumStats * kStatSlots]
+    unsigned long long *stats;   // [kNumStats * kStatSlots] counters of commit/finalize
+    unsigned long long *bstats;  // [kNumBuckets * kNumStats * kStatSlots] best-move counters per bucket
+    hipEvent_t *bev;             // nullptr, or 2 events per bucket around its best-move launch
     int64_t m2;                  // 2m as an integer
     double tw;                   // the reference's total_weight (m)
     uint32_t v_end;              // vertices >= v_end are not examined (chunk emulation)
@@ -79,6 +81,8 @@ void fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipStream_t
 void iota_u32(uint32_t *p, size_t n, hipStream_t s);
 // dst[idx[i]] = val[i]
 void scatter_u32(uint32_t *dst, const uint32_t *idx, const uint32_t *val, size_t n, hipStream_t s);
+// plain device copy of `bytes` (a multiple of 16) with k_stream_copy
+void stream_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 // deg[i] = row[ids[i]+1] - row[ids[i]]
 void gather_degrees(const uint64_t *row, const uint32_t *ids, uint32_t n, uint64_t *deg,
                     hipStream_t s);

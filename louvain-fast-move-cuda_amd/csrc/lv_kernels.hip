@@ -62,6 +62,14 @@ __global__ void k_scatter_u32(uint32_t *dst, const uint32_t *idx, const uint32_t
         dst[idx[i]] = val[i];
 }
 
+// Plain streaming copy (16 bytes per lane per step): the bandwidth probe and
+// the known-size reference for hardware traffic counters.
+__global__ __launch_bounds__(kBlock) void k_stream_copy(const uint4 *src, uint4 *dst, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_gather_degrees(const uint64_t *row, const uint32_t *ids, uint32_t n,
                                  uint64_t *deg) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
@@ -292,6 +300,7 @@ __device__ inline uint32_t src_of_entry(const uint64_t *row, uint32_t n, uint64_
 }
 
 constexpr int kChunk = 16;   // entries per thread in entry-parallel kernels
+constexpr int kIlp = 4;      // entries per lane with loads in flight together
 
 // sigma_in = sum of w over entries with comm[src]==comm[dst]; sum of tot^2.
 __global__ __launch_bounds__(kBlock) void k_modularity_parts(LevelView g, const uint32_t *comm,
@@ -360,7 +369,6 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
     using SC = Score<GAIN>;
     using S = typename SC::T;
     constexpr int GPB = kBlock / G;
-    constexpr int LOG2T = ilog2(T);
     __shared__ uint32_t s_key[GPB * T];
     __shared__ uint32_t s_val[GPB * T];
     const int lane = threadIdx.x & (G - 1);
@@ -371,24 +379,42 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
     const bool live = valid && v < a.v_end;
     uint32_t *K = s_key + gi * T;
     uint32_t *V = s_val + gi * T;
-    for (int s = lane; s < T; s += G) { K[s] = LV_EMPTY; V[s] = 0u; }
-    __syncthreads();
-    uint32_t own = 0;
-    uint64_t nself = 0;
+    // the vertex's table: the smallest power of two >= 2*degree (<= T)
+    uint32_t own = 0, lg = 1;
+    uint64_t beg = 0, end = 0;
     if (live) {
         own = a.comm[v];
-        const uint64_t beg = a.row[v], end = a.row[v + 1];
-        for (uint64_t j = beg + lane; j < end; j += G) {
-            const uint32_t x = a.col[j];
-            if (x == v) continue;
-            ++nself;
-            const uint32_t c = a.comm[x];
-            const uint32_t w = WEIGHTED ? a.wt[j] : 1u;
-            uint32_t h = slot_hash<LOG2T>(c);
-            for (;;) {
-                const uint32_t prev = atomicCAS(&K[h], LV_EMPTY, c);
-                if (prev == LV_EMPTY || prev == c) { atomicAdd(&V[h], w); break; }
-                h = (h + 1) & (T - 1);
+        beg = a.row[v];
+        end = a.row[v + 1];
+        lg = table_log2(end - beg);
+    }
+    const uint32_t tv = 1u << lg, mask = tv - 1u;
+    for (uint32_t s = lane; s < tv; s += G) { K[s] = LV_EMPTY; V[s] = 0u; }
+    __syncthreads();
+    uint64_t nself = 0;
+    if (live) {
+        // kIlp entries per lane in flight: loads first, then gathers, then inserts
+        for (uint64_t jb = beg + lane; jb < end; jb += (uint64_t)G * kIlp) {
+            uint32_t xs[kIlp], ws[kIlp], cs[kIlp];
+#pragma unroll
+            for (int u = 0; u < kIlp; ++u) {
+                const uint64_t j = jb + (uint64_t)u * G;
+                xs[u] = j < end ? stream_load(&a.col[j]) : v;
+                ws[u] = (WEIGHTED && j < end) ? stream_load(&a.wt[j]) : 1u;
+            }
+#pragma unroll
+            for (int u = 0; u < kIlp; ++u) cs[u] = xs[u] != v ? a.comm[xs[u]] : 0u;
+#pragma unroll
+            for (int u = 0; u < kIlp; ++u) {
+                if (xs[u] == v) continue;
+                ++nself;
+                const uint32_t c = cs[u];
+                uint32_t h = slot_hash_dyn(c, lg);
+                for (;;) {
+                    const uint32_t prev = atomicCAS(&K[h], LV_EMPTY, c);
+                    if (prev == LV_EMPTY || prev == c) { atomicAdd(&V[h], ws[u]); break; }
+                    h = (h + 1) & mask;
+                }
             }
         }
     }
@@ -398,31 +424,45 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
     uint64_t ncand = 0;
     uint32_t self = 0, kv = 0, tot_own = 0;
     if (live) {
-        uint32_t h = slot_hash<LOG2T>(own);
+        uint32_t h = slot_hash_dyn(own, lg);
         for (;;) {
             const uint32_t c = K[h];
             if (c == own) { self = V[h]; break; }
             if (c == LV_EMPTY) break;
-            h = (h + 1) & (T - 1);
+            h = (h + 1) & mask;
         }
         kv = a.kdeg[v];
         tot_own = a.tot[own];
-        for (int s = lane; s < T; s += G) {
-            const uint32_t c = K[s];
-            if (c == LV_EMPTY || c == own) continue;
-            ++ncand;
-            const S sc = SC::of(V[s], self, kv, tot_own, a.tot[c], a.m2, a.tw);
-            if (better_candidate(sc, c, best, best_c, a.highest)) { best = sc; best_c = c; }
+        for (uint32_t sb = lane; sb < tv; sb += (uint32_t)G * kIlp) {
+            uint32_t cs[kIlp], Ws[kIlp], ts[kIlp];
+#pragma unroll
+            for (int u = 0; u < kIlp; ++u) {
+                const uint32_t s2 = sb + (uint32_t)u * G;
+                const uint32_t c = s2 < tv ? K[s2] : LV_EMPTY;
+                cs[u] = c == own ? LV_EMPTY : c;
+                Ws[u] = cs[u] != LV_EMPTY ? V[s2] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < kIlp; ++u) ts[u] = cs[u] != LV_EMPTY ? a.tot[cs[u]] : 0u;
+#pragma unroll
+            for (int u = 0; u < kIlp; ++u) {
+                if (cs[u] == LV_EMPTY) continue;
+                ++ncand;
+                const S sc = SC::of(Ws[u], self, kv, tot_own, ts[u], a.m2, a.tw);
+                if (better_candidate(sc, cs[u], best, best_c, a.highest)) { best = sc; best_c = cs[u]; }
+            }
         }
     }
-    uint64_t considered = 0;
+    uint64_t considered = 0, twrite = 0;
     if constexpr (G <= 64) {
         group_best<G, S>(best, best_c, a.highest);
         const uint64_t gself = group_sum_u64<G>(nself);
         if (live && lane == 0) {
             considered = gself > 0;
-            if (best != SC::none() && SC::positive(best, self, kv, tot_own, a.m2))
+            if (best != SC::none() && SC::positive(best, self, kv, tot_own, a.m2)) {
                 a.target[v] = best_c;
+                twrite = 1;
+            }
         }
     } else {
         __shared__ S r_s[kBlock / 64];
@@ -443,11 +483,14 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
                 }
             }
             considered = tn > 0;
-            if (best != SC::none() && SC::positive(best, self, kv, tot_own, a.m2))
+            if (best != SC::none() && SC::positive(best, self, kv, tot_own, a.m2)) {
                 a.target[v] = best_c;
+                twrite = 1;
+            }
         }
     }
-    block_add_stats(a.stats, ST_E_ACTIVE, nself, ST_CANDIDATES, ncand, ST_CONSIDERED, considered);
+    block_add_stats(a.stats, ST_E_ACTIVE, nself, ST_CANDIDATES, ncand, ST_CONSIDERED, considered,
+                    ST_TWRITES, twrite);
 }
 
 // First sweep of a level (identity partition, phaseOptimizationSort.cu:28-158):
@@ -473,23 +516,35 @@ __global__ __launch_bounds__(kBlock) void k_move_fast(MoveCtx a, const uint32_t 
         kv = a.kdeg[v];
         tot_own = a.tot[v];
         const uint64_t beg = a.row[v], end = a.row[v + 1];
-        for (uint64_t j = beg + lane; j < end; j += G) {
-            const uint32_t x = a.col[j];
-            if (x == v) continue;
-            ++nself;
-            const uint32_t w = WEIGHTED ? a.wt[j] : 1u;
-            const S sc = SC::of(w, 0u, kv, tot_own, a.tot[x], a.m2, a.tw);
-            if (better_candidate(sc, x, best, best_c, false)) { best = sc; best_c = x; }
+        for (uint64_t jb = beg + lane; jb < end; jb += (uint64_t)G * kIlp) {
+            uint32_t xs[kIlp], ws[kIlp], ts[kIlp];
+#pragma unroll
+            for (int u = 0; u < kIlp; ++u) {
+                const uint64_t j = jb + (uint64_t)u * G;
+                xs[u] = j < end ? stream_load(&a.col[j]) : v;
+                ws[u] = (WEIGHTED && j < end) ? stream_load(&a.wt[j]) : 1u;
+            }
+#pragma unroll
+            for (int u = 0; u < kIlp; ++u) ts[u] = xs[u] != v ? a.tot[xs[u]] : 0u;
+#pragma unroll
+            for (int u = 0; u < kIlp; ++u) {
+                if (xs[u] == v) continue;
+                ++nself;
+                const S sc = SC::of(ws[u], 0u, kv, tot_own, ts[u], a.m2, a.tw);
+                if (better_candidate(sc, xs[u], best, best_c, false)) { best = sc; best_c = xs[u]; }
+            }
         }
     }
-    uint64_t considered = 0;
+    uint64_t considered = 0, twrite = 0;
     if constexpr (G <= 64) {
         group_best<G, S>(best, best_c, false);
         const uint64_t gself = group_sum_u64<G>(nself);
         if (live && lane == 0) {
             considered = gself > 0;
-            if (best != SC::none() && SC::positive(best, 0u, kv, tot_own, a.m2) && v < best_c)
+            if (best != SC::none() && SC::positive(best, 0u, kv, tot_own, a.m2) && v < best_c) {
                 a.target[v] = best_c;
+                twrite = 1;
+            }
         }
     } else {
         __shared__ S r_s[kBlock / 64];
@@ -510,11 +565,14 @@ __global__ __launch_bounds__(kBlock) void k_move_fast(MoveCtx a, const uint32_t 
                 }
             }
             considered = tn > 0;
-            if (best != SC::none() && SC::positive(best, 0u, kv, tot_own, a.m2) && v < best_c)
+            if (best != SC::none() && SC::positive(best, 0u, kv, tot_own, a.m2) && v < best_c) {
                 a.target[v] = best_c;
+                twrite = 1;
+            }
         }
     }
-    block_add_stats(a.stats, ST_E_ACTIVE, nself, ST_CANDIDATES, nself, ST_CONSIDERED, considered);
+    block_add_stats(a.stats, ST_E_ACTIVE, nself, ST_CANDIDATES, nself, ST_CONSIDERED, considered,
+                    ST_TWRITES, twrite);
 }
 
 // ---- hubs: kHubParts workgroups per vertex, pooled global tables ----
@@ -539,31 +597,43 @@ __global__ __launch_bounds__(kBlock) void k_hub_insert(HubCtx a, const uint32_t 
             const uint32_t mask = (1u << lg) - 1u;
             unsigned long long *S = a.h.slot + off;
             const uint64_t beg = a.m.row[v], end = a.m.row[v + 1];
-            for (uint64_t j = beg + (uint64_t)part * kBlock + threadIdx.x; j < end;
-                 j += (uint64_t)kHubParts * kBlock) {
-                const uint32_t x = a.m.col[j];
-                if (x == v) continue;
-                ++nself;
-                const uint32_t c = a.m.comm[x];
-                const uint32_t w = WEIGHTED ? a.m.wt[j] : 1u;
-                // one atomic per entry: add to the packed slot holding c, or
-                // claim an empty slot with (c << 32 | w)
-                uint32_t h = slot_hash_dyn(c, lg);
-                for (;;) {
-                    const unsigned long long cur =
-                        __hip_atomic_load(&S[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    const uint32_t kk = (uint32_t)(cur >> 32);
-                    if (kk == c) { atomicAdd(&S[h], (unsigned long long)w); break; }
-                    if (kk == LV_EMPTY) {
-                        const unsigned long long prev =
-                            atomicCAS(&S[h], kHubEmpty, ((unsigned long long)c << 32) | w);
-                        if (prev == kHubEmpty) break;
-                        if ((uint32_t)(prev >> 32) == c) {
-                            atomicAdd(&S[h], (unsigned long long)w);
-                            break;
+            const uint64_t stride = (uint64_t)kHubParts * kBlock;
+            for (uint64_t jb = beg + (uint64_t)part * kBlock + threadIdx.x; jb < end;
+                 jb += stride * kIlp) {
+                uint32_t xs[kIlp], ws[kIlp], cs[kIlp];
+#pragma unroll
+                for (int u = 0; u < kIlp; ++u) {
+                    const uint64_t j = jb + (uint64_t)u * stride;
+                    xs[u] = j < end ? stream_load(&a.m.col[j]) : v;
+                    ws[u] = (WEIGHTED && j < end) ? stream_load(&a.m.wt[j]) : 1u;
+                }
+#pragma unroll
+                for (int u = 0; u < kIlp; ++u) cs[u] = xs[u] != v ? a.m.comm[xs[u]] : 0u;
+#pragma unroll
+                for (int u = 0; u < kIlp; ++u) {
+                    if (xs[u] == v) continue;
+                    ++nself;
+                    const uint32_t c = cs[u];
+                    const uint32_t w = ws[u];
+                    // one atomic per entry: add to the packed slot holding c, or
+                    // claim an empty slot with (c << 32 | w)
+                    uint32_t h = slot_hash_dyn(c, lg);
+                    for (;;) {
+                        const unsigned long long cur =
+                            __hip_atomic_load(&S[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        const uint32_t kk = (uint32_t)(cur >> 32);
+                        if (kk == c) { atomicAdd(&S[h], (unsigned long long)w); break; }
+                        if (kk == LV_EMPTY) {
+                            const unsigned long long prev =
+                                atomicCAS(&S[h], kHubEmpty, ((unsigned long long)c << 32) | w);
+                            if (prev == kHubEmpty) break;
+                            if ((uint32_t)(prev >> 32) == c) {
+                                atomicAdd(&S[h], (unsigned long long)w);
+                                break;
+                            }
                         }
+                        h = (h + 1) & mask;
                     }
-                    h = (h + 1) & mask;
                 }
             }
         }
@@ -784,17 +854,26 @@ __global__ __launch_bounds__(kBlock) void k_commit(CommitCtx a, const uint32_t *
             }
             const uint64_t beg = a.row[v], end = a.row[v + 1];
             const uint64_t stride = (uint64_t)G * (G == kBlock ? parts : 1u);
-            for (uint64_t j = beg + (uint64_t)part * G + lane; j < end; j += stride) {
-                const uint32_t x = a.col[j];
-                if (x == v) continue;
-                const uint32_t ox = a.comm[x];
-                const uint32_t nx = a.target[x];
-                const int d = (int)(nx == c) - (int)(ox == own);
-                if (d) {
-                    const int64_t w = WEIGHTED ? (int64_t)a.wt[j] : 1;
-                    dsig += w * d * (ox != nx ? 1 : 2);
+            for (uint64_t jb = beg + (uint64_t)part * G + lane; jb < end; jb += stride * kIlp) {
+                uint32_t xs[kIlp], ws[kIlp], ox[kIlp], nx[kIlp];
+#pragma unroll
+                for (int u = 0; u < kIlp; ++u) {
+                    const uint64_t j = jb + (uint64_t)u * stride;
+                    xs[u] = j < end ? stream_load(&a.col[j]) : v;
+                    ws[u] = (WEIGHTED && j < end) ? stream_load(&a.wt[j]) : 1u;
                 }
-                if (nx != c) a.mark[x] = 1;
+#pragma unroll
+                for (int u = 0; u < kIlp; ++u) {
+                    ox[u] = xs[u] != v ? a.comm[xs[u]] : 0u;
+                    nx[u] = xs[u] != v ? a.target[xs[u]] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < kIlp; ++u) {
+                    if (xs[u] == v) continue;
+                    const int d = (int)(nx[u] == c) - (int)(ox[u] == own);
+                    if (d) dsig += (int64_t)ws[u] * d * (ox[u] != nx[u] ? 1 : 2);
+                    if (nx[u] != c) a.mark[xs[u]] = 1;
+                }
             }
         }
     }
@@ -876,6 +955,13 @@ void scatter_u32(uint32_t *dst, const uint32_t *idx, const uint32_t *val, size_t
     if (!n) return;
     hipLaunchKernelGGL(k_scatter_u32, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, dst, idx,
                        val, n);
+}
+
+void stream_copy(const void *src, void *dst, size_t bytes, hipStream_t s) {
+    const size_t n = bytes / sizeof(uint4);
+    if (!n) return;
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid_for(n, kBlock * 8, 1u << 20)), dim3(kBlock), 0, s,
+                       (const uint4 *)src, (uint4 *)dst, n);
 }
 
 void gather_degrees(const uint64_t *row, const uint32_t *ids, uint32_t n, uint64_t *deg,
@@ -1013,7 +1099,7 @@ void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
                const uint32_t *counts, hipStream_t s) {
     MoveCtx m;
     m.row = a.g.row; m.col = a.g.col; m.wt = a.g.wt; m.kdeg = a.g.kdeg;
-    m.comm = a.comm; m.tot = a.tot; m.target = a.target; m.stats = a.stats;
+    m.comm = a.comm; m.tot = a.tot; m.target = a.target;
     m.m2 = a.m2; m.tw = a.tw; m.v_end = a.v_end; m.highest = a.highest;
     const bool wtd = a.g.wt != nullptr;
     const bool fast = a.kind == 0;
@@ -1021,6 +1107,8 @@ void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
         const uint32_t cnt = counts[b];
         if (!cnt) continue;
         const uint32_t *L = lists[b];
+        m.stats = a.bstats + (size_t)b * kNumStats * kStatSlots;
+        if (a.bev) (void)hipEventRecord(a.bev[2 * b], s);
         if (fast) {
             switch (b) {
                 case 0: launch_move_fast<4>(m, wtd, a.gain, L, cnt, s); break;
@@ -1074,6 +1162,7 @@ void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
                 }
             }
         }
+        if (a.bev) (void)hipEventRecord(a.bev[2 * b + 1], s);
     }
 }
 

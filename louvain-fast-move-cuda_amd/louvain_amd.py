@@ -63,7 +63,9 @@ class Result(C.Structure):
 class KernelStats(C.Structure):
     _fields_ = [("move_launches", C.c_uint64), ("move_ms", C.c_double),
                 ("move_bytes", C.c_uint64), ("move_entries", C.c_uint64),
-                ("sweep_ms", C.c_double), ("sweeps", C.c_uint64)]
+                ("sweep_ms", C.c_double), ("sweeps", C.c_uint64),
+                ("variant_launches", C.c_uint64 * 32), ("variant_ms", C.c_double * 32),
+                ("variant_bytes", C.c_uint64 * 32), ("variant_entries", C.c_uint64 * 32)]
 
 
 # name -> (restype, argtypes); every symbol include/lv_api.h declares
@@ -102,6 +104,8 @@ SIGNATURES = {
                                                 C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "lv_kernel_stats_get": (C.c_int, [_vp, C.POINTER(KernelStats)]),
     "lv_last_run_kernel_stats": (C.c_int, [C.POINTER(KernelStats)]),
+    "lv_kernel_variant_name": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.c_size_t]),
+    "lv_bandwidth_probe": (C.c_int, [C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
     "lv_gen_rmat": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64,
                               _vp, _vp, C.c_uint64, C.POINTER(C.c_uint64),
                               C.POINTER(C.c_uint32)]),
@@ -330,7 +334,26 @@ def last_run_kernel_stats():
 
 
 def _rec(s):
-    return {name: getattr(s, name) for name, _ in s._fields_ if not name.startswith("_")}
+    out = {}
+    for name, _ in s._fields_:
+        if name.startswith("_"):
+            continue
+        v = getattr(s, name)
+        out[name] = list(v) if isinstance(v, C.Array) else v
+    return out
+
+
+def bandwidth_probe(nbytes=1 << 30, reps=5):
+    """GB/s (read + write) of the plain device copy kernel k_stream_copy."""
+    g = C.c_double()
+    _check(lib.lv_bandwidth_probe(int(nbytes), int(reps), C.byref(g)))
+    return g.value
+
+
+def kernel_variant_name(variant, gain=GAIN_EXACT):
+    buf = C.create_string_buffer(96)
+    _check(lib.lv_kernel_variant_name(int(variant), int(gain), buf, 96))
+    return buf.value.decode()
 
 
 def gen_rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=2):
