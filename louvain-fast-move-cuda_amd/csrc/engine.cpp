@@ -408,7 +408,7 @@ void launch_sweep(Community &c, const lv_params &p, int kind, bool apply, uint32
     a.bucket = st.bucket.data();
     a.stats = st.stats.data() + kGlobalSection;
     a.bstats = st.stats.data();
-    a.bev = p.profile ? st.bev : nullptr;
+    a.bev = (p.profile && kind != LV_SWEEP_SORT) ? st.bev : nullptr;
     a.m2 = (int64_t)(2.0 * c.graph.total_weight);
     a.tw = c.graph.total_weight;
     a.v_end = v_end;
@@ -418,7 +418,8 @@ void launch_sweep(Community &c, const lv_params &p, int kind, bool apply, uint32
     fill_u64(st.stats.data(), 0ull, st.stats.size(), s);
     uint32_t *const *cur = st.lists(st.cur);
     if (p.profile) LV_HIP(hipEventRecord(st.ev[0], s));
-    move_pass(a, st.hub_tables(), cur, st.count, s);
+    if (kind == LV_SWEEP_SORT) sort_sweep(a, cur, st.count, st.bucket_off, st.ws, s);
+    else move_pass(a, st.hub_tables(), cur, st.count, s);
     if (p.profile) LV_HIP(hipEventRecord(st.ev[1], s));
     if (apply) {
         commit_pass(a, cur, st.count, s);
@@ -484,7 +485,7 @@ int OptimizationPhase::run(Community &c, const lv_params &p, lv_sweep_rec *rec, 
                 const uint64_t bt = bucket_slots(h, b, ST_TWRITES);
                 const uint64_t bb = bytes_of(be, bc, verts, bt);
                 bytes += bb;
-                if (b == kHubBucket) continue;
+                if (b == kHubBucket || kind == LV_SWEEP_SORT) continue;
                 float ms_b = 0;
                 LV_HIP(hipEventElapsedTime(&ms_b, st.bev[2 * b], st.bev[2 * b + 1]));
                 const int vi = move_variant(b, fast, wtd);
@@ -542,53 +543,60 @@ namespace lv {
 uint32_t chunk_vertex_end(Community &c, const lv_params &p, int kind, const uint8_t *active_host) {
     (void)kind;
     (void)active_host;
-    // Chunk bounds depend on the CSR only; the early return needs the number
-    // of selected entries per chunk for the current active set, which the
-    // device lists give.  Computed on the host from the active lists.
     SweepState &st = *c.state;
     hipStream_t s = c.stream();
     const uint32_t n = c.graph.n_nodes;
     const uint64_t E = c.graph.n_entries;
-    if (E <= p.chunk_entries) return n;   // one chunk: empty only if nothing is selected
-    std::vector<uint64_t> row(n + 1);
-    std::vector<uint32_t> col(E);
-    LV_HIP(hipMemcpyAsync(row.data(), c.graph.neighboorhood_sum.data(), (n + 1) * sizeof(uint64_t),
-                          hipMemcpyDeviceToHost, s));
-    LV_HIP(hipMemcpyAsync(col.data(), c.graph.edge_destination.data(), E * sizeof(uint32_t),
-                          hipMemcpyDeviceToHost, s));
-    std::vector<uint8_t> active(n, 0);
-    std::vector<uint32_t> lst;
-    for (int b = 0; b < kNumBuckets; ++b) {
-        if (!st.count[b]) continue;
-        lst.resize(st.count[b]);
-        LV_HIP(hipMemcpyAsync(lst.data(), st.lists(st.cur)[b], st.count[b] * sizeof(uint32_t),
-                              hipMemcpyDeviceToHost, s));
+    if (E <= p.chunk_entries || n == 0) return n;   // one chunk: empty only if nothing is selected
+    // chunk bounds depend on the CSR only: once per level
+    const ChunkKey key{c.graph.neighboorhood_sum.data(), n, E, p.chunk_entries};
+    if (!(st.chunk_key == key)) {
+        std::vector<uint64_t> row((size_t)n + 1);
+        LV_HIP(hipMemcpyAsync(row.data(), c.graph.neighboorhood_sum.data(),
+                              ((size_t)n + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
         LV_HIP(hipStreamSynchronize(s));
-        for (uint32_t v : lst) active[v] = 1;
+        // limit = round + STEP; limit = limit >= E ? E : row[src(limit) - 1]
+        // (phaseOptimizationSort.cu:46-60); a bound that would not advance moves
+        // to the next vertex boundary instead of looping forever
+        std::vector<uint32_t> cv{0};
+        uint64_t round = 0;
+        while (round < E) {
+            uint64_t limit = round + p.chunk_entries;
+            if (limit >= E) {
+                limit = E;
+            } else {
+                const uint32_t src =
+                    (uint32_t)(std::upper_bound(row.begin(), row.end(), limit) - row.begin()) - 1;
+                uint64_t cand = src > 0 ? row[src - 1] : 0;
+                if (cand <= round) cand = row[src] > round ? row[src] : row[src + 1];
+                limit = cand;
+            }
+            if (limit < E)
+                cv.push_back((uint32_t)(std::upper_bound(row.begin(), row.end(), limit) - row.begin()) - 1);
+            round = limit;
+        }
+        if (cv.size() > kMaxChunks) throw Error(LV_ERR_RANGE, "too many emulated chunks");
+        st.chunk_cv.assign(cv.begin(), cv.end());
+        uint32_t *cv_dev = st.ws.get<uint32_t>(kChunkSlot, cv.size());
+        LV_HIP(hipMemcpyAsync(cv_dev, cv.data(), cv.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        uint32_t *nsel = st.ws.get<uint32_t>(kChunkSlot + 1, n);
+        nonself_counts(c.graph.neighboorhood_sum.data(), c.graph.edge_destination.data(), n, nsel, s);
+        st.chunk_key = key;
     }
+    const uint32_t nc = (uint32_t)st.chunk_cv.size();
+    unsigned long long *csel = st.ws.get<unsigned long long>(kChunkSlot + 2, nc);
+    fill_u64(csel, 0ull, nc, s);
+    uint32_t *const *L = st.lists(st.cur);
+    for (int b = 0; b < kNumBuckets; ++b)
+        chunk_selected(L[b], st.count[b], st.ws.get<uint32_t>(kChunkSlot + 1, n),
+                       st.ws.get<uint32_t>(kChunkSlot, nc), nc, csel, s);
+    std::vector<unsigned long long> h(nc);
+    LV_HIP(hipMemcpyAsync(h.data(), csel, nc * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     LV_HIP(hipStreamSynchronize(s));
-    uint64_t round = 0;
-    uint32_t v = 0;
-    while (round < E) {
-        uint64_t limit = round + p.chunk_entries;
-        if (limit >= E) {
-            limit = E;
-        } else {
-            const uint32_t src = (uint32_t)(std::upper_bound(row.begin(), row.end(), limit) - row.begin()) - 1;
-            uint64_t cand = src > 0 ? row[src - 1] : 0;
-            if (cand <= round) cand = row[src] > round ? row[src] : row[src + 1];
-            limit = cand;
-        }
-        uint64_t sel = 0;
-        const uint32_t v0 = v;
-        while (v < n && row[v] < limit) {
-            if (active[v])
-                for (uint64_t j = row[v]; j < row[v + 1]; ++j) sel += col[j] != v;
-            ++v;
-        }
-        if (sel == 0) return v0;
-        round = limit;
-    }
+    // the fast/sort sweeps return at the first chunk that selects nothing
+    // (phaseOptimizationSort.cu:81-86, :235-240)
+    for (uint32_t k = 0; k < nc; ++k)
+        if (h[k] == 0) return st.chunk_cv[k];
     return n;
 }
 

@@ -11,6 +11,8 @@
 #include "../../include/graph.h"
 #include "lv_kernels.h"
 
+struct Community;
+
 namespace lv {
 
 // best-move kernel variants timed one by one: (bucket 0-7, fast, weighted)
@@ -60,6 +62,17 @@ struct Workspace {
 };
 
 constexpr size_t kSetupSlot = 40;   // workspace slots used by level setup
+constexpr size_t kSortSlot = 60;    // workspace slots used by the sort sweep
+constexpr size_t kChunkSlot = 90;   // workspace slots used by chunk emulation
+
+struct ChunkKey {
+    const void *row = nullptr;
+    uint32_t n = 0;
+    uint64_t E = 0, step = 0;
+    bool operator==(const ChunkKey &o) const {
+        return row == o.row && n == o.n && E == o.E && step == o.step;
+    }
+};
 
 struct SweepState {
     uint32_t n = 0;
@@ -85,6 +98,9 @@ struct SweepState {
     DeviceArray<uint32_t> hub_part_c;
     DeviceArray<uint32_t> hub_self;
     Workspace ws;
+    // chunk emulation: level the bounds were computed for, chunk start vertices
+    ChunkKey chunk_key;
+    std::vector<uint32_t> chunk_cv;
     // two reusable storages for contracted levels (level l+1 is built while
     // level l, held in the other one or in the resident level-0 graph, is read)
     GraphDevice level_buf[2];
@@ -109,9 +125,12 @@ struct SweepState {
 };
 
 LevelView level_view(const GraphDevice &g);
+// The sort variant of the best-move pass (lv_sort_sweep.hip).
+void sort_sweep(const SweepArgs &x, uint32_t *const *lists, const uint32_t *counts,
+                const uint64_t *bucket_off, Workspace &ws, hipStream_t s);
 uint64_t sum_slots(const unsigned long long *h, int stat);            // global section
 uint64_t bucket_slots(const unsigned long long *h, int bucket, int stat);
 uint64_t all_bucket_slots(const unsigned long long *h, int stat);
-void launch_sweep(struct ::Community &c, const lv_params &p, int kind, bool apply, uint32_t v_end);
+void launch_sweep(::Community &c, const lv_params &p, int kind, bool apply, uint32_t v_end);
 
 }  // namespace lv

@@ -52,6 +52,7 @@ lv_status need_device() {
 namespace lv {
 void community_totals(const uint32_t *comm, const uint32_t *kdeg, uint32_t n, uint32_t *tot,
                       hipStream_t s);
+uint32_t chunk_vertex_end(Community &c, const lv_params &p, int kind, const uint8_t *active_host);
 }
 
 extern "C" {
@@ -306,7 +307,10 @@ lv_status lv_best_moves(lv_community *c, int kind, const lv_params *p, const uin
             st.cur ^= 1;
             std::memcpy(st.count, cnt, sizeof(cnt));
         }
-        lv::launch_sweep(C, P, kind, false, n);
+        const uint32_t v_end = (P.chunk_emulation && kind != LV_SWEEP_HASH)
+                                   ? lv::chunk_vertex_end(C, P, kind, nullptr)
+                                   : n;
+        lv::launch_sweep(C, P, kind, false, v_end);
         LV_HIP(hipStreamSynchronize(s));
         if (e_active) *e_active = lv::all_bucket_slots(st.host_stats, lv::ST_E_ACTIVE);
         LV_HIP(hipMemcpy(target, st.target.data(), n * sizeof(uint32_t), hipMemcpyDeviceToHost));

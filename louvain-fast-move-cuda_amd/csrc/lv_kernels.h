@@ -21,6 +21,7 @@ constexpr int kNumBuckets = 9;        // 8 LDS-table buckets + global-table hubs
 constexpr int kHubBucket = 8;
 constexpr int kStatSlots = 64;        // spread of the per-block statistic atomics
 constexpr int kHubParts = 32;         // workgroups per hub vertex
+constexpr uint32_t kMaxChunks = 4096; // reference chunks per level (STEP_ROUND emulation)
 constexpr unsigned long long kHubEmpty = 0xFFFFFFFF00000000ull;   // empty packed slot
 
 // Maximum degree handled by each LDS bucket; larger goes to the hub bucket.
@@ -83,6 +84,12 @@ void iota_u32(uint32_t *p, size_t n, hipStream_t s);
 void scatter_u32(uint32_t *dst, const uint32_t *idx, const uint32_t *val, size_t n, hipStream_t s);
 // plain device copy of `bytes` (a multiple of 16) with k_stream_copy
 void stream_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
+// number of non-self entries of every vertex
+void nonself_counts(const uint64_t *row, const uint32_t *col, uint32_t n, uint32_t *nsel,
+                    hipStream_t s);
+// csel[k] += non-self entries of listed vertices in chunk k (vertices [cv[k], cv[k+1]))
+void chunk_selected(const uint32_t *list, uint32_t count, const uint32_t *nsel, const uint32_t *cv,
+                    uint32_t nc, unsigned long long *csel, hipStream_t s);
 // deg[i] = row[ids[i]+1] - row[ids[i]]
 void gather_degrees(const uint64_t *row, const uint32_t *ids, uint32_t n, uint64_t *deg,
                     hipStream_t s);

@@ -70,6 +70,41 @@ __global__ __launch_bounds__(kBlock) void k_stream_copy(const uint4 *src, uint4 
         dst[i] = src[i];
 }
 
+// Non-self entries of every vertex (rows of contracted levels are unsorted).
+__global__ void k_nonself_counts(const uint64_t *row, const uint32_t *col, uint32_t n,
+                                 uint32_t *nsel) {
+    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n;
+         v += (size_t)gridDim.x * blockDim.x) {
+        uint32_t c = 0;
+        for (uint64_t j = row[v]; j < row[v + 1]; ++j) c += col[j] != (uint32_t)v;
+        nsel[v] = c;
+    }
+}
+
+// Selected entries per reference chunk for the listed vertices; chunk k holds
+// vertices [cv[k], cv[k+1]).
+__global__ __launch_bounds__(kBlock) void k_chunk_selected(const uint32_t *list, uint32_t count,
+                                                          const uint32_t *nsel, const uint32_t *cv,
+                                                          uint32_t nc, unsigned long long *csel) {
+    __shared__ unsigned long long h[kMaxChunks];
+    for (uint32_t i = threadIdx.x; i < nc; i += kBlock) h[i] = 0;
+    __syncthreads();
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t v = list[i];
+        uint32_t lo = 0, hi = nc;   // largest k with cv[k] <= v
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (cv[mid] <= v) lo = mid; else hi = mid;
+        }
+        const uint32_t c = nsel[v];
+        if (c) atomicAdd(&h[lo], (unsigned long long)c);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nc; i += kBlock)
+        if (h[i]) atomicAdd(&csel[i], h[i]);
+}
+
 __global__ void k_gather_degrees(const uint64_t *row, const uint32_t *ids, uint32_t n,
                                  uint64_t *deg) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
@@ -962,6 +997,20 @@ void stream_copy(const void *src, void *dst, size_t bytes, hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(k_stream_copy, dim3(grid_for(n, kBlock * 8, 1u << 20)), dim3(kBlock), 0, s,
                        (const uint4 *)src, (uint4 *)dst, n);
+}
+
+void nonself_counts(const uint64_t *row, const uint32_t *col, uint32_t n, uint32_t *nsel,
+                    hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_nonself_counts, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, row, col, n,
+                       nsel);
+}
+
+void chunk_selected(const uint32_t *list, uint32_t count, const uint32_t *nsel, const uint32_t *cv,
+                    uint32_t nc, unsigned long long *csel, hipStream_t s) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_chunk_selected, dim3(grid_for(count, kBlock * 8, 4096u)), dim3(kBlock), 0,
+                       s, list, count, nsel, cv, nc, csel);
 }
 
 void gather_degrees(const uint64_t *row, const uint32_t *ids, uint32_t n, uint64_t *deg,

@@ -213,3 +213,43 @@ def test_phase_api_matches_louvain(fixture_path):
     best, nc = c.best_communities(g.n)
     assert np.array_equal(best, ref["best"]) and nc == ref["n_comm"]
     assert c.modularity == ref["modularity"]
+
+
+def test_chunk_emulation_early_return_snapshot():
+    """An empty first chunk stops the fast/sort sweeps but not the hash sweep."""
+    n = 200
+    a = np.arange(n - 1)
+    b = a + 1
+    g = L.Graph.from_edges(a, b)
+    og = O.Graph.from_edges(a, b)
+    c = L.Community(g)
+    comm = np.arange(n, dtype=np.uint32)
+    act = np.zeros(n, np.uint8)
+    act[150:] = 1
+    for kind in (L.SWEEP_SORT, L.SWEEP_HASH):
+        for emu in (0, 1):
+            say(f"chunk snapshot kind={kind} emu={emu}")
+            tg, _ = c.best_moves(kind, L.params(chunk_emulation=emu, chunk_entries=100), act)
+            to, _, _, _ = og.best_moves(comm, act, kind,
+                                        O.params(chunk_emulation=emu, chunk_entries=100))
+            assert np.array_equal(tg, to)
+    tg, _ = c.best_moves(L.SWEEP_SORT, L.params(chunk_emulation=1, chunk_entries=100), act)
+    assert np.array_equal(tg, comm)          # nothing moves: the early return hit chunk 0
+
+
+def test_sort_sweep_rmat_snapshot():
+    """Sort-variant decisions (radix sort + runs) equal the oracle on R-MAT with hubs."""
+    ea, eb, n = L.gen_rmat(15, seed=9)
+    g = L.Graph.from_edges(ea, eb)
+    og = O.Graph.from_edges(ea, eb)
+    c = L.Community(g)
+    rng = np.random.default_rng(3)
+    for ncomm in (n, 2000, 50):
+        comm = rng.integers(0, ncomm, n).astype(np.uint32)
+        act = (rng.random(n) < 0.7).astype(np.uint8)
+        c.set_partition(comm)
+        for kind in (L.SWEEP_SORT, L.SWEEP_HASH):
+            say(f"rmat15 snapshot ncomm={ncomm} kind={kind}")
+            tg, eg = c.best_moves(kind, L.params(), act)
+            to, _, eo, _ = og.best_moves(comm, act, kind, O.params())
+            assert np.array_equal(tg, to) and eg == eo
