@@ -132,3 +132,23 @@ def test_planted_generator_properties():
     inside = np.mean(planted[ea] == planted[eb])
     assert 0.55 < inside < 0.85                       # mixing mu = 0.3
     assert q > 0.5
+
+
+def test_ba_10000_oracle_q_matches_fp64_recompute():
+    """The CPU-path workload (BA n=10,000, 2 edges per vertex, seed 1): the
+    oracle's final Q equals an independent FP64 recomputation and networkx."""
+    import networkx as nx
+    from oracle import oracle as O
+    from qcheck import modularity_fp64
+    ea, eb = L.gen_ba(10000, 2, seed=1)
+    assert len(ea) == 19996 and int(max(ea.max(), eb.max())) == 9999
+    og = O.Graph.from_edges(ea, eb)
+    r = og.louvain(O.params(sweep_threshold=1e-6, level_threshold=1e-6))
+    row, col, _ = og.arrays()[:3]
+    assert abs(r["modularity"] - modularity_fp64(row, col, None, r["best"])) < 1e-9
+    G = nx.Graph()
+    G.add_edges_from(zip(ea.tolist(), eb.tolist()))
+    parts = {}
+    for v, c in enumerate(r["best"]):
+        parts.setdefault(int(c), set()).add(v)
+    assert abs(r["modularity"] - nx.community.modularity(G, list(parts.values()))) < 1e-9
