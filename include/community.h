@@ -22,6 +22,7 @@
 
 namespace lv {
 struct SweepState;
+struct BatchState;
 }
 
 struct Community {
@@ -40,6 +41,8 @@ struct Community {
     int32_t level = 0;
     uint32_t n_original = 0;
     std::unique_ptr<lv::SweepState> state;
+    // set when this community is the disjoint union of a batch of graphs
+    std::unique_ptr<lv::BatchState> batch;
     // level-0 CSR, kept resident so that reset() can start over without an upload
     GraphDevice graph0;
 

@@ -28,6 +28,9 @@ struct GraphHost {
     std::vector<uint32_t> col;     // 2*n_links - (#self-loops) ... entries
     std::vector<uint32_t> weights; // empty when every weight is 1
     bool unit_weights = true;
+    // several independent graphs side by side (a batch): 2m < 2^31 holds per
+    // graph and is checked by the batch, not over the union
+    bool disjoint_union = false;
 
     GraphHost() = default;
     // graph.cu:17 -- throws lv::Error on failure

@@ -178,6 +178,41 @@ lv_status lv_bandwidth_probe(uint64_t bytes, int reps, double *gbs);
 /* Kernel name of best-move variant v for gain mode `gain` (as rocprofv3 lists it). */
 lv_status lv_kernel_variant_name(int variant, int gain, char *buf, size_t len);
 
+/* ---- batches: many independent graphs in one launch sequence ----
+ * No counterpart in the reference (it runs one graph per process, main.cu:16-22).
+ * The graphs run side by side as one disjoint union on the device; each keeps
+ * the reference's own level loop, thresholds and counters, and its result
+ * equals lv_louvain on that graph alone.  For callers with thousands of small
+ * graphs, where per-graph launches and synchronisations dominate. */
+typedef struct lv_batch lv_batch;
+
+typedef struct lv_batch_result {
+    /* per graph (arrays of n_graphs, each may be NULL) */
+    double *modularity;
+    uint32_t *n_of_best_communities;
+    int32_t *levels;
+    int32_t *sweeps;
+    /* best communities of every graph back to back, graph-local ids (sum of
+     * the graphs' vertex counts), or NULL */
+    uint32_t *best_communities;
+    /* whole batch */
+    int32_t union_levels;
+    int32_t union_sweeps;
+    double seconds;
+    uint64_t traversed_entries;
+} lv_batch_result;
+
+/* The graphs are copied (host) and uploaded as one union; stream may be NULL. */
+lv_status lv_batch_create(const lv_graph *const *graphs, uint32_t n_graphs, void *stream,
+                          lv_batch **out);
+void lv_batch_destroy(lv_batch *b);
+lv_status lv_batch_info(const lv_batch *b, uint32_t *n_graphs, uint64_t *n_vertices,
+                        uint64_t *n_entries);
+/* Runs every graph from the identity partition (the batch can be run again).
+ * Chunk emulation applies only to graphs of at most chunk_entries entries
+ * (one chunk); a larger graph with chunk_emulation set is LV_ERR_ARG. */
+lv_status lv_batch_louvain(lv_batch *b, const lv_params *p, lv_batch_result *r);
+
 /* Seeded synthetic graph generators (host, multithreaded; counter-based PRNG,
  * so the output is independent of the thread count).  Edges are written as
  * (a[i], b[i], w[i]) with dense ids; *m receives the count.  Pass NULL arrays

@@ -199,7 +199,8 @@ GraphDevice::GraphDevice(const GraphHost &g, hipStream_t s) {
         m2w += s2;
     }
     m2 = m2w;
-    if (m2 >= 0x80000000ull) throw Error(LV_ERR_RANGE, "total weight 2m >= 2^31");
+    if (m2 >= 0x80000000ull && !g.disjoint_union)
+        throw Error(LV_ERR_RANGE, "total weight 2m >= 2^31");
     LV_HIP(hipMemcpyAsync(neighboorhood_sum.data(), g.row.data(), g.row.size() * sizeof(uint64_t),
                           hipMemcpyHostToDevice, s));
     if (n_entries)
@@ -265,6 +266,7 @@ void Community::reset() {
 }
 
 Community::~Community() {
+    batch.reset();
     state.reset();
     if (own_stream_ && streams[0]) (void)hipStreamDestroy(streams[0]);
 }
@@ -281,6 +283,7 @@ Community::Community(Community &&o) noexcept
       level(o.level),
       n_original(o.n_original),
       state(std::move(o.state)),
+      batch(std::move(o.batch)),
       graph0(std::move(o.graph0)),
       own_stream_(o.own_stream_) {
     streams[0] = o.streams[0];
@@ -291,6 +294,7 @@ Community::Community(Community &&o) noexcept
 
 Community &Community::operator=(Community &&o) noexcept {
     if (this != &o) {
+        batch.reset();
         state.reset();
         if (own_stream_ && streams[0]) (void)hipStreamDestroy(streams[0]);
         graph = std::move(o.graph);
@@ -304,6 +308,7 @@ Community &Community::operator=(Community &&o) noexcept {
         level = o.level;
         n_original = o.n_original;
         state = std::move(o.state);
+        batch = std::move(o.batch);
         graph0 = std::move(o.graph0);
         own_stream_ = o.own_stream_;
         streams[0] = o.streams[0];
@@ -415,6 +420,7 @@ void launch_sweep(Community &c, const lv_params &p, int kind, bool apply, uint32
     a.kind = kind;
     a.gain = p.gain;
     a.highest = (p.tie == LV_TIE_REFERENCE && kind == LV_SWEEP_HASH);
+    if (c.batch) a.bt = c.batch->view();
     fill_u64(st.stats.data(), 0ull, st.stats.size(), s);
     uint32_t *const *cur = st.lists(st.cur);
     if (p.profile) LV_HIP(hipEventRecord(st.ev[0], s));
@@ -619,19 +625,28 @@ static int bits_for(uint64_t x) {   // smallest b with 2^b >= x (x >= 1)
 }
 }  // namespace lv
 
-void AggregationPhase::run(Community &c, bool aggregate, MODE mode) {
-    (void)mode;  // the sort and hash contractions of the reference give the same exact graph
+namespace lv {
+// community_map = inclusive_scan(tot > 0) (phaseAggregation.cuh:27-43), then
+// best_communities composed with it (:45); returns the device map.
+uint32_t *renumber(Community &c, uint32_t &n_new) {
     hipStream_t s = c.stream();
     Workspace &ws = c.state->ws;
     const uint32_t n = c.graph.n_nodes;
-    // community_map = inclusive_scan(tot > 0) (phaseAggregation.cuh:27-43)
     uint32_t *map = ws.get<uint32_t>(W_MAP, n);
     renumber_flags(c.communities_weight.data(), n, map, s);
     scan_u32(map, map, n, true, ws.get<uint32_t>(W_SCAN32, scan_scratch_elems(n)), s);
-    c.update_best(map);                                               // :45
-    uint32_t n_new = 0;
+    c.update_best(map);
+    n_new = 0;
     if (n) LV_HIP(hipMemcpyAsync(&n_new, map + (n - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     LV_HIP(hipStreamSynchronize(s));
+    return map;
+}
+}  // namespace lv
+
+void AggregationPhase::run(Community &c, bool aggregate, MODE mode) {
+    (void)mode;  // the sort and hash contractions of the reference give the same exact graph
+    uint32_t n_new = 0;
+    uint32_t *map = renumber(c, n_new);
     c.n_of_best_communities = n_new;                                  // :46
     if (!aggregate) return;
     const int64_t sigma_in = c.sigma_in;

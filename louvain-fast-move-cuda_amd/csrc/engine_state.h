@@ -124,6 +124,47 @@ struct SweepState {
     HubTables hub_tables();
 };
 
+// Per-graph state of a batch run as one disjoint union (csrc/batch.cpp).
+struct BatchState {
+    uint32_t n_graphs = 0;
+    std::vector<uint32_t> voff;        // [G+1] level-0 vertex offsets of the graphs
+    std::vector<uint64_t> entries;     // [G] level-0 entries of each graph
+    std::vector<double> tw;            // [G] m
+    std::vector<int64_t> m2;           // [G] 2m as an integer
+    std::vector<int64_t> sigma0;       // [G] identity-partition sigma_in (self-loop weight)
+    std::vector<uint64_t> sumsq0;      // [G] identity-partition sum of k^2
+    DeviceArray<uint32_t> gid0;        // [n0] graph of each level-0 vertex (resident)
+    DeviceArray<uint32_t> gid_buf[2];  // graph ids of contracted levels
+    const uint32_t *gid = nullptr;     // graph ids of the current level
+    int next_gid = 0;
+    DeviceArray<int64_t> m2_dev;
+    DeviceArray<double> tw_dev;
+    DeviceArray<uint8_t> live_dev;
+    DeviceArray<unsigned long long> acc;       // [3G]: dsig, sumsq, community counts
+    unsigned long long *host_acc = nullptr;    // pinned [3G]
+    uint8_t *host_live = nullptr;              // pinned [G]
+
+    BatchState() = default;
+    ~BatchState();
+    BatchState(const BatchState &) = delete;
+    BatchState &operator=(const BatchState &) = delete;
+
+    BatchView view() {
+        BatchView v;
+        v.gid = gid;
+        v.m2 = m2_dev.data();
+        v.tw = tw_dev.data();
+        v.live = live_dev.data();
+        v.dsig = acc.data();
+        v.sumsq = acc.data() + n_graphs;
+        return v;
+    }
+};
+
+// renumbering of the non-empty communities (device map, 1-based), with
+// best_communities composed with it; n_new receives the community count
+uint32_t *renumber(::Community &c, uint32_t &n_new);
+
 LevelView level_view(const GraphDevice &g);
 // The sort variant of the best-move pass (lv_sort_sweep.hip).
 void sort_sweep(const SweepArgs &x, uint32_t *const *lists, const uint32_t *counts,

@@ -9,6 +9,7 @@
 #include <new>
 #include <vector>
 
+#include "../../include/batch.h"
 #include "../../include/community.h"
 #include "../../include/modularityAlgorithms.h"
 #include "engine_state.h"
@@ -16,6 +17,10 @@
 
 struct lv_graph {
     GraphHost g;
+};
+struct lv_batch {
+    GraphBatch b;
+    lv_batch(const std::vector<const GraphHost *> &g, hipStream_t s) : b(g, s) {}
 };
 struct lv_community {
     Community c;
@@ -395,6 +400,38 @@ lv_status lv_last_run_kernel_stats(lv_kernel_stats *s) {
     if (!s) return LV_ERR_ARG;
     fill_stats(s, lv::g_last_run_stats);
     return LV_OK;
+}
+
+lv_status lv_batch_create(const lv_graph *const *graphs, uint32_t n_graphs, void *stream,
+                          lv_batch **out) {
+    if (!out || !graphs || n_graphs == 0) return LV_ERR_ARG;
+    *out = nullptr;
+    for (uint32_t i = 0; i < n_graphs; ++i)
+        if (!graphs[i]) return LV_ERR_ARG;
+    if (lv_status st = need_device()) return st;
+    return guarded([&] {
+        std::vector<const GraphHost *> gs(n_graphs);
+        for (uint32_t i = 0; i < n_graphs; ++i) gs[i] = &graphs[i]->g;
+        *out = new lv_batch(gs, (hipStream_t)stream);
+    });
+}
+
+void lv_batch_destroy(lv_batch *b) { delete b; }
+
+lv_status lv_batch_info(const lv_batch *b, uint32_t *n_graphs, uint64_t *n_vertices,
+                        uint64_t *n_entries) {
+    if (!b) return LV_ERR_ARG;
+    if (n_graphs) *n_graphs = b->b.n_graphs();
+    if (n_vertices) *n_vertices = b->b.n_vertices();
+    if (n_entries) *n_entries = b->b.n_entries();
+    return LV_OK;
+}
+
+lv_status lv_batch_louvain(lv_batch *b, const lv_params *p, lv_batch_result *r) {
+    if (!b) return LV_ERR_ARG;
+    lv_params d;
+    lv_default_params(&d);
+    return guarded([&] { b->b.run(p ? *p : d, r); });
 }
 
 }  // extern "C"

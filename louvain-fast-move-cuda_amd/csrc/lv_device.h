@@ -85,6 +85,34 @@ __device__ inline void block_add_stats(unsigned long long *stats, int st0, uint6
     __syncthreads();
 }
 
+// Adds val to base[key] for the lanes with `has`.  A wave whose contributing
+// lanes share one key (the usual case: lists and tiles follow vertex order and
+// a graph of a batch is a vertex range) issues one atomic.  Every lane of the
+// wave must call it.
+__device__ inline void wave_keyed_add(unsigned long long *base, uint32_t key, uint64_t val,
+                                      bool has) {
+    const unsigned long long m = __ballot(has);
+    if (m == 0) return;
+    const int leader = __ffsll((long long)m) - 1;
+    const uint32_t k0 = (uint32_t)__shfl((int)key, leader);
+    if (__ballot(has && key != k0) == 0) {
+        const uint64_t t = group_sum_u64<64>(has ? val : 0ull);
+        if ((int)(threadIdx.x & 63) == leader) atomicAdd(&base[k0], (unsigned long long)t);
+    } else if (has) {
+        atomicAdd(&base[key], (unsigned long long)val);
+    }
+}
+
+// 2m and m of vertex v's graph; false if that graph no longer sweeps.
+__device__ inline bool vertex_graph(const BatchView &b, uint32_t v, int64_t m2, double tw,
+                                    int64_t &m2v, double &twv) {
+    if (!b.gid) { m2v = m2; twv = tw; return true; }
+    const uint32_t g = b.gid[v];
+    m2v = b.m2[g];
+    twv = b.tw[g];
+    return b.live[g] != 0;
+}
+
 // Adjacency streams are read once per pass: non-temporal loads keep them from
 // evicting the gathered comm/tot lines from L2.
 __device__ inline uint32_t stream_load(const uint32_t *p) { return __builtin_nontemporal_load(p); }

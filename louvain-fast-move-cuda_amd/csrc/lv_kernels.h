@@ -57,7 +57,20 @@ struct HubTables {
     uint32_t *self;           // [n_hubs] weight to own community
 };
 
+// Several graphs swept as one disjoint union: the graph id of a vertex selects
+// its graph's 2m and m, and whether that graph is still sweeping this level.
+// Communities never span graphs (no entry does), so everything else is shared.
+struct BatchView {
+    const uint32_t *gid = nullptr;        // [n] graph of each vertex; nullptr: a single graph
+    const int64_t *m2 = nullptr;          // [graphs] 2m
+    const double *tw = nullptr;           // [graphs] m (the reference's total_weight)
+    const uint8_t *live = nullptr;        // [graphs] 1 while the graph sweeps at this level
+    unsigned long long *dsig = nullptr;   // [graphs] sigma_in change of the sweep (two's complement)
+    unsigned long long *sumsq = nullptr;  // [graphs] sum of tot^2 after the sweep
+};
+
 struct SweepArgs {
+    BatchView bt;
     LevelView g;
     const uint32_t *comm;
     uint32_t *target;
@@ -128,6 +141,14 @@ void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *cou
 // comm := target for movers, sum tot^2, compact marked vertices into next lists
 void finalize_pass(const SweepArgs &a, uint32_t *const *next_lists, uint32_t *next_counts,
                    hipStream_t s);
+
+// ---- batches of graphs ----
+// counts[gid[c]] += 1 for every non-empty community c
+void batch_comm_counts(const uint32_t *tot, const uint32_t *gid, uint32_t n,
+                       unsigned long long *counts, hipStream_t s);
+// gid_new[map[c]-1] = gid[c] for every non-empty community c
+void batch_next_gid(const uint32_t *tot, const uint32_t *map, const uint32_t *gid, uint32_t n,
+                    uint32_t *gid_new, hipStream_t s);
 
 // ---- aggregation ----
 // new_id[c] = map[c]-1 for communities with tot>0; n_new via *map_last (device)

@@ -383,6 +383,7 @@ __global__ __launch_bounds__(kBlock) void k_identity_parts(LevelView g,
 // ---------------------------------------------------------- best-move pass
 
 struct MoveCtx {
+    BatchView bt;
     const uint64_t *row;
     const uint32_t *col;
     const uint32_t *wt;
@@ -411,7 +412,9 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
     const uint32_t idx = blockIdx.x * GPB + gi;
     const bool valid = idx < count;
     const uint32_t v = valid ? list[idx] : 0u;
-    const bool live = valid && v < a.v_end;
+    int64_t m2 = a.m2;
+    double tw = a.tw;
+    const bool live = valid && v < a.v_end && vertex_graph(a.bt, v, a.m2, a.tw, m2, tw);
     uint32_t *K = s_key + gi * T;
     uint32_t *V = s_val + gi * T;
     // the vertex's table: the smallest power of two >= 2*degree (<= T)
@@ -483,7 +486,7 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
             for (int u = 0; u < kIlp; ++u) {
                 if (cs[u] == LV_EMPTY) continue;
                 ++ncand;
-                const S sc = SC::of(Ws[u], self, kv, tot_own, ts[u], a.m2, a.tw);
+                const S sc = SC::of(Ws[u], self, kv, tot_own, ts[u], m2, tw);
                 if (better_candidate(sc, cs[u], best, best_c, a.highest)) { best = sc; best_c = cs[u]; }
             }
         }
@@ -494,7 +497,7 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
         const uint64_t gself = group_sum_u64<G>(nself);
         if (live && lane == 0) {
             considered = gself > 0;
-            if (best != SC::none() && SC::positive(best, self, kv, tot_own, a.m2)) {
+            if (best != SC::none() && SC::positive(best, self, kv, tot_own, m2)) {
                 a.target[v] = best_c;
                 twrite = 1;
             }
@@ -518,7 +521,7 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
                 }
             }
             considered = tn > 0;
-            if (best != SC::none() && SC::positive(best, self, kv, tot_own, a.m2)) {
+            if (best != SC::none() && SC::positive(best, self, kv, tot_own, m2)) {
                 a.target[v] = best_c;
                 twrite = 1;
             }
@@ -542,7 +545,9 @@ __global__ __launch_bounds__(kBlock) void k_move_fast(MoveCtx a, const uint32_t 
     const uint32_t idx = blockIdx.x * GPB + gi;
     const bool valid = idx < count;
     const uint32_t v = valid ? list[idx] : 0u;
-    const bool live = valid && v < a.v_end;
+    int64_t m2 = a.m2;
+    double tw = a.tw;
+    const bool live = valid && v < a.v_end && vertex_graph(a.bt, v, a.m2, a.tw, m2, tw);
     S best = SC::none();
     uint32_t best_c = LV_EMPTY;
     uint64_t nself = 0;
@@ -565,7 +570,7 @@ __global__ __launch_bounds__(kBlock) void k_move_fast(MoveCtx a, const uint32_t 
             for (int u = 0; u < kIlp; ++u) {
                 if (xs[u] == v) continue;
                 ++nself;
-                const S sc = SC::of(ws[u], 0u, kv, tot_own, ts[u], a.m2, a.tw);
+                const S sc = SC::of(ws[u], 0u, kv, tot_own, ts[u], m2, tw);
                 if (better_candidate(sc, xs[u], best, best_c, false)) { best = sc; best_c = xs[u]; }
             }
         }
@@ -576,7 +581,7 @@ __global__ __launch_bounds__(kBlock) void k_move_fast(MoveCtx a, const uint32_t 
         const uint64_t gself = group_sum_u64<G>(nself);
         if (live && lane == 0) {
             considered = gself > 0;
-            if (best != SC::none() && SC::positive(best, 0u, kv, tot_own, a.m2) && v < best_c) {
+            if (best != SC::none() && SC::positive(best, 0u, kv, tot_own, m2) && v < best_c) {
                 a.target[v] = best_c;
                 twrite = 1;
             }
@@ -600,7 +605,7 @@ __global__ __launch_bounds__(kBlock) void k_move_fast(MoveCtx a, const uint32_t 
                 }
             }
             considered = tn > 0;
-            if (best != SC::none() && SC::positive(best, 0u, kv, tot_own, a.m2) && v < best_c) {
+            if (best != SC::none() && SC::positive(best, 0u, kv, tot_own, m2) && v < best_c) {
                 a.target[v] = best_c;
                 twrite = 1;
             }
@@ -625,7 +630,9 @@ __global__ __launch_bounds__(kBlock) void k_hub_insert(HubCtx a, const uint32_t 
     uint64_t nself = 0;
     if (hi < count) {
         const uint32_t v = list[hi];
-        if (v < a.m.v_end) {
+        int64_t m2;
+        double tw;
+        if (v < a.m.v_end && vertex_graph(a.m.bt, v, a.m.m2, a.m.tw, m2, tw)) {
             const uint32_t slot = a.h.slot_of[v];
             const uint64_t off = a.h.tab_off[slot];
             const uint32_t lg = a.h.tab_log2[slot];
@@ -716,6 +723,9 @@ __global__ __launch_bounds__(kBlock) void k_hub_scan(HubCtx a, const uint32_t *_
     if (ok) {
         const uint32_t v = list[hi];
         slot = a.h.slot_of[v];
+        int64_t m2;
+        double tw;
+        (void)vertex_graph(a.m.bt, v, a.m.m2, a.m.tw, m2, tw);   // a halted graph's table is empty
         if (v < a.m.v_end) {
             const uint32_t own = a.m.comm[v];
             const uint64_t off = a.h.tab_off[slot];
@@ -732,7 +742,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_scan(HubCtx a, const uint32_t *_
                 a.h.slot[off + s] = kHubEmpty;
                 if (c == own) continue;
                 ++ncand;
-                const S sc = SC::of(W, self, kv, tot_own, a.m.tot[c], a.m.m2, a.m.tw);
+                const S sc = SC::of(W, self, kv, tot_own, a.m.tot[c], m2, tw);
                 if (better_candidate(sc, c, best, best_c, a.m.highest)) { best = sc; best_c = c; }
             }
         }
@@ -775,7 +785,9 @@ __global__ __launch_bounds__(kBlock) void k_hub_fast(HubCtx a, const uint32_t *_
     if (ok) {
         const uint32_t v = list[hi];
         slot = a.h.slot_of[v];
-        if (v < a.m.v_end) {
+        int64_t m2;
+        double tw;
+        if (v < a.m.v_end && vertex_graph(a.m.bt, v, a.m.m2, a.m.tw, m2, tw)) {
             const uint32_t kv = a.m.kdeg[v];
             const uint32_t tot_own = a.m.tot[v];
             const uint64_t beg = a.m.row[v], end = a.m.row[v + 1];
@@ -785,7 +797,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_fast(HubCtx a, const uint32_t *_
                 if (x == v) continue;
                 ++nself;
                 const uint32_t wgt = WEIGHTED ? a.m.wt[j] : 1u;
-                const S sc = SC::of(wgt, 0u, kv, tot_own, a.m.tot[x], a.m.m2, a.m.tw);
+                const S sc = SC::of(wgt, 0u, kv, tot_own, a.m.tot[x], m2, tw);
                 if (better_candidate(sc, x, best, best_c, false)) { best = sc; best_c = x; }
             }
         }
@@ -834,13 +846,15 @@ __global__ void k_hub_decide(HubCtx a, const uint32_t *__restrict__ list, uint32
         best_c = a.h.part_c[(size_t)slot * kHubParts + lane];
     }
     group_best<64, S>(best, best_c, highest);
-    if (lane == 0 && v < a.m.v_end) {
+    int64_t m2;
+    double tw;
+    if (lane == 0 && v < a.m.v_end && vertex_graph(a.m.bt, v, a.m.m2, a.m.tw, m2, tw)) {
         const uint32_t own = a.m.comm[v];
         const uint32_t kv = a.m.kdeg[v];
         const uint32_t tot_own = a.m.tot[own];
         const uint32_t self = a.h.self[slot];
         const bool has = best != SC::none();
-        if (has && SC::positive(best, self, kv, tot_own, a.m.m2) && (!fast || v < best_c))
+        if (has && SC::positive(best, self, kv, tot_own, m2) && (!fast || v < best_c))
             a.m.target[v] = best_c;
         // a hub row holds > 4096 entries and at most two self-loop entries, so
         // it always has candidates
@@ -860,6 +874,8 @@ struct CommitCtx {
     uint32_t *tot;
     uint8_t *mark;
     unsigned long long *stats;
+    const uint32_t *gid;          // batches: graph of each vertex (nullptr: one graph)
+    unsigned long long *g_dsig;   // batches: sigma_in change per graph
 };
 
 // Group (G <= 64) or `parts` workgroups (G == 256) per listed vertex.
@@ -877,8 +893,9 @@ __global__ __launch_bounds__(kBlock) void k_commit(CommitCtx a, const uint32_t *
         part = 0;
     }
     int64_t dsig = 0;
+    uint32_t v = 0;
     if (idx < count) {
-        const uint32_t v = list[idx];
+        v = list[idx];
         const uint32_t own = a.comm[v];
         const uint32_t c = a.target[v];
         if (c != own) {
@@ -912,6 +929,7 @@ __global__ __launch_bounds__(kBlock) void k_commit(CommitCtx a, const uint32_t *
             }
         }
     }
+    if (a.gid) wave_keyed_add(a.g_dsig, idx < count ? a.gid[v] : 0u, (uint64_t)dsig, dsig != 0);
     block_add_stats(a.stats, ST_DSIGMA, (uint64_t)dsig, -1, 0, -1, 0);
 }
 
@@ -921,23 +939,30 @@ __global__ __launch_bounds__(kBlock) void k_finalize(uint32_t *comm, const uint3
                                                      const uint32_t *tot, uint8_t *mark,
                                                      const uint8_t *bucket, uint32_t n,
                                                      ListPtrs next, uint32_t *next_counts,
-                                                     unsigned long long *stats) {
+                                                     unsigned long long *stats,
+                                                     const uint32_t *gid,
+                                                     unsigned long long *g_sumsq) {
     uint64_t moved = 0, sq = 0;
     const uint32_t ntiles = (n + kBlock - 1) / kBlock;
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint32_t v = t * kBlock + threadIdx.x;
         bool sel = false;
         int b = 0;
+        uint64_t q = 0;
+        uint32_t g = 0;
         if (v < n) {
             const uint32_t tv = target[v];
             if (tv != comm[v]) { comm[v] = tv; ++moved; }
-            sq += square_u64(tot[v]);
+            q = square_u64(tot[v]);
+            sq += q;
+            if (gid) g = gid[v];
             if (mark[v]) {
                 mark[v] = 0;
                 sel = true;
                 b = bucket[v];
             }
         }
+        if (gid) wave_keyed_add(g_sumsq, g, q, q != 0);   // gid is uniform: whole waves call it
         append_to_bucket(sel, v, b, next, next_counts);
     }
     block_add_stats(stats, ST_MOVED, moved, ST_SUMSQ, sq, -1, 0);
@@ -965,9 +990,43 @@ __global__ void k_new_degrees(const uint32_t *tot, const uint32_t *map, uint32_t
         if (tot[c] > 0) k_new[renumbered(map, (uint32_t)c)] = tot[c];
 }
 
+// ------------------------------------------------------- batches of graphs
+
+__global__ __launch_bounds__(kBlock) void k_batch_comm_counts(const uint32_t *tot,
+                                                              const uint32_t *gid, uint32_t n,
+                                                              unsigned long long *counts) {
+    const uint32_t ntiles = (n + kBlock - 1) / kBlock;
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t c = t * kBlock + threadIdx.x;
+        const bool has = c < n && tot[c] > 0;
+        wave_keyed_add(counts, has ? gid[c] : 0u, 1ull, has);
+    }
+}
+
+__global__ void k_batch_next_gid(const uint32_t *tot, const uint32_t *map, const uint32_t *gid,
+                                 uint32_t n, uint32_t *gid_new) {
+    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n;
+         c += (size_t)gridDim.x * blockDim.x)
+        if (tot[c] > 0) gid_new[renumbered(map, (uint32_t)c)] = gid[c];
+}
+
 }  // namespace
 
 // ====================================================================== host
+
+void batch_comm_counts(const uint32_t *tot, const uint32_t *gid, uint32_t n,
+                       unsigned long long *counts, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_batch_comm_counts, dim3(grid_for(n, kBlock * 4, 16384u)), dim3(kBlock), 0,
+                       s, tot, gid, n, counts);
+}
+
+void batch_next_gid(const uint32_t *tot, const uint32_t *map, const uint32_t *gid, uint32_t n,
+                    uint32_t *gid_new, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_batch_next_gid, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, tot,
+                       map, gid, n, gid_new);
+}
 
 void fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
     if (!n) return;
@@ -1150,6 +1209,7 @@ void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
     m.row = a.g.row; m.col = a.g.col; m.wt = a.g.wt; m.kdeg = a.g.kdeg;
     m.comm = a.comm; m.tot = a.tot; m.target = a.target;
     m.m2 = a.m2; m.tw = a.tw; m.v_end = a.v_end; m.highest = a.highest;
+    m.bt = a.bt;
     const bool wtd = a.g.wt != nullptr;
     const bool fast = a.kind == 0;
     for (int b = 0; b < kNumBuckets; ++b) {
@@ -1220,6 +1280,7 @@ void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *cou
     CommitCtx c;
     c.row = a.g.row; c.col = a.g.col; c.wt = a.g.wt; c.kdeg = a.g.kdeg;
     c.comm = a.comm; c.target = a.target; c.tot = a.tot; c.mark = a.mark; c.stats = a.stats;
+    c.gid = a.bt.gid; c.g_dsig = a.bt.dsig;
     const bool wtd = a.g.wt != nullptr;
     for (int b = 0; b < kNumBuckets; ++b) {
         const uint32_t cnt = counts[b];
@@ -1257,7 +1318,8 @@ void finalize_pass(const SweepArgs &a, uint32_t *const *next_lists, uint32_t *ne
     if (!n) return;
     hipLaunchKernelGGL(k_finalize, dim3(grid_for(n, kBlock * 4, 16384u)), dim3(kBlock), 0, s,
                        const_cast<uint32_t *>(a.comm), (const uint32_t *)a.target,
-                       (const uint32_t *)a.tot, a.mark, a.bucket, n, lp, next_counts, a.stats);
+                       (const uint32_t *)a.tot, a.mark, a.bucket, n, lp, next_counts, a.stats,
+                       a.bt.gid, a.bt.sumsq);
 }
 
 void renumber_flags(const uint32_t *tot, uint32_t n, uint32_t *flags, hipStream_t s) {

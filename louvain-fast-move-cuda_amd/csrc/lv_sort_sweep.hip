@@ -18,6 +18,7 @@ namespace lv {
 namespace {
 
 struct SortCtx {
+    BatchView bt;
     const uint64_t *row;
     const uint32_t *col;
     const uint32_t *wt;
@@ -58,7 +59,9 @@ __global__ __launch_bounds__(kBlock) void k_sort_gather(SortCtx a, const uint32_
         const uint64_t beg = a.row[v], end = a.row[v + 1];
         const uint64_t out = a.off[list_base + idx] - beg;
         if (lane == 0 && part == 0) { a.rbeg[v] = 0u; a.rend[v] = 0u; }
-        const bool live = v < a.v_end;
+        int64_t m2;
+        double tw;
+        const bool live = v < a.v_end && vertex_graph(a.bt, v, a.m2, a.tw, m2, tw);
         const uint64_t stride = (uint64_t)G * (G == kBlock ? parts : 1u);
         for (uint64_t j = beg + (uint64_t)part * G + lane; j < end; j += stride) {
             const uint32_t x = stream_load(&a.col[j]);
@@ -119,7 +122,9 @@ __global__ __launch_bounds__(kBlock) void k_sort_decide(SortCtx a, const uint32_
     const uint32_t idx = blockIdx.x * GPB + threadIdx.x / G;
     const bool valid = idx < count;
     const uint32_t v = valid ? list[idx] : 0u;
-    const bool live = valid && v < a.v_end;
+    int64_t m2 = a.m2;
+    double tw = a.tw;
+    const bool live = valid && v < a.v_end && vertex_graph(a.bt, v, a.m2, a.tw, m2, tw);
     const uint64_t cmask = (1ull << a.bits) - 1;
     S best = SC::none();
     uint32_t best_c = a.highest ? 0u : LV_EMPTY;
@@ -144,7 +149,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_decide(SortCtx a, const uint32_
             if (c == own) continue;
             ++ncand;
             const uint32_t W = (uint32_t)(we[r] - ws[r]);
-            const S sc = SC::of(W, self, kv, tot_own, a.tot[c], a.m2, a.tw);
+            const S sc = SC::of(W, self, kv, tot_own, a.tot[c], m2, tw);
             if (better_candidate(sc, c, best, best_c, a.highest)) { best = sc; best_c = c; }
         }
     }
@@ -153,7 +158,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_decide(SortCtx a, const uint32_
         group_best<G, S>(best, best_c, a.highest);
         if (live && lane == 0) {
             considered = r1 > r0;
-            if (best != SC::none() && SC::positive(best, self, kv, tot_own, a.m2)) {
+            if (best != SC::none() && SC::positive(best, self, kv, tot_own, m2)) {
                 a.target[v] = best_c;
                 twrite = 1;
             }
@@ -172,7 +177,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_decide(SortCtx a, const uint32_
                     best_c = r_c[i];
                 }
             considered = r1 > r0;
-            if (best != SC::none() && SC::positive(best, self, kv, tot_own, a.m2)) {
+            if (best != SC::none() && SC::positive(best, self, kv, tot_own, m2)) {
                 a.target[v] = best_c;
                 twrite = 1;
             }
@@ -223,6 +228,7 @@ void sort_sweep(const SweepArgs &x, uint32_t *const *lists, const uint32_t *coun
     a.m2 = x.m2;
     a.tw = x.tw;
     a.highest = x.highest;
+    a.bt = x.bt;
     const bool wtd = x.g.wt != nullptr;
     for (int b = 0; b < kNumBuckets; ++b) {
         const uint32_t cnt = counts[b];

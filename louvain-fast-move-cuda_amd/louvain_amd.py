@@ -68,6 +68,15 @@ class KernelStats(C.Structure):
                 ("variant_bytes", C.c_uint64 * 32), ("variant_entries", C.c_uint64 * 32)]
 
 
+class BatchResult(C.Structure):
+    _fields_ = [("modularity", C.POINTER(C.c_double)),
+                ("n_of_best_communities", C.POINTER(C.c_uint32)),
+                ("levels", C.POINTER(C.c_int32)), ("sweeps", C.POINTER(C.c_int32)),
+                ("best_communities", C.POINTER(C.c_uint32)),
+                ("union_levels", C.c_int32), ("union_sweeps", C.c_int32),
+                ("seconds", C.c_double), ("traversed_entries", C.c_uint64)]
+
+
 # name -> (restype, argtypes); every symbol include/lv_api.h declares
 _vp = C.c_void_p
 _u32p = C.POINTER(C.c_uint32)
@@ -106,6 +115,11 @@ SIGNATURES = {
     "lv_last_run_kernel_stats": (C.c_int, [C.POINTER(KernelStats)]),
     "lv_kernel_variant_name": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.c_size_t]),
     "lv_bandwidth_probe": (C.c_int, [C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
+    "lv_batch_create": (C.c_int, [C.POINTER(_vp), C.c_uint32, _vp, C.POINTER(_vp)]),
+    "lv_batch_destroy": (None, [_vp]),
+    "lv_batch_info": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                C.POINTER(C.c_uint64)]),
+    "lv_batch_louvain": (C.c_int, [_vp, C.POINTER(Params), C.POINTER(BatchResult)]),
     "lv_gen_rmat": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64,
                               _vp, _vp, C.c_uint64, C.POINTER(C.c_uint64),
                               C.POINTER(C.c_uint32)]),
@@ -306,6 +320,52 @@ class Community:
         k = KernelStats()
         _check(lib.lv_kernel_stats_get(self.h, C.byref(k)))
         return _rec(k)
+
+
+class Batch:
+    """Many graphs run together as one disjoint union (lv_batch_*)."""
+
+    def __init__(self, graphs, stream=None):
+        self.graphs = list(graphs)
+        arr = (_vp * len(self.graphs))(*[g.h for g in self.graphs])
+        h = _vp()
+        _check(lib.lv_batch_create(arr, len(self.graphs), stream, C.byref(h)))
+        self.h = h
+        self.sizes = [g.n for g in self.graphs]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.lv_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def info(self):
+        G = C.c_uint32(); n = C.c_uint64(); E = C.c_uint64()
+        _check(lib.lv_batch_info(self.h, C.byref(G), C.byref(n), C.byref(E)))
+        return G.value, n.value, E.value
+
+    def louvain(self, prm=None, best=True):
+        G = len(self.graphs)
+        q = np.empty(G, np.float64); nc = np.empty(G, np.uint32)
+        lev = np.empty(G, np.int32); sw = np.empty(G, np.int32)
+        out = np.empty(sum(self.sizes), np.uint32) if best else None
+        r = BatchResult()
+        r.modularity = q.ctypes.data_as(C.POINTER(C.c_double))
+        r.n_of_best_communities = nc.ctypes.data_as(C.POINTER(C.c_uint32))
+        r.levels = lev.ctypes.data_as(C.POINTER(C.c_int32))
+        r.sweeps = sw.ctypes.data_as(C.POINTER(C.c_int32))
+        if best:
+            r.best_communities = out.ctypes.data_as(C.POINTER(C.c_uint32))
+        _check(lib.lv_batch_louvain(self.h, C.byref(prm or params()), C.byref(r)))
+        bests = None
+        if best:
+            offs = np.concatenate([[0], np.cumsum(self.sizes)])
+            bests = [out[offs[i]:offs[i + 1]] for i in range(G)]
+        return {"modularity": q, "n_comm": nc, "levels": lev, "sweeps": sw, "best": bests,
+                "union_levels": r.union_levels, "union_sweeps": r.union_sweeps,
+                "seconds": r.seconds, "traversed_entries": r.traversed_entries}
 
 
 def _run_result(call, n, sweeps_cap, levels_cap, hash_trace, best):

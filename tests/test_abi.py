@@ -96,6 +96,20 @@ def test_community_without_device_reports_no_device(fixture_path):
     with pytest.raises(L.LouvainError) as e:
         g.louvain()
     assert e.value.status == 8
+    with pytest.raises(L.LouvainError) as e:
+        L.Batch([g, g])
+    assert e.value.status == 8
+
+
+def test_batch_argument_errors():
+    import ctypes as C
+    h = C.c_void_p()
+    assert L.lib.lv_batch_create(None, 3, None, C.byref(h)) == 1          # LV_ERR_ARG
+    arr = (C.c_void_p * 2)(None, None)
+    assert L.lib.lv_batch_create(arr, 2, None, C.byref(h)) == 1
+    assert L.lib.lv_batch_create(arr, 0, None, C.byref(h)) == 1
+    assert L.lib.lv_batch_louvain(None, None, None) == 1
+    assert L.lib.lv_batch_info(None, None, None, None) == 1
 
 
 def test_rmat_generator_properties():
