@@ -8,9 +8,12 @@ One step = one full Louvain run to convergence (all levels, the reference's
 
 value = traversed directed entries per second over the whole run (sum over
 sweeps of the entries of the vertices examined, divided by the step time).
-The roofline block covers the best-move pass (every degree bucket's kernel of
-each sweep), timed with HIP events on the library stream; the CPU baseline is
-the single-threaded oracle on the first sweeps of level 0 of the same graph.
+The roofline block covers the dominant best-move kernel, timed with HIP
+events on the library stream; the CPU baseline is the single-threaded oracle
+on the first sweeps of level 0 of the same graph.  The `batched` sub-object
+is the batched configuration (4,096 Barabasi-Albert graphs in one
+disjoint-union launch sequence, graphs/s, against a loop of per-graph runs);
+`--workload batched` prints it as its own line with value = graphs/s.
 """
 import argparse
 import json
@@ -56,6 +59,134 @@ def cpu_baseline(row, col, sweeps):
                       f"nproc {os.cpu_count()}"}
 
 
+def modularity_fp64(row, col, w, part):
+    """Q of a partition recomputed in FP64 with numpy (community.h:104's definition)."""
+    import numpy as np
+    row = np.asarray(row, np.int64)
+    part = np.asarray(part, np.int64)
+    n = len(row) - 1
+    wt = np.ones(len(col), np.float64) if w is None else np.asarray(w, np.float64)
+    src = np.repeat(np.arange(n), np.diff(row))
+    two_m = wt.sum()
+    sig = wt[part[src] == part[np.asarray(col, np.int64)]].sum()
+    tot = np.bincount(part, weights=np.bincount(src, weights=wt, minlength=n))
+    return float(sig / two_m - np.dot(tot, tot) / (two_m * two_m))
+
+
+def batched_graphs(L, count, seed, n_lo=200, n_hi=200_000, m_lo=2, m_hi=8):
+    """Barabasi-Albert graphs, n log-uniform in [n_lo, n_hi], m uniform in
+    {m_lo..m_hi}; graph i uses generator seed seed*1e6 + i."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(n_lo), np.log(n_hi), count)).astype(np.int64)
+    ms = rng.integers(m_lo, m_hi + 1, count)
+
+    def one(i):
+        ea, eb = L.gen_ba(int(ns[i]), int(ms[i]), seed=seed * 1_000_000 + i)
+        return L.Graph.from_edges(ea, eb)
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        return list(ex.map(one, range(count)))
+
+
+def batched_result(args, L):
+    """4,096 BA graphs in one batch versus a loop of per-graph runs."""
+    import numpy as np
+    t = time.perf_counter()
+    graphs = batched_graphs(L, args.graphs, 4)
+    n_tot = sum(g.n for g in graphs)
+    e_tot = sum(g.E for g in graphs)
+    log(f"{len(graphs)} BA graphs: n={n_tot} E={e_tot} ({time.perf_counter() - t:.1f}s)")
+    prm = L.params(mode=args.mode, sweep_threshold=args.threshold, level_threshold=args.threshold)
+    t = time.perf_counter()
+    B = L.Batch(graphs)
+    log(f"batch union + upload ({time.perf_counter() - t:.1f}s)")
+    for i in range(args.warmup):
+        r = B.louvain(prm, best=False)
+        log(f"warmup {i}: {r['seconds']:.3f} s")
+    times, res = [], None
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        res = B.louvain(prm, best=(i == args.steps - 1))
+        times.append(time.perf_counter() - t0)
+        log(f"step {i}: {times[-1]:.3f} s, union levels {res['union_levels']}, "
+            f"union sweeps {res['union_sweeps']}, traversed {res['traversed_entries']}")
+    batch_s = sum(times) / len(times)
+    # the same graphs one after the other, each resident (upload untimed)
+    order = list(range(len(graphs)))
+    if args.loop_sample > 1:
+        order = order[::args.loop_sample]
+    loop_s, q_loop, t_lat = 0.0, {}, []
+    for k, i in enumerate(order):
+        c = L.Community(graphs[i])
+        t0 = time.perf_counter()
+        c.reset()
+        r = c.run(prm, sweeps_cap=1, levels_cap=1, best=False, n0=graphs[i].n)
+        dt = time.perf_counter() - t0
+        loop_s += dt
+        t_lat.append(dt)
+        q_loop[i] = r["modularity"]
+        c.close()
+        if k % 512 == 0:
+            log(f"per-graph loop: {k}/{len(order)} graphs, {loop_s:.2f} s")
+    loop_gps = len(order) / loop_s
+    dq = max(abs(res["modularity"][i] - q_loop[i]) for i in order)
+    # FP64 recomputation of Q for a few graphs (the largest and the smallest ones)
+    by_size = sorted(range(len(graphs)), key=lambda i: graphs[i].E)
+    check = by_size[:2] + by_size[len(by_size) // 2:len(by_size) // 2 + 2] + by_size[-4:]
+    dq64 = 0.0
+    for i in check:
+        row, col, w = graphs[i].csr()[:3]
+        dq64 = max(dq64, abs(res["modularity"][i] - modularity_fp64(row, col, None, res["best"][i])))
+    gps = len(graphs) / batch_s
+    return {
+        "graphs_per_s": gps,
+        "seconds_per_batch": batch_s,
+        "steps": args.steps,
+        "config": {"workload": f"batched Louvain: {len(graphs)} Barabasi-Albert graphs, n "
+                               f"log-uniform in [200, 200000], m uniform in {{2..8}}, seed 4, "
+                               f"unit weights, one disjoint-union launch sequence",
+                   "graphs": len(graphs), "n": n_tot, "E": e_tot, "mode": args.mode,
+                   "threshold": args.threshold},
+        "edges_per_s": res["traversed_entries"] / batch_s,
+        "per_graph_loop": {"graphs_per_s": loop_gps, "graphs_timed": len(order),
+                           "seconds": loop_s,
+                           "note": "each graph resident, reset + run timed, upload not timed"},
+        "speedup_vs_loop": gps / loop_gps,
+        "max_abs_dq_batch_vs_loop": dq,
+        "max_abs_dq_vs_fp64": dq64,
+        "fp64_checked_graphs": len(check),
+        "mean_q": float(np.mean(res["modularity"])),
+        "union_levels": res["union_levels"],
+        "union_sweeps": res["union_sweeps"],
+    }
+
+
+def bench_batched(args, L):
+    """The batched configuration as its own bench line (value = graphs/s)."""
+    b = batched_result(args, L)
+    out = {
+        "metric": METRIC,
+        "value": b["graphs_per_s"],
+        "unit": "graphs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": b["seconds_per_batch"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic",
+        "config": b.pop("config"),
+    }
+    for k in ("graphs_per_s", "seconds_per_batch", "steps"):
+        b.pop(k)
+    out.update(b)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,10 +195,16 @@ def main():
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--mode", default="hash")
     ap.add_argument("--threshold", type=float, default=0.001)
-    ap.add_argument("--cpu-sweeps", type=int, default=2)
+    ap.add_argument("--cpu-sweeps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe-bytes", type=int, default=1 << 30,
                     help="size of the plain copy kernel run as the bandwidth reference")
+    ap.add_argument("--workload", default="rmat", choices=["rmat", "batched"])
+    ap.add_argument("--no-batched", action="store_true",
+                    help="rmat: leave out the batched configuration's sub-object")
+    ap.add_argument("--graphs", type=int, default=4096, help="batched: number of graphs")
+    ap.add_argument("--loop-sample", type=int, default=1,
+                    help="batched: time the per-graph loop on every k-th graph only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -77,6 +214,8 @@ def main():
     import louvain_amd as L
     if L.device_count() < 1:
         raise SystemExit("bench.py: no GPU visible")
+    if args.workload == "batched":
+        return bench_batched(args, L)
     copy_gbs = L.bandwidth_probe(args.probe_bytes, 5) if args.probe_bytes > 0 else None
     log(f"plain copy kernel: {copy_gbs} GB/s")
     t = time.perf_counter()
@@ -168,6 +307,11 @@ def main():
         row, col, _ = g.csr()
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(row, col, args.cpu_sweeps)
+        del row, col
+    del c, g
+    if not args.no_batched:
+        log("batched configuration ...")
+        out["batched"] = batched_result(args, L)
     print(json.dumps(out), flush=True)
     return 0
 
