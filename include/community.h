@@ -48,6 +48,9 @@ struct Community {
 
     explicit Community(GraphHost &g, hipStream_t stream = nullptr);  // community.h:31-46
     Community(const GraphHost &g, hipStream_t stream);
+    // No graph yet: the owner installs each level's graph and calls update()
+    // (the serving loop, whose level graphs are assembled from many jobs).
+    explicit Community(hipStream_t stream);
     ~Community();
     Community(Community &&) noexcept;
     Community &operator=(Community &&) noexcept;

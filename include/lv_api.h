@@ -213,6 +213,44 @@ lv_status lv_batch_info(const lv_batch *b, uint32_t *n_graphs, uint64_t *n_verti
  * (one chunk); a larger graph with chunk_emulation set is LV_ERR_ARG. */
 lv_status lv_batch_louvain(lv_batch *b, const lv_params *p, lv_batch_result *r);
 
+/* ---- served Louvain: continuous batching of submitted graphs ----
+ * No counterpart in the reference.  Jobs are admitted at union-level
+ * boundaries while their estimated device footprint fits hbm_budget_bytes,
+ * run together on the device (one disjoint union per level, every job with
+ * its own level loop), and retired when their level loop ends; a job's result
+ * equals lv_louvain on its graph alone.  One host thread drives a server. */
+typedef struct lv_server lv_server;
+
+typedef struct lv_job_result {
+    lv_status status;                 /* LV_OK, or why the job could not run */
+    double modularity;
+    uint32_t n_of_best_communities;
+    uint32_t n;                       /* vertices of the job's graph */
+    int32_t levels;
+    int32_t sweeps;
+    double seconds_queued;            /* submit -> admission */
+    double seconds_total;             /* submit -> retirement */
+} lv_job_result;
+
+lv_status lv_server_create(const lv_params *p, uint64_t hbm_budget_bytes, void *stream,
+                           lv_server **out);
+void lv_server_destroy(lv_server *s);
+/* The graph must stay valid until the job is retired.  A graph whose
+ * footprint exceeds the budget is LV_ERR_RANGE. */
+lv_status lv_server_submit(lv_server *s, const lv_graph *g, uint64_t *job_id);
+/* One union level; counts after it (each may be NULL). */
+lv_status lv_server_step(lv_server *s, uint32_t *n_active, uint32_t *n_queued,
+                         uint32_t *n_retired);
+/* Ids of retired jobs not yet taken, oldest first: up to cap are written and
+ * *count says how many; the rest stay for the next call. */
+lv_status lv_server_take_finished(lv_server *s, uint64_t *ids, size_t cap, size_t *count);
+/* Result of a retired job; best may be NULL (else n >= the job's vertex
+ * count).  LV_ERR_STATE if the job is not retired.  release != 0 frees it. */
+lv_status lv_server_result(lv_server *s, uint64_t job_id, lv_job_result *r, uint32_t *best,
+                           size_t n, int release);
+/* Estimated device bytes of a job while it is active. */
+lv_status lv_server_footprint(const lv_graph *g, uint64_t *bytes);
+
 /* Seeded synthetic graph generators (host, multithreaded; counter-based PRNG,
  * so the output is independent of the thread count).  Edges are written as
  * (a[i], b[i], w[i]) with dense ids; *m receives the count.  Pass NULL arrays

@@ -220,6 +220,17 @@ GraphDevice::GraphDevice(const GraphHost &g, hipStream_t s) {
 // --------------------------------------------------------------- Community
 
 Community::Community(GraphHost &g, hipStream_t stream) { init(g, stream); }
+
+Community::Community(hipStream_t stream) {
+    if (stream) {
+        streams[0] = stream;
+    } else {
+        LV_HIP(hipStreamCreateWithFlags(&streams[0], hipStreamNonBlocking));
+        own_stream_ = true;
+    }
+    streams[1] = streams[0];
+    state = std::make_unique<SweepState>();
+}
 Community::Community(const GraphHost &g, hipStream_t stream) { init(g, stream); }
 
 void Community::init(const GraphHost &g, hipStream_t stream) {

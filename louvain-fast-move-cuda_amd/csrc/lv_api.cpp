@@ -11,6 +11,7 @@
 
 #include "../../include/batch.h"
 #include "../../include/community.h"
+#include "../../include/server.h"
 #include "../../include/modularityAlgorithms.h"
 #include "engine_state.h"
 #include "lv_kernels.h"
@@ -21,6 +22,10 @@ struct lv_graph {
 struct lv_batch {
     GraphBatch b;
     lv_batch(const std::vector<const GraphHost *> &g, hipStream_t s) : b(g, s) {}
+};
+struct lv_server {
+    LouvainServer s;
+    lv_server(const lv_params &p, uint64_t b, hipStream_t st) : s(p, b, st) {}
 };
 struct lv_community {
     Community c;
@@ -432,6 +437,59 @@ lv_status lv_batch_louvain(lv_batch *b, const lv_params *p, lv_batch_result *r) 
     lv_params d;
     lv_default_params(&d);
     return guarded([&] { b->b.run(p ? *p : d, r); });
+}
+
+lv_status lv_server_create(const lv_params *p, uint64_t hbm_budget_bytes, void *stream,
+                           lv_server **out) {
+    if (!out || hbm_budget_bytes == 0) return LV_ERR_ARG;
+    *out = nullptr;
+    if (lv_status st = need_device()) return st;
+    lv_params d;
+    lv_default_params(&d);
+    return guarded([&] { *out = new lv_server(p ? *p : d, hbm_budget_bytes, (hipStream_t)stream); });
+}
+
+void lv_server_destroy(lv_server *s) { delete s; }
+
+lv_status lv_server_submit(lv_server *s, const lv_graph *g, uint64_t *job_id) {
+    if (!s || !g || !job_id) return LV_ERR_ARG;
+    return guarded([&] { *job_id = s->s.submit(&g->g); });
+}
+
+lv_status lv_server_step(lv_server *s, uint32_t *n_active, uint32_t *n_queued,
+                         uint32_t *n_retired) {
+    if (!s) return LV_ERR_ARG;
+    return guarded([&] {
+        const uint32_t r = s->s.step();
+        if (n_retired) *n_retired = r;
+        if (n_active) *n_active = s->s.active();
+        if (n_queued) *n_queued = s->s.queued();
+    });
+}
+
+lv_status lv_server_take_finished(lv_server *s, uint64_t *ids, size_t cap, size_t *count) {
+    if (!s || !count || (cap && !ids)) return LV_ERR_ARG;
+    return guarded([&] {
+        const std::vector<uint64_t> v = s->s.take_finished(cap);   // the rest stay pending
+        std::copy(v.begin(), v.end(), ids);
+        *count = v.size();
+    });
+}
+
+lv_status lv_server_result(lv_server *s, uint64_t job_id, lv_job_result *r, uint32_t *best,
+                           size_t n, int release) {
+    if (!s) return LV_ERR_ARG;
+    lv_status st = LV_OK;
+    lv_status g = guarded([&] {
+        if (!s->s.result(job_id, r, best, n, release != 0)) st = LV_ERR_STATE;
+    });
+    return g != LV_OK ? g : st;
+}
+
+lv_status lv_server_footprint(const lv_graph *g, uint64_t *bytes) {
+    if (!g || !bytes) return LV_ERR_ARG;
+    *bytes = LouvainServer::footprint(g->g);
+    return LV_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,429 @@
+// server.cpp -- the serving loop: continuous batching of Louvain jobs.
+//
+// Each step is one union level.  Per job it performs what one iteration of
+// ModularityAlgorithms::run does for a single graph (modularityAlgorithms.h:
+// 31-67, with the sweep loop of phaseOptimization.cuh:158-181): jobs differ
+// only in where their level starts (a just-admitted job at its level 0, a
+// continuing job at its next level), and every job's level starts with the
+// fast sweep, so sharing the sweep index within a union level is exact.
+#include "../../include/server.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <unordered_map>
+
+#include "../../include/phaseAggregation.h"
+#include "../../include/phaseOptimization.h"
+#include "engine_state.h"
+#include "lv_kernels.h"
+#include "lv_server.h"
+
+using namespace lv;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+struct Job {
+    uint64_t id = 0;
+    const GraphHost *g = nullptr;
+    uint32_t n0 = 0;
+    uint64_t E0 = 0;
+    uint64_t bytes = 0;
+    double tw = 0;
+    int64_t m2 = 0;
+    int64_t sig = 0;
+    uint64_t sq = 0;
+    double q = 0, q_start = 0;
+    int32_t levels = 0, sweeps = 0;
+    bool cont = true;
+    bool fresh = true;             // the next union takes its level-0 graph
+    GraphDevice staging;           // level-0 graph from admission to the first union
+    DeviceArray<uint32_t> best;    // [n0] local community ids
+    uint32_t voff = 0, n = 0;      // vertex range in the current union
+    uint32_t cvoff = 0, cn = 0;    // vertex range in the last contracted graph
+    uint64_t ce0 = 0, cE = 0;      // its entries
+    uint32_t ncomm = 0;
+    lv_status status = LV_OK;
+    Clock::time_point t_submit, t_admit, t_done;
+    std::vector<uint32_t> best_host;
+};
+
+double q_of(int64_t sigma_in, uint64_t sumsq, double tw) {
+    return (double)sigma_in / (2 * tw) - (double)sumsq / (4 * tw * tw);   // community.h:104
+}
+
+double secs(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double>(b - a).count();
+}
+
+}  // namespace
+
+struct LouvainServer::Impl {
+    lv_params p;
+    uint64_t budget;
+    uint64_t used = 0, next_id = 1;
+    Community C;
+    std::deque<std::unique_ptr<Job>> queue;
+    std::vector<std::unique_ptr<Job>> active;
+    std::unordered_map<uint64_t, std::unique_ptr<Job>> done;
+    std::vector<uint64_t> newly_done;
+    GraphDevice ubuf;                       // the current union
+    DeviceArray<Segment> segs;
+    DeviceArray<JobBest> jobs;
+    DeviceArray<uint32_t> voff_dev, idx_dev;
+    DeviceArray<uint64_t> eoff_dev;
+    DeviceArray<unsigned long long> idstats;
+    uint32_t host_cap = 0;
+
+    Impl(const lv_params &pp, uint64_t b, hipStream_t s) : p(pp), budget(b), C(s) {
+        C.batch = std::make_unique<BatchState>();
+    }
+
+    hipStream_t stream() { return C.stream(); }
+
+    void ensure_slots(uint32_t A) {
+        BatchState &B = *C.batch;
+        if (A > host_cap) {
+            const uint32_t cap = std::max<uint32_t>(A, 2 * host_cap);
+            if (B.host_acc) LV_HIP(hipHostFree(B.host_acc));
+            if (B.host_live) LV_HIP(hipHostFree(B.host_live));
+            B.host_acc = nullptr;
+            B.host_live = nullptr;
+            LV_HIP(hipHostMalloc((void **)&B.host_acc, 3 * (size_t)cap * sizeof(unsigned long long),
+                                 hipHostMallocDefault));
+            LV_HIP(hipHostMalloc((void **)&B.host_live, cap, hipHostMallocDefault));
+            host_cap = cap;
+        }
+        B.n_graphs = A;
+        B.m2_dev.ensure(A);
+        B.tw_dev.ensure(A);
+        B.live_dev.ensure(A);
+        B.acc.ensure(3 * (size_t)A);
+    }
+
+    void retire(std::unique_ptr<Job> j) {
+        j->t_done = Clock::now();
+        j->staging = GraphDevice();
+        j->best.release();
+        used -= j->bytes;
+        newly_done.push_back(j->id);
+        const uint64_t id = j->id;
+        done[id] = std::move(j);
+    }
+
+    // Upload, identity-partition parts (Community::update for one graph).
+    void admit(std::unique_ptr<Job> j) {
+        hipStream_t s = stream();
+        j->t_admit = Clock::now();
+        used += j->bytes;
+        try {
+            j->staging = GraphDevice(*j->g, s);
+            j->E0 = j->staging.n_entries;
+            idstats.ensure((size_t)kNumStats * kStatSlots);
+            fill_u64(idstats.data(), 0ull, idstats.size(), s);
+            identity_parts(level_view(j->staging), idstats.data(), s);
+            std::vector<unsigned long long> h(idstats.size());
+            LV_HIP(hipMemcpyAsync(h.data(), idstats.data(), h.size() * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, s));
+            j->best.resize(std::max<size_t>(j->n0, 1));
+            iota_u32(j->best.data(), j->n0, s);
+            LV_HIP(hipStreamSynchronize(s));
+            j->sig = 0;
+            j->sq = 0;
+            for (int i = 0; i < kStatSlots; ++i) {
+                j->sig += (int64_t)h[ST_DSIGMA * kStatSlots + i];
+                j->sq += h[ST_SUMSQ * kStatSlots + i];
+            }
+            j->tw = j->g->total_weight;
+            j->m2 = (int64_t)(2.0 * j->g->total_weight);
+            j->q = q_of(j->sig, j->sq, j->tw);
+        } catch (const lv::Error &e) {
+            j->status = e.status;
+            retire(std::move(j));
+            return;
+        } catch (const std::bad_alloc &) {
+            j->status = LV_ERR_OOM;
+            retire(std::move(j));
+            return;
+        }
+        active.push_back(std::move(j));
+    }
+
+    // The level graph: every active job's current graph side by side.
+    void assemble() {
+        hipStream_t s = stream();
+        const uint32_t A = (uint32_t)active.size();
+        std::vector<Segment> sg(A);
+        std::vector<uint32_t> voff(A + 1);
+        uint64_t n_tot = 0, e_tot = 0, maxlen = 0;
+        const GraphDevice &cg = C.graph;   // the last contraction
+        for (uint32_t i = 0; i < A; ++i) {
+            Job &j = *active[i];
+            Segment &x = sg[i];
+            if (j.fresh) {
+                x.row = j.staging.neighboorhood_sum.data();
+                x.col = j.staging.edge_destination.data();
+                x.w = j.staging.unit_weights ? nullptr : j.staging.weights.data();
+                x.kdeg = j.staging.tot_weight_per_nodes.data();
+                x.src_voff = 0;
+                x.n = j.n0;
+                x.E = j.E0;
+            } else {
+                x.row = cg.neighboorhood_sum.data();
+                x.col = cg.edge_destination.data();
+                x.w = cg.weights.data();
+                x.kdeg = cg.tot_weight_per_nodes.data();
+                x.src_voff = j.cvoff;
+                x.n = j.cn;
+                x.E = j.cE;
+            }
+            x.dst_voff = (uint32_t)n_tot;
+            x.dst_eoff = e_tot;
+            j.voff = (uint32_t)n_tot;
+            j.n = x.n;
+            voff[i] = (uint32_t)n_tot;
+            n_tot += x.n;
+            e_tot += x.E;
+            maxlen = std::max<uint64_t>(maxlen, std::max<uint64_t>(x.n, x.E));
+            if (n_tot > 0x7FFFFFFFull) throw Error(LV_ERR_RANGE, "active jobs exceed 2^31-1 vertices");
+        }
+        voff[A] = (uint32_t)n_tot;
+        GraphDevice &U = ubuf;
+        U.neighboorhood_sum.ensure((size_t)n_tot + 1);
+        U.edge_destination.ensure(std::max<size_t>(e_tot, 1));
+        U.weights.ensure(std::max<size_t>(e_tot, 1));
+        U.tot_weight_per_nodes.ensure(std::max<size_t>(n_tot, 1));
+        U.n_nodes = (uint32_t)n_tot;
+        U.n_entries = e_tot;
+        U.n_links = e_tot / 2;
+        U.total_weight = 1.0;   // per-job m lives in the batch view
+        U.m2 = 0;
+        U.unit_weights = false;
+        segs.ensure(A);
+        LV_HIP(hipMemcpyAsync(segs.data(), sg.data(), A * sizeof(Segment), hipMemcpyHostToDevice, s));
+        segment_copy(segs.data(), A, maxlen, U.neighboorhood_sum.data(), U.edge_destination.data(),
+                     U.weights.data(), U.tot_weight_per_nodes.data(), s);
+        set_u64(U.neighboorhood_sum.data() + n_tot, e_tot, s);
+        BatchState &B = *C.batch;
+        ensure_slots(A);
+        B.gid_buf[0].ensure(std::max<size_t>(n_tot, 1));
+        voff_dev.ensure((size_t)A + 1);
+        LV_HIP(hipMemcpyAsync(voff_dev.data(), voff.data(), (A + 1) * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, s));
+        fill_gid_ranges(voff_dev.data(), A, (uint32_t)n_tot, B.gid_buf[0].data(), s);
+        B.gid = B.gid_buf[0].data();
+        B.m2.resize(A);
+        B.tw.resize(A);
+        for (uint32_t i = 0; i < A; ++i) {
+            B.m2[i] = active[i]->m2;
+            B.tw[i] = active[i]->tw;
+        }
+        LV_HIP(hipMemcpyAsync(B.m2_dev.data(), B.m2.data(), A * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        LV_HIP(hipMemcpyAsync(B.tw_dev.data(), B.tw.data(), A * sizeof(double), hipMemcpyHostToDevice, s));
+        LV_HIP(hipStreamSynchronize(s));
+        for (auto &j : active)
+            if (j->fresh) {
+                j->staging = GraphDevice();
+                j->fresh = false;
+            }
+        C.graph = GraphDevice::view_of(U);
+        C.level = 0;
+        C.update(0, 0);   // identity partition; per-job parts live on the host
+    }
+};
+
+LouvainServer::LouvainServer(const lv_params &p, uint64_t hbm_budget_bytes, hipStream_t stream)
+    : d_(std::make_unique<Impl>(p, hbm_budget_bytes, stream)) {
+    if (p.chunk_emulation) throw Error(LV_ERR_ARG, "chunk emulation is not served");
+}
+
+LouvainServer::~LouvainServer() = default;
+
+uint64_t LouvainServer::footprint(const GraphHost &g) {
+    const uint64_t n = g.n_nodes, E = g.col.size();
+    return 32 * E + 64 * n + (1ull << 20);
+}
+
+uint64_t LouvainServer::submit(const GraphHost *g) {
+    Impl &d = *d_;
+    if (!g || g->n_nodes == 0) throw Error(LV_ERR_ARG, "empty graph");
+    if (2.0 * g->total_weight >= 2147483648.0) throw Error(LV_ERR_RANGE, "total weight 2m >= 2^31");
+    auto j = std::make_unique<Job>();
+    j->id = d.next_id++;
+    j->g = g;
+    j->n0 = g->n_nodes;
+    j->E0 = g->col.size();
+    j->bytes = footprint(*g);
+    if (j->bytes > d.budget) throw Error(LV_ERR_RANGE, "job larger than the HBM budget");
+    j->t_submit = Clock::now();
+    const uint64_t id = j->id;
+    d.queue.push_back(std::move(j));
+    return id;
+}
+
+uint32_t LouvainServer::active() const { return (uint32_t)d_->active.size(); }
+uint32_t LouvainServer::queued() const { return (uint32_t)d_->queue.size(); }
+
+std::vector<uint64_t> LouvainServer::take_finished(size_t max) {
+    std::vector<uint64_t> &v = d_->newly_done;
+    const size_t k = std::min(max, v.size());
+    std::vector<uint64_t> out(v.begin(), v.begin() + k);
+    v.erase(v.begin(), v.begin() + k);
+    return out;
+}
+
+bool LouvainServer::result(uint64_t id, lv_job_result *r, uint32_t *best, size_t n, bool release) {
+    Impl &d = *d_;
+    auto it = d.done.find(id);
+    if (it == d.done.end()) return false;
+    Job &j = *it->second;
+    if (r) {
+        r->status = j.status;
+        r->modularity = j.q;
+        r->n_of_best_communities = j.ncomm;
+        r->n = j.n0;
+        r->levels = j.levels;
+        r->sweeps = j.sweeps;
+        r->seconds_queued = secs(j.t_submit, j.t_admit);
+        r->seconds_total = secs(j.t_submit, j.t_done);
+    }
+    if (best) {
+        if (n < j.n0) throw Error(LV_ERR_ARG, "best buffer too small");
+        if (j.status == LV_OK) std::memcpy(best, j.best_host.data(), j.n0 * sizeof(uint32_t));
+    }
+    if (release) d.done.erase(it);
+    return true;
+}
+
+uint32_t LouvainServer::step() {
+    Impl &d = *d_;
+    const size_t done_before = d.newly_done.size();
+    // admission by footprint (the first job of an idle server always fits:
+    // submit rejects jobs larger than the budget)
+    while (!d.queue.empty()) {
+        if (!d.active.empty() && d.used + d.queue.front()->bytes > d.budget) break;
+        auto j = std::move(d.queue.front());
+        d.queue.pop_front();
+        d.admit(std::move(j));
+    }
+    if (d.active.empty()) return (uint32_t)(d.newly_done.size() - done_before);
+    d.assemble();
+    Community &C = d.C;
+    BatchState &B = *C.batch;
+    SweepState &st = *C.state;
+    hipStream_t s = d.stream();
+    const lv_params &p = d.p;
+    const uint32_t A = (uint32_t)d.active.size();
+    unsigned long long *dsig = B.host_acc, *sumsq = B.host_acc + A, *cnt = B.host_acc + 2 * A;
+    // the sweep loop of every job at its own level (phaseOptimization.cuh:158-181)
+    uint32_t running = A;
+    for (uint32_t i = 0; i < A; ++i) {
+        d.active[i]->q_start = d.active[i]->q;
+        B.host_live[i] = 1;
+    }
+    st.build_static_lists(s);
+    int64_t exec = 0;
+    while (running > 0) {
+        const int kind = OptimizationPhase::sweep_kind(p.mode, exec);
+        LV_HIP(hipMemcpyAsync(B.live_dev.data(), B.host_live, A, hipMemcpyHostToDevice, s));
+        fill_u64(B.acc.data(), 0ull, 2 * (size_t)A, s);
+        launch_sweep(C, p, kind, true, C.graph.n_nodes);
+        LV_HIP(hipMemcpyAsync(B.host_acc, B.acc.data(), 2 * (size_t)A * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, s));
+        LV_HIP(hipStreamSynchronize(s));
+        ++exec;
+        for (uint32_t i = 0; i < A; ++i) {
+            if (!B.host_live[i]) continue;
+            Job &j = *d.active[i];
+            j.sig += (int64_t)dsig[i];
+            j.sq = sumsq[i];
+            const double old = j.q;
+            j.q = q_of(j.sig, j.sq, j.tw);
+            ++j.sweeps;
+            if (!(exec <= p.max_sweeps && j.q - old > p.sweep_threshold)) {
+                B.host_live[i] = 0;
+                --running;
+            }
+        }
+        st.cur ^= 1;
+        std::memcpy(st.count, st.host_stats + kStatWordsOffset, sizeof(st.count));
+    }
+    // level test per job (modularityAlgorithms.h:52-53)
+    bool any = false;
+    for (auto &jp : d.active) {
+        Job &j = *jp;
+        ++j.levels;
+        j.cont = j.q - j.q_start > p.level_threshold;
+        if (p.max_levels > 0 && j.levels >= p.max_levels) j.cont = false;
+        any = any || j.cont;
+    }
+    // renumbering (phaseAggregation.cuh:27-46) and every job's best_communities
+    uint32_t n_new = 0;
+    uint32_t *map = renumber(C, n_new);
+    const uint32_t n = C.graph.n_nodes;
+    fill_u64(B.acc.data() + 2 * (size_t)A, 0ull, A, s);
+    batch_comm_counts(C.communities_weight.data(), B.gid, n, B.acc.data() + 2 * (size_t)A, s);
+    LV_HIP(hipMemcpyAsync(cnt, B.acc.data() + 2 * (size_t)A, A * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, s));
+    LV_HIP(hipStreamSynchronize(s));
+    std::vector<JobBest> jb(A);
+    std::vector<uint32_t> base(A + 1, 0);
+    uint32_t max_n0 = 0;
+    for (uint32_t i = 0; i < A; ++i) {
+        Job &j = *d.active[i];
+        j.ncomm = (uint32_t)cnt[i];
+        base[i + 1] = base[i] + j.ncomm;
+        jb[i] = JobBest{j.best.data(), j.n0, j.voff, base[i]};
+        max_n0 = std::max(max_n0, j.n0);
+    }
+    d.jobs.ensure(A);
+    LV_HIP(hipMemcpyAsync(d.jobs.data(), jb.data(), A * sizeof(JobBest), hipMemcpyHostToDevice, s));
+    update_best_jobs(d.jobs.data(), A, max_n0, C.communities.data(), map, s);
+    // retire the jobs whose level loop ended
+    for (auto &jp : d.active)
+        if (!jp->cont) {
+            jp->best_host.resize(jp->n0);
+            LV_HIP(hipMemcpyAsync(jp->best_host.data(), jp->best.data(), jp->n0 * sizeof(uint32_t),
+                                  hipMemcpyDeviceToHost, s));
+        }
+    LV_HIP(hipStreamSynchronize(s));
+    std::vector<std::unique_ptr<Job>> next;
+    for (uint32_t i = 0; i < A; ++i) {
+        auto &jp = d.active[i];
+        if (jp->cont) {
+            jp->cvoff = base[i];
+            jp->cn = jp->ncomm;
+            next.push_back(std::move(jp));
+        } else {
+            d.retire(std::move(jp));
+        }
+    }
+    d.active.swap(next);
+    // contraction of the level (jobs retired above are contracted too and
+    // dropped by the next assembly); entry ranges of the continuing jobs
+    if (any) {
+        AggregationPhase::contract(C, map, n_new);
+        const uint32_t K = (uint32_t)d.active.size();
+        std::vector<uint32_t> idx(2 * K);
+        for (uint32_t i = 0; i < K; ++i) {
+            idx[2 * i] = d.active[i]->cvoff;
+            idx[2 * i + 1] = d.active[i]->cvoff + d.active[i]->cn;
+        }
+        d.idx_dev.ensure(2 * K);
+        d.eoff_dev.ensure(2 * K);
+        LV_HIP(hipMemcpyAsync(d.idx_dev.data(), idx.data(), 2 * K * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, s));
+        gather_u64(C.graph.neighboorhood_sum.data(), d.idx_dev.data(), 2 * K, d.eoff_dev.data(), s);
+        std::vector<uint64_t> eo(2 * K);
+        LV_HIP(hipMemcpyAsync(eo.data(), d.eoff_dev.data(), 2 * K * sizeof(uint64_t),
+                              hipMemcpyDeviceToHost, s));
+        LV_HIP(hipStreamSynchronize(s));
+        for (uint32_t i = 0; i < K; ++i) {
+            d.active[i]->ce0 = eo[2 * i];
+            d.active[i]->cE = eo[2 * i + 1] - eo[2 * i];
+        }
+    }
+    return (uint32_t)(d.newly_done.size() - done_before);
+}

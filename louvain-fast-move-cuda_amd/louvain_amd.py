@@ -77,6 +77,13 @@ class BatchResult(C.Structure):
                 ("seconds", C.c_double), ("traversed_entries", C.c_uint64)]
 
 
+class JobResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("modularity", C.c_double),
+                ("n_of_best_communities", C.c_uint32), ("n", C.c_uint32),
+                ("levels", C.c_int32), ("sweeps", C.c_int32),
+                ("seconds_queued", C.c_double), ("seconds_total", C.c_double)]
+
+
 # name -> (restype, argtypes); every symbol include/lv_api.h declares
 _vp = C.c_void_p
 _u32p = C.POINTER(C.c_uint32)
@@ -120,6 +127,15 @@ SIGNATURES = {
     "lv_batch_info": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
                                 C.POINTER(C.c_uint64)]),
     "lv_batch_louvain": (C.c_int, [_vp, C.POINTER(Params), C.POINTER(BatchResult)]),
+    "lv_server_create": (C.c_int, [C.POINTER(Params), C.c_uint64, _vp, C.POINTER(_vp)]),
+    "lv_server_destroy": (None, [_vp]),
+    "lv_server_submit": (C.c_int, [_vp, _vp, C.POINTER(C.c_uint64)]),
+    "lv_server_step": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_uint32)]),
+    "lv_server_take_finished": (C.c_int, [_vp, _vp, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "lv_server_result": (C.c_int, [_vp, C.c_uint64, C.POINTER(JobResult), _vp, C.c_size_t,
+                                   C.c_int]),
+    "lv_server_footprint": (C.c_int, [_vp, C.POINTER(C.c_uint64)]),
     "lv_gen_rmat": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64,
                               _vp, _vp, C.c_uint64, C.POINTER(C.c_uint64),
                               C.POINTER(C.c_uint32)]),
@@ -366,6 +382,63 @@ class Batch:
         return {"modularity": q, "n_comm": nc, "levels": lev, "sweeps": sw, "best": bests,
                 "union_levels": r.union_levels, "union_sweeps": r.union_sweeps,
                 "seconds": r.seconds, "traversed_entries": r.traversed_entries}
+
+
+class Server:
+    """Served Louvain with continuous batching (lv_server_*).  Submitted
+    graphs are referenced until their job is retired."""
+
+    def __init__(self, prm=None, hbm_budget=200 << 30, stream=None):
+        h = _vp()
+        _check(lib.lv_server_create(C.byref(prm or params()), int(hbm_budget), stream, C.byref(h)))
+        self.h = h
+        self._graphs = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.lv_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, graph):
+        jid = C.c_uint64()
+        _check(lib.lv_server_submit(self.h, graph.h, C.byref(jid)))
+        self._graphs[jid.value] = graph
+        return jid.value
+
+    def step(self):
+        """One union level; returns (active, queued, retired)."""
+        a = C.c_uint32(); q = C.c_uint32(); r = C.c_uint32()
+        _check(lib.lv_server_step(self.h, C.byref(a), C.byref(q), C.byref(r)))
+        return a.value, q.value, r.value
+
+    def take_finished(self, cap=4096):
+        ids = np.empty(cap, np.uint64)
+        k = C.c_size_t()
+        _check(lib.lv_server_take_finished(self.h, _p(ids), cap, C.byref(k)))
+        return [int(x) for x in ids[:k.value]]
+
+    def result(self, job_id, best=True, release=True):
+        g = self._graphs.get(job_id)
+        n = g.n if (g is not None and best) else 0
+        out = np.empty(max(n, 1), np.uint32) if best else None
+        r = JobResult()
+        _check(lib.lv_server_result(self.h, job_id, C.byref(r),
+                                    _p(out) if best else None, n, int(release)))
+        if release:
+            self._graphs.pop(job_id, None)
+        return {"status": r.status, "modularity": r.modularity,
+                "n_comm": r.n_of_best_communities, "n": r.n, "levels": r.levels,
+                "sweeps": r.sweeps, "seconds_queued": r.seconds_queued,
+                "seconds_total": r.seconds_total, "best": out[:r.n] if best else None}
+
+
+def footprint(graph):
+    b = C.c_uint64()
+    _check(lib.lv_server_footprint(graph.h, C.byref(b)))
+    return b.value
 
 
 def _run_result(call, n, sweeps_cap, levels_cap, hash_trace, best):
