@@ -162,6 +162,147 @@ def batched_result(args, L):
     }
 
 
+def served_result(args, L):
+    """Poisson arrivals (rate args.rate for args.window s, seed 5) served with
+    continuous batching under an HBM budget; 95% planted-partition graphs
+    (the planted configuration's degree and community-size laws at
+    args.served_n vertices, weights 1..10), 5% R-MAT scale 20.  Requests draw
+    from a pool of distinct seeded graphs; every request is its own job (its
+    own upload, device state and result)."""
+    import numpy as np
+    t = time.perf_counter()
+    pool_p = []
+    for i in range(args.pool):
+        ea, eb, ew, _ = L.gen_planted(n=args.served_n, n_comm=max(2, args.served_n // 500),
+                                      seed=5000 + i)
+        pool_p.append(L.Graph.from_edges(ea, eb, ew))
+    pool_r = []
+    for i in range(max(1, args.pool // 4)):
+        ea, eb, _ = L.gen_rmat(20, 16, 0.57, 0.19, 0.19, seed=500 + i)
+        pool_r.append(L.Graph.from_edges(ea, eb))
+    log(f"served pool: {len(pool_p)} planted (n={args.served_n}, mean E="
+        f"{np.mean([g.E for g in pool_p]):.0f}), {len(pool_r)} R-MAT-20 (mean E="
+        f"{np.mean([g.E for g in pool_r]):.0f}) ({time.perf_counter() - t:.1f}s)")
+    rng = np.random.default_rng(5)
+    arr = np.cumsum(rng.exponential(1.0 / args.rate, int(args.rate * args.window * 1.5) + 10))
+    arr = arr[arr < args.window]
+    is_r = rng.random(len(arr)) < 0.05
+    pick = rng.integers(0, 1 << 30, len(arr))
+    req = [pool_r[pick[i] % len(pool_r)] if is_r[i] else pool_p[pick[i] % len(pool_p)]
+           for i in range(len(arr))]
+    rng_b = np.random.default_rng(6)
+
+    def req_pool(r):
+        return pool_r[r.integers(len(pool_r))] if r.random() < 0.05 else pool_p[r.integers(len(pool_p))]
+    prm = L.params(mode=args.mode, sweep_threshold=args.threshold, level_threshold=args.threshold)
+    budget = int(args.hbm_budget_gb * (1 << 30))
+    # warm the server's buffers and the code paths with a few jobs
+    S = L.Server(prm, hbm_budget=budget)
+    for g in pool_p[:2] + pool_r[:1]:
+        S.submit(g)
+    while S.step()[:2] != (0, 0):
+        pass
+    for jid in S.take_finished():
+        S.result(jid, best=False)
+    # open loop in real time
+    lat, kind_of, q_of = [], [], {}
+    job_req = {}
+    start = time.perf_counter()
+    i, n_done, steps, max_active = 0, 0, 0, 0
+    last_log = start
+    while n_done < len(arr):
+        now = time.perf_counter() - start
+        while i < len(arr) and arr[i] <= now:
+            job_req[S.submit(req[i])] = i
+            i += 1
+        a, q, _ = S.step()
+        steps += 1
+        max_active = max(max_active, a)
+        t_now = time.perf_counter() - start
+        for jid in S.take_finished():
+            r = S.result(jid, best=False)
+            k = job_req.pop(jid)
+            lat.append(t_now - arr[k])
+            kind_of.append(bool(is_r[k]))
+            q_of[k] = r["modularity"]
+            n_done += 1
+        if a == 0 and q == 0 and i < len(arr):
+            time.sleep(max(0.0, arr[i] - (time.perf_counter() - start)))
+        if time.perf_counter() - last_log > 5:
+            last_log = time.perf_counter()
+            log(f"served: t={t_now:.1f}s submitted {i} done {n_done} active {a} queued {q}")
+    elapsed = max(time.perf_counter() - start, 1e-9)
+    lat = np.array(lat) if lat else np.zeros(1)
+    kind_of = np.array(kind_of) if kind_of else np.zeros(1, bool)
+    # saturation: a burst of the same request mix, server versus a per-graph loop
+    burst = [req_pool(rng_b) for _ in range(args.burst)] if len(req) < args.burst else req[:args.burst]
+    st0 = S.stats()
+    t0 = time.perf_counter()
+    ids = [S.submit(g) for g in burst]
+    while S.step()[:2] != (0, 0):
+        pass
+    srv_s = time.perf_counter() - t0
+    st1 = S.stats()
+    phases = {k: st1[k] - st0[k] for k in ("seconds_assemble", "seconds_sweeps",
+                                           "seconds_aggregate", "uploaded_bytes", "steps")}
+    q_srv = {}
+    for jid in S.take_finished():
+        q_srv[jid] = S.result(jid, best=False)["modularity"]
+    q_srv = [q_srv[j] for j in ids]
+    t0 = time.perf_counter()
+    q_loop = []
+    for g in burst:
+        r = g.louvain(prm, sweeps_cap=1, levels_cap=1, best=False)
+        q_loop.append(r["modularity"])
+    loop_s = time.perf_counter() - t0
+    dq = max(abs(x - y) for x, y in zip(q_srv, q_loop))
+    S.close()
+    return {
+        "config": {"workload": f"served Louvain: Poisson arrivals at {args.rate:g} req/s for "
+                               f"{args.window:g} s (seed 5); 95% planted-partition graphs "
+                               f"(n={args.served_n}, degree exponent 2.5, mean 20, max 5000, "
+                               f"community sizes exponent 1.5 in [50, 5000], mixing 0.3, weights "
+                               f"1..10), 5% R-MAT scale 20; HBM budget {args.hbm_budget_gb:g} GB",
+                   "requests": int(len(arr)), "pool_planted": len(pool_p),
+                   "pool_rmat20": len(pool_r)},
+        "graphs_per_s": len(arr) / elapsed,
+        "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "p50_latency_planted_s": float(np.percentile(lat[~kind_of], 50)),
+        "p99_latency_rmat20_s": float(np.percentile(lat[kind_of], 99)) if kind_of.any() else None,
+        "elapsed_s": elapsed,
+        "union_levels": steps,
+        "max_active_jobs": max_active,
+        "saturation": {"requests": len(burst), "server_graphs_per_s": len(burst) / srv_s,
+                       "per_graph_loop_graphs_per_s": len(burst) / loop_s,
+                       "speedup": loop_s / srv_s, "max_abs_dq": dq, "server_phases": phases,
+                       "note": "the same burst through the server and through lv_louvain per "
+                               "graph (upload + run each), both timed end to end"},
+    }
+
+
+def bench_served(args, L):
+    b = served_result(args, L)
+    out = {
+        "metric": METRIC,
+        "value": b["p99_latency_s"],
+        "unit": "s (p99 latency)",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": b["elapsed_s"] * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic",
+        "config": b.pop("config"),
+    }
+    out.update(b)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def bench_batched(args, L):
     """The batched configuration as its own bench line (value = graphs/s)."""
     b = batched_result(args, L)
@@ -199,7 +340,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe-bytes", type=int, default=1 << 30,
                     help="size of the plain copy kernel run as the bandwidth reference")
-    ap.add_argument("--workload", default="rmat", choices=["rmat", "batched"])
+    ap.add_argument("--workload", default="rmat", choices=["rmat", "batched", "served"])
+    ap.add_argument("--rate", type=float, default=200.0, help="served: arrivals per second")
+    ap.add_argument("--window", type=float, default=60.0, help="served: arrival window (s)")
+    ap.add_argument("--served-n", type=int, default=1_000_000,
+                    help="served: vertices of the planted-partition requests")
+    ap.add_argument("--pool", type=int, default=16, help="served: distinct planted graphs")
+    ap.add_argument("--burst", type=int, default=200, help="served: saturation burst size")
+    ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-batched", action="store_true",
                     help="rmat: leave out the batched configuration's sub-object")
     ap.add_argument("--graphs", type=int, default=4096, help="batched: number of graphs")
@@ -216,6 +364,8 @@ def main():
         raise SystemExit("bench.py: no GPU visible")
     if args.workload == "batched":
         return bench_batched(args, L)
+    if args.workload == "served":
+        return bench_served(args, L)
     copy_gbs = L.bandwidth_probe(args.probe_bytes, 5) if args.probe_bytes > 0 else None
     log(f"plain copy kernel: {copy_gbs} GB/s")
     t = time.perf_counter()

@@ -251,6 +251,18 @@ lv_status lv_server_result(lv_server *s, uint64_t job_id, lv_job_result *r, uint
 /* Estimated device bytes of a job while it is active. */
 lv_status lv_server_footprint(const lv_graph *g, uint64_t *bytes);
 
+/* Cumulative counters of a server (host wall time per phase of its steps). */
+typedef struct lv_server_stats {
+    uint64_t steps;
+    uint64_t admitted;
+    uint64_t retired;
+    uint64_t uploaded_bytes;         /* host-to-device bytes of admitted graphs */
+    double seconds_assemble;         /* union assembly, uploads included */
+    double seconds_sweeps;
+    double seconds_aggregate;        /* renumbering, retirement, contraction */
+} lv_server_stats;
+lv_status lv_server_stats_get(const lv_server *s, lv_server_stats *out);
+
 /* Seeded synthetic graph generators (host, multithreaded; counter-based PRNG,
  * so the output is independent of the thread count).  Edges are written as
  * (a[i], b[i], w[i]) with dense ids; *m receives the count.  Pass NULL arrays

@@ -45,6 +45,7 @@ struct LouvainServer {
     bool result(uint64_t id, lv_job_result *r, uint32_t *best, size_t n, bool release);
     // Estimated device bytes a job holds while active.
     static uint64_t footprint(const GraphHost &g);
+    lv_server_stats stats() const;
 
   private:
     struct Impl;

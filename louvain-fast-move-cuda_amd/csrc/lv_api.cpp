@@ -492,4 +492,10 @@ lv_status lv_server_footprint(const lv_graph *g, uint64_t *bytes) {
     return LV_OK;
 }
 
+lv_status lv_server_stats_get(const lv_server *s, lv_server_stats *out) {
+    if (!s || !out) return LV_ERR_ARG;
+    *out = s->s.stats();
+    return LV_OK;
+}
+
 }  // extern "C"

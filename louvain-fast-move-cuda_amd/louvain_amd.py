@@ -84,6 +84,12 @@ class JobResult(C.Structure):
                 ("seconds_queued", C.c_double), ("seconds_total", C.c_double)]
 
 
+class ServerStats(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("admitted", C.c_uint64), ("retired", C.c_uint64),
+                ("uploaded_bytes", C.c_uint64), ("seconds_assemble", C.c_double),
+                ("seconds_sweeps", C.c_double), ("seconds_aggregate", C.c_double)]
+
+
 # name -> (restype, argtypes); every symbol include/lv_api.h declares
 _vp = C.c_void_p
 _u32p = C.POINTER(C.c_uint32)
@@ -136,6 +142,7 @@ SIGNATURES = {
     "lv_server_result": (C.c_int, [_vp, C.c_uint64, C.POINTER(JobResult), _vp, C.c_size_t,
                                    C.c_int]),
     "lv_server_footprint": (C.c_int, [_vp, C.POINTER(C.c_uint64)]),
+    "lv_server_stats_get": (C.c_int, [_vp, C.POINTER(ServerStats)]),
     "lv_gen_rmat": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64,
                               _vp, _vp, C.c_uint64, C.POINTER(C.c_uint64),
                               C.POINTER(C.c_uint32)]),
@@ -419,6 +426,11 @@ class Server:
         k = C.c_size_t()
         _check(lib.lv_server_take_finished(self.h, _p(ids), cap, C.byref(k)))
         return [int(x) for x in ids[:k.value]]
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib.lv_server_stats_get(self.h, C.byref(st)))
+        return _rec(st)
 
     def result(self, job_id, best=True, release=True):
         g = self._graphs.get(job_id)

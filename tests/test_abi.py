@@ -107,8 +107,8 @@ def test_community_without_device_reports_no_device(fixture_path):
 def test_server_footprint_and_argument_errors(fixture_path):
     import ctypes as C
     g = L.Graph.load_edge(fixture_path)
-    # 32 bytes per entry + 64 per vertex + 1 MiB
-    assert L.footprint(g) == 32 * 39992 + 64 * 10000 + (1 << 20)
+    # 48 bytes per entry + 96 per vertex + 1 MiB
+    assert L.footprint(g) == 48 * 39992 + 96 * 10000 + (1 << 20)
     h = C.c_void_p()
     assert L.lib.lv_server_create(None, 0, None, C.byref(h)) == 1      # LV_ERR_ARG
     assert L.lib.lv_server_step(None, None, None, None) == 1
