@@ -13,7 +13,10 @@ events on the library stream; the CPU baseline is the single-threaded oracle
 on the first sweeps of level 0 of the same graph.  The `batched` sub-object
 is the batched configuration (4,096 Barabasi-Albert graphs in one
 disjoint-union launch sequence, graphs/s, against a loop of per-graph runs);
-`--workload batched` prints it as its own line with value = graphs/s.
+`--workload batched` prints it as its own line with value = graphs/s.  The
+`served` sub-object is the served configuration (Poisson arrivals at 200
+req/s, continuous batching, p50/p99 latency, graphs/s) over a 20 s arrival
+window; `--workload served` prints it as its own line over 60 s.
 """
 import argparse
 import json
@@ -346,10 +349,14 @@ def main():
     ap.add_argument("--served-n", type=int, default=1_000_000,
                     help="served: vertices of the planted-partition requests")
     ap.add_argument("--pool", type=int, default=16, help="served: distinct planted graphs")
-    ap.add_argument("--burst", type=int, default=200, help="served: saturation burst size")
+    ap.add_argument("--burst", type=int, default=400, help="served: saturation burst size")
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-batched", action="store_true",
                     help="rmat: leave out the batched configuration's sub-object")
+    ap.add_argument("--no-served", action="store_true",
+                    help="rmat: leave out the served configuration's sub-object")
+    ap.add_argument("--served-window", type=float, default=20.0,
+                    help="rmat: arrival window of the served sub-object (s)")
     ap.add_argument("--graphs", type=int, default=4096, help="batched: number of graphs")
     ap.add_argument("--loop-sample", type=int, default=1,
                     help="batched: time the per-graph loop on every k-th graph only")
@@ -462,6 +469,12 @@ def main():
     if not args.no_batched:
         log("batched configuration ...")
         out["batched"] = batched_result(args, L)
+    if not args.no_served:
+        log("served configuration ...")
+        sv = argparse.Namespace(**vars(args))
+        sv.window = args.served_window
+        sv.served_n = 100_000
+        out["served"] = served_result(sv, L)
     print(json.dumps(out), flush=True)
     return 0
 
