@@ -100,7 +100,8 @@ def batched_result(args, L):
     n_tot = sum(g.n for g in graphs)
     e_tot = sum(g.E for g in graphs)
     log(f"{len(graphs)} BA graphs: n={n_tot} E={e_tot} ({time.perf_counter() - t:.1f}s)")
-    prm = L.params(mode=args.mode, sweep_threshold=args.threshold, level_threshold=args.threshold)
+    prm = L.params(mode=args.mode, sweep_threshold=args.threshold, level_threshold=args.threshold,
+                   autotune=args.autotune)
     t = time.perf_counter()
     B = L.Batch(graphs)
     log(f"batch union + upload ({time.perf_counter() - t:.1f}s)")
@@ -344,6 +345,8 @@ def main():
     ap.add_argument("--probe-bytes", type=int, default=1 << 30,
                     help="size of the plain copy kernel run as the bandwidth reference")
     ap.add_argument("--workload", default="rmat", choices=["rmat", "batched", "served"])
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: tune the best-move tiles during the warm-up run")
     ap.add_argument("--rate", type=float, default=200.0, help="served: arrivals per second")
     ap.add_argument("--window", type=float, default=60.0, help="served: arrival window (s)")
     ap.add_argument("--served-n", type=int, default=1_000_000,
@@ -387,7 +390,7 @@ def main():
     c = L.Community(g)
     log(f"upload + level setup ({time.perf_counter() - t:.2f}s)")
     prm = L.params(mode=args.mode, sweep_threshold=args.threshold,
-                   level_threshold=args.threshold, profile=1)
+                   level_threshold=args.threshold, profile=1, autotune=args.autotune)
     for i in range(args.warmup):
         c.reset()
         r = c.run(prm, best=False, n0=info["n"])

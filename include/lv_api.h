@@ -59,6 +59,9 @@ typedef struct lv_params {
     int64_t max_sweeps;       /* EARLY_STOP_LIMIT (constants.h:16) */
     int32_t max_levels;       /* <= 0: no limit */
     int32_t profile;          /* 1: time every best-move launch with HIP events */
+    int32_t autotune;         /* 1: on a community's first fast and first hash sweep, time
+                                 each degree bucket's tile candidates and keep the fastest
+                                 (decisions, hence results, do not depend on the tile) */
 } lv_params;
 
 typedef struct lv_sweep_rec {
@@ -99,12 +102,13 @@ typedef struct lv_kernel_stats {
     uint64_t move_entries;      /* entries traversed by those launches */
     double sweep_ms;            /* whole sweeps (move + commit + modularity), event timed */
     uint64_t sweeps;
-    /* per best-move kernel variant v = (bucket*2 + fast)*2 + weighted, bucket 0-7
-     * (one kernel launch each; lv_kernel_variant_name gives the kernel name) */
-    uint64_t variant_launches[32];
-    double variant_ms[32];
-    uint64_t variant_bytes[32];
-    uint64_t variant_entries[32];
+    /* per best-move kernel variant v = ((bucket*2 + fast)*2 + weighted)*3 + tile,
+     * bucket 0-7, tile = index of the bucket's tile candidate (one kernel launch
+     * each; lv_kernel_variant_name gives the kernel name) */
+    uint64_t variant_launches[96];
+    double variant_ms[96];
+    uint64_t variant_bytes[96];
+    uint64_t variant_entries[96];
 } lv_kernel_stats;
 
 typedef struct lv_graph lv_graph;          /* GraphHost (graphHost.cuh:7-18): host CSR */
@@ -168,6 +172,14 @@ lv_status lv_best_moves(lv_community *c, int kind, const lv_params *p, const uin
 /* Exact modularity parts of the current partition, from scratch on the device. */
 lv_status lv_community_modularity_parts(lv_community *c, int64_t *sigma_in, uint64_t *sum_tot_sq,
                                         double *q);
+/* Tile choice of the best-move kernels: sel[2*b + f] = candidate index of
+ * degree bucket b (0-7) for hash/sort (f = 0) and fast (f = 1) sweeps.  get
+ * returns the tuned (or default) choice; set pins one, e.g. a table tuned
+ * earlier, and marks the community tuned.  The candidates per bucket are
+ * lanes per vertex {4,2,8}, {8,4,16}, {16,8,32}, {32,16,64}, {64,32},
+ * {64,256}, {256}, {256}. */
+lv_status lv_community_get_tiles(const lv_community *c, uint8_t sel[16]);
+lv_status lv_community_set_tiles(lv_community *c, const uint8_t sel[16]);
 lv_status lv_kernel_stats_get(const lv_community *c, lv_kernel_stats *s);
 lv_status lv_last_run_kernel_stats(lv_kernel_stats *s);
 /* Device copy bandwidth (read + write bytes per second / 1e9) of `reps`

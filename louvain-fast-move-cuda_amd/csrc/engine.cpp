@@ -410,11 +410,8 @@ namespace lv {
 // (phaseOptimizationSort.cu:46-60, :81-86, :186-204, :235-240).
 uint32_t chunk_vertex_end(Community &c, const lv_params &p, int kind, const uint8_t *active_host);
 
-// One sweep from the current lists; returns stats.  Kernels only; the caller
-// syncs.  Used by the sweep loop and by lv_best_moves.
-void launch_sweep(Community &c, const lv_params &p, int kind, bool apply, uint32_t v_end) {
+static SweepArgs sweep_args(Community &c, const lv_params &p, int kind, uint32_t v_end) {
     SweepState &st = *c.state;
-    hipStream_t s = c.stream();
     SweepArgs a;
     a.g = level_view(c.graph);
     a.comm = c.communities.data();
@@ -431,7 +428,50 @@ void launch_sweep(Community &c, const lv_params &p, int kind, bool apply, uint32
     a.kind = kind;
     a.gain = p.gain;
     a.highest = (p.tie == LV_TIE_REFERENCE && kind == LV_SWEEP_HASH);
+    a.tiles = st.tiles;
     if (c.batch) a.bt = c.batch->view();
+    return a;
+}
+
+constexpr uint32_t kTuneMinVertices = 4096;   // smaller buckets keep the default tile
+
+void autotune_tiles(Community &c, const lv_params &p, int kind, uint32_t v_end) {
+    SweepState &st = *c.state;
+    if (kind == LV_SWEEP_SORT) return;
+    const int f = kind == LV_SWEEP_FAST ? 1 : 0;
+    if (st.tuned[f]) return;
+    st.tuned[f] = true;
+    hipStream_t s = c.stream();
+    const SweepArgs a = sweep_args(c, p, kind, v_end);   // the sweep's own vertex bound
+    uint32_t *const *L = st.lists(st.cur);
+    // best-move launches only write the decisions, the same ones each time
+    for (int b = 0; b < kNumBuckets - 1; ++b) {
+        const uint32_t cnt = st.count[b];
+        if (cnt < kTuneMinVertices) continue;
+        float best_ms = 0;
+        int best = 0;
+        for (int k = 0; k < kTileChoices && kTileG[b][k] != 0; ++k) {
+            move_bucket(a, b, k, L[b], cnt, s);   // warm-up
+            LV_HIP(hipEventRecord(st.ev[0], s));
+            move_bucket(a, b, k, L[b], cnt, s);
+            move_bucket(a, b, k, L[b], cnt, s);
+            LV_HIP(hipEventRecord(st.ev[1], s));
+            LV_HIP(hipEventSynchronize(st.ev[1]));
+            float ms = 0;
+            LV_HIP(hipEventElapsedTime(&ms, st.ev[0], st.ev[1]));
+            if (k == 0 || ms < best_ms) { best_ms = ms; best = k; }
+        }
+        st.tiles.sel[b][f] = (uint8_t)best;
+    }
+}
+
+// One sweep from the current lists; returns stats.  Kernels only; the caller
+// syncs.  Used by the sweep loop and by lv_best_moves.
+void launch_sweep(Community &c, const lv_params &p, int kind, bool apply, uint32_t v_end) {
+    SweepState &st = *c.state;
+    hipStream_t s = c.stream();
+    if (p.autotune) autotune_tiles(c, p, kind, v_end);
+    SweepArgs a = sweep_args(c, p, kind, v_end);
     fill_u64(st.stats.data(), 0ull, st.stats.size(), s);
     uint32_t *const *cur = st.lists(st.cur);
     if (p.profile) LV_HIP(hipEventRecord(st.ev[0], s));
@@ -505,7 +545,7 @@ int OptimizationPhase::run(Community &c, const lv_params &p, lv_sweep_rec *rec, 
                 if (b == kHubBucket || kind == LV_SWEEP_SORT) continue;
                 float ms_b = 0;
                 LV_HIP(hipEventElapsedTime(&ms_b, st.bev[2 * b], st.bev[2 * b + 1]));
-                const int vi = move_variant(b, fast, wtd);
+                const int vi = move_variant(b, fast, wtd, st.tiles.sel[b][fast ? 1 : 0]);
                 st.acc.var_launches[vi] += 1;
                 st.acc.var_ms[vi] += ms_b;
                 st.acc.var_bytes[vi] += bb;

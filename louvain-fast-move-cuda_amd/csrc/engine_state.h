@@ -15,10 +15,10 @@ struct Community;
 
 namespace lv {
 
-// best-move kernel variants timed one by one: (bucket 0-7, fast, weighted)
-constexpr int kMoveVariants = 32;
-inline int move_variant(int bucket, bool fast, bool weighted) {
-    return (bucket * 2 + (fast ? 1 : 0)) * 2 + (weighted ? 1 : 0);
+// best-move kernel variants timed one by one: (bucket 0-7, fast, weighted, tile)
+constexpr int kMoveVariants = 96;
+inline int move_variant(int bucket, bool fast, bool weighted, int tile) {
+    return ((bucket * 2 + (fast ? 1 : 0)) * 2 + (weighted ? 1 : 0)) * kTileChoices + tile;
 }
 
 struct KernelStatsAcc {
@@ -105,6 +105,9 @@ struct SweepState {
     // level l, held in the other one or in the resident level-0 graph, is read)
     GraphDevice level_buf[2];
     int next_level_buf = 0;
+    // tiles of the best-move kernels (tuned once per community when asked)
+    TileChoice tiles;
+    bool tuned[2] = {false, false};   // [fast sweep]
     // profiling
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     hipEvent_t bev[2 * kNumBuckets] = {};
@@ -173,5 +176,8 @@ uint64_t sum_slots(const unsigned long long *h, int stat);            // global 
 uint64_t bucket_slots(const unsigned long long *h, int bucket, int stat);
 uint64_t all_bucket_slots(const unsigned long long *h, int stat);
 void launch_sweep(::Community &c, const lv_params &p, int kind, bool apply, uint32_t v_end);
+// Times each LDS bucket's tile candidates on the current lists (once per
+// community and sweep class) and keeps the fastest in state->tiles.
+void autotune_tiles(::Community &c, const lv_params &p, int kind, uint32_t v_end);
 
 }  // namespace lv

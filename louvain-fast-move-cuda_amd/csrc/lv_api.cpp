@@ -78,6 +78,7 @@ void lv_default_params(lv_params *p) {
     p->max_sweeps = EARLY_STOP_LIMIT;
     p->max_levels = 0;
     p->profile = 0;
+    p->autotune = 0;
 }
 
 const char *lv_status_string(lv_status s) {
@@ -386,12 +387,33 @@ lv_status lv_bandwidth_probe(uint64_t bytes, int reps, double *gbs) {
 
 lv_status lv_kernel_variant_name(int variant, int gain, char *buf, size_t len) {
     if (variant < 0 || variant >= lv::kMoveVariants || !buf || len == 0) return LV_ERR_ARG;
-    static const int G[8] = {4, 8, 16, 32, 64, 64, 256, 256};
     static const int T[8] = {8, 16, 32, 64, 128, 512, 2048, 8192};
-    const int b = variant / 4, fast = (variant / 2) % 2, wtd = variant % 2;
+    const int sel = variant % lv::kTileChoices, rest = variant / lv::kTileChoices;
+    const int b = rest / 4, fast = (rest / 2) % 2, wtd = rest % 2;
+    const int G = lv::kTileG[b][sel];
+    if (G == 0) return LV_ERR_ARG;
     const char *w = wtd ? "true" : "false";
-    if (fast) std::snprintf(buf, len, "k_move_fast<%d, %d, %s>", G[b], gain, w);
-    else std::snprintf(buf, len, "k_move_table<%d, %d, %d, %s>", G[b], T[b], gain, w);
+    if (fast) std::snprintf(buf, len, "k_move_fast<%d, %d, %s>", G, gain, w);
+    else std::snprintf(buf, len, "k_move_table<%d, %d, %d, %s>", G, T[b], gain, w);
+    return LV_OK;
+}
+
+lv_status lv_community_get_tiles(const lv_community *c, uint8_t sel[16]) {
+    if (!c || !sel) return LV_ERR_ARG;
+    for (int b = 0; b < 8; ++b)
+        for (int f = 0; f < 2; ++f) sel[2 * b + f] = c->c.state->tiles.sel[b][f];
+    return LV_OK;
+}
+
+lv_status lv_community_set_tiles(lv_community *c, const uint8_t sel[16]) {
+    if (!c || !sel) return LV_ERR_ARG;
+    for (int b = 0; b < 8; ++b)
+        for (int f = 0; f < 2; ++f)
+            if (sel[2 * b + f] >= lv::kTileChoices || lv::kTileG[b][sel[2 * b + f]] == 0)
+                return LV_ERR_ARG;
+    for (int b = 0; b < 8; ++b)
+        for (int f = 0; f < 2; ++f) c->c.state->tiles.sel[b][f] = sel[2 * b + f];
+    c->c.state->tuned[0] = c->c.state->tuned[1] = true;
     return LV_OK;
 }
 

@@ -27,6 +27,18 @@ constexpr unsigned long long kHubEmpty = 0xFFFFFFFF00000000ull;   // empty packe
 // Maximum degree handled by each LDS bucket; larger goes to the hub bucket.
 constexpr uint32_t kBucketMaxDeg[kNumBuckets - 1] = {4, 8, 16, 32, 64, 256, 1024, 4096};
 
+// Tile (lanes per vertex) candidates of each LDS bucket's best-move kernels;
+// the first is the default, 0 ends a list.  The LDS table of a bucket stays
+// 2 x its maximum degree whatever the tile.
+constexpr int kTileChoices = 3;
+constexpr int kTileG[kNumBuckets - 1][kTileChoices] = {
+    {4, 2, 8}, {8, 4, 16}, {16, 8, 32}, {32, 16, 64}, {64, 32, 0}, {64, 256, 0}, {256, 0, 0},
+    {256, 0, 0}};
+// Chosen candidate per (bucket, fast sweep or not).
+struct TileChoice {
+    uint8_t sel[kNumBuckets - 1][2] = {};
+};
+
 enum Stat : int {
     ST_E_ACTIVE = 0,     // non-self entries of examined vertices
     ST_CONSIDERED = 1,   // examined vertices with >= 1 non-self entry
@@ -86,7 +98,13 @@ struct SweepArgs {
     int kind;                    // 0 fast, 1 sort, 2 hash
     int gain;                    // 0 exact, 1 ref_f32
     bool highest;                // ties to the highest community id
+    TileChoice tiles;            // best-move kernel tile per bucket
 };
+
+// One LDS bucket's best-move launch with tile candidate `sel` (the sweep kind
+// and the bucket's list as in move_pass); used by the tile tuner.
+void move_bucket(const SweepArgs &a, int bucket, int sel, const uint32_t *list, uint32_t count,
+                 hipStream_t s);
 
 // ---- primitives ----
 void fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);

@@ -1201,15 +1201,60 @@ void launch_move_fast(const MoveCtx &m, bool weighted, int gain, const uint32_t 
     }
 }
 
-}  // namespace
+// The (G, T) instantiations behind kTileG.
+void launch_table_tile(int b, int G, const MoveCtx &m, bool wtd, int gain, const uint32_t *L,
+                       uint32_t cnt, hipStream_t s) {
+#define LV_T(GG, TT) launch_move_table<GG, TT>(m, wtd, gain, L, cnt, s)
+    switch (b) {
+        case 0: if (G == 2) LV_T(2, 8); else if (G == 8) LV_T(8, 8); else LV_T(4, 8); break;
+        case 1: if (G == 4) LV_T(4, 16); else if (G == 16) LV_T(16, 16); else LV_T(8, 16); break;
+        case 2: if (G == 8) LV_T(8, 32); else if (G == 32) LV_T(32, 32); else LV_T(16, 32); break;
+        case 3: if (G == 16) LV_T(16, 64); else if (G == 64) LV_T(64, 64); else LV_T(32, 64); break;
+        case 4: if (G == 32) LV_T(32, 128); else LV_T(64, 128); break;
+        case 5: if (G == 256) LV_T(256, 512); else LV_T(64, 512); break;
+        case 6: LV_T(256, 2048); break;
+        default: LV_T(256, 8192); break;
+    }
+#undef LV_T
+}
 
-void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
-               const uint32_t *counts, hipStream_t s) {
+void launch_fast_tile(int G, const MoveCtx &m, bool wtd, int gain, const uint32_t *L, uint32_t cnt,
+                      hipStream_t s) {
+    switch (G) {
+        case 2: launch_move_fast<2>(m, wtd, gain, L, cnt, s); break;
+        case 4: launch_move_fast<4>(m, wtd, gain, L, cnt, s); break;
+        case 8: launch_move_fast<8>(m, wtd, gain, L, cnt, s); break;
+        case 16: launch_move_fast<16>(m, wtd, gain, L, cnt, s); break;
+        case 32: launch_move_fast<32>(m, wtd, gain, L, cnt, s); break;
+        case 64: launch_move_fast<64>(m, wtd, gain, L, cnt, s); break;
+        default: launch_move_fast<256>(m, wtd, gain, L, cnt, s); break;
+    }
+}
+
+MoveCtx move_ctx(const SweepArgs &a) {
     MoveCtx m;
     m.row = a.g.row; m.col = a.g.col; m.wt = a.g.wt; m.kdeg = a.g.kdeg;
     m.comm = a.comm; m.tot = a.tot; m.target = a.target;
     m.m2 = a.m2; m.tw = a.tw; m.v_end = a.v_end; m.highest = a.highest;
     m.bt = a.bt;
+    return m;
+}
+
+}  // namespace
+
+void move_bucket(const SweepArgs &a, int b, int sel, const uint32_t *list, uint32_t count,
+                 hipStream_t s) {
+    if (!count || b >= kNumBuckets - 1) return;
+    MoveCtx m = move_ctx(a);
+    m.stats = a.bstats + (size_t)b * kNumStats * kStatSlots;
+    const int G = kTileG[b][sel];
+    if (a.kind == 0) launch_fast_tile(G, m, a.g.wt != nullptr, a.gain, list, count, s);
+    else launch_table_tile(b, G, m, a.g.wt != nullptr, a.gain, list, count, s);
+}
+
+void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
+               const uint32_t *counts, hipStream_t s) {
+    MoveCtx m = move_ctx(a);
     const bool wtd = a.g.wt != nullptr;
     const bool fast = a.kind == 0;
     for (int b = 0; b < kNumBuckets; ++b) {
@@ -1220,14 +1265,9 @@ void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
         if (a.bev) (void)hipEventRecord(a.bev[2 * b], s);
         if (fast) {
             switch (b) {
-                case 0: launch_move_fast<4>(m, wtd, a.gain, L, cnt, s); break;
-                case 1: launch_move_fast<8>(m, wtd, a.gain, L, cnt, s); break;
-                case 2: launch_move_fast<16>(m, wtd, a.gain, L, cnt, s); break;
-                case 3: launch_move_fast<32>(m, wtd, a.gain, L, cnt, s); break;
-                case 4: launch_move_fast<64>(m, wtd, a.gain, L, cnt, s); break;
-                case 5: launch_move_fast<64>(m, wtd, a.gain, L, cnt, s); break;
-                case 6: launch_move_fast<256>(m, wtd, a.gain, L, cnt, s); break;
-                case 7: launch_move_fast<256>(m, wtd, a.gain, L, cnt, s); break;
+                case 0: case 1: case 2: case 3: case 4: case 5: case 6: case 7:
+                    launch_fast_tile(kTileG[b][a.tiles.sel[b][1]], m, wtd, a.gain, L, cnt, s);
+                    break;
                 default: {
                     HubCtx h{m, hub};
                     const dim3 grid(cnt * kHubParts), blk(kBlock);
@@ -1246,14 +1286,9 @@ void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
             }
         } else {
             switch (b) {
-                case 0: launch_move_table<4, 8>(m, wtd, a.gain, L, cnt, s); break;
-                case 1: launch_move_table<8, 16>(m, wtd, a.gain, L, cnt, s); break;
-                case 2: launch_move_table<16, 32>(m, wtd, a.gain, L, cnt, s); break;
-                case 3: launch_move_table<32, 64>(m, wtd, a.gain, L, cnt, s); break;
-                case 4: launch_move_table<64, 128>(m, wtd, a.gain, L, cnt, s); break;
-                case 5: launch_move_table<64, 512>(m, wtd, a.gain, L, cnt, s); break;
-                case 6: launch_move_table<256, 2048>(m, wtd, a.gain, L, cnt, s); break;
-                case 7: launch_move_table<256, 8192>(m, wtd, a.gain, L, cnt, s); break;
+                case 0: case 1: case 2: case 3: case 4: case 5: case 6: case 7:
+                    launch_table_tile(b, kTileG[b][a.tiles.sel[b][0]], m, wtd, a.gain, L, cnt, s);
+                    break;
                 default: {
                     HubCtx h{m, hub};
                     const dim3 grid(cnt * kHubParts), blk(kBlock);

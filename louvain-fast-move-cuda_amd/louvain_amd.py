@@ -34,7 +34,8 @@ class Params(C.Structure):
     _fields_ = [("mode", C.c_int32), ("gain", C.c_int32), ("tie", C.c_int32),
                 ("chunk_emulation", C.c_int32), ("chunk_entries", C.c_uint64),
                 ("sweep_threshold", C.c_double), ("level_threshold", C.c_double),
-                ("max_sweeps", C.c_int64), ("max_levels", C.c_int32), ("profile", C.c_int32)]
+                ("max_sweeps", C.c_int64), ("max_levels", C.c_int32), ("profile", C.c_int32),
+                ("autotune", C.c_int32)]
 
 
 class SweepRec(C.Structure):
@@ -64,8 +65,8 @@ class KernelStats(C.Structure):
     _fields_ = [("move_launches", C.c_uint64), ("move_ms", C.c_double),
                 ("move_bytes", C.c_uint64), ("move_entries", C.c_uint64),
                 ("sweep_ms", C.c_double), ("sweeps", C.c_uint64),
-                ("variant_launches", C.c_uint64 * 32), ("variant_ms", C.c_double * 32),
-                ("variant_bytes", C.c_uint64 * 32), ("variant_entries", C.c_uint64 * 32)]
+                ("variant_launches", C.c_uint64 * 96), ("variant_ms", C.c_double * 96),
+                ("variant_bytes", C.c_uint64 * 96), ("variant_entries", C.c_uint64 * 96)]
 
 
 class BatchResult(C.Structure):
@@ -124,6 +125,8 @@ SIGNATURES = {
                                 C.POINTER(C.c_uint64)]),
     "lv_community_modularity_parts": (C.c_int, [_vp, C.POINTER(C.c_int64),
                                                 C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
+    "lv_community_get_tiles": (C.c_int, [_vp, _vp]),
+    "lv_community_set_tiles": (C.c_int, [_vp, _vp]),
     "lv_kernel_stats_get": (C.c_int, [_vp, C.POINTER(KernelStats)]),
     "lv_last_run_kernel_stats": (C.c_int, [C.POINTER(KernelStats)]),
     "lv_kernel_variant_name": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.c_size_t]),
@@ -188,7 +191,10 @@ def device_count():
 def params(**kw):
     p = Params()
     lib.lv_default_params(C.byref(p))
+    names = {f[0] for f in Params._fields_}
     for k, v in kw.items():
+        if k not in names:
+            raise TypeError(f"unknown lv_params field {k!r}")
         if k == "mode" and isinstance(v, str):
             v = MODES[v]
         setattr(p, k, v)
@@ -338,6 +344,16 @@ class Community:
         n0 = n0 if n0 is not None else self.level_info()[0]
         return _run_result(lambda r: lib.lv_community_run(self.h, C.byref(prm or params()), r),
                            n0, sweeps_cap, levels_cap, hash_trace, best)
+
+    def tiles(self):
+        t = np.zeros(16, np.uint8)
+        _check(lib.lv_community_get_tiles(self.h, _p(t)))
+        return t
+
+    def set_tiles(self, sel):
+        t = np.ascontiguousarray(sel, dtype=np.uint8)
+        assert t.shape == (16,)
+        _check(lib.lv_community_set_tiles(self.h, _p(t)))
 
     def kernel_stats(self):
         k = KernelStats()

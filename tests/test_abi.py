@@ -181,3 +181,18 @@ def test_ba_10000_oracle_q_matches_fp64_recompute():
     for v, c in enumerate(r["best"]):
         parts.setdefault(int(c), set()).add(v)
     assert abs(r["modularity"] - nx.community.modularity(G, list(parts.values()))) < 1e-9
+
+
+def test_params_layout_and_tile_variant_names():
+    import ctypes as C
+    assert C.sizeof(L.Params) == 64
+    with pytest.raises(TypeError):
+        L.params(no_such_field=1)
+    assert L.params(autotune=1).autotune == 1
+    # variant = ((bucket*2 + fast)*2 + weighted)*3 + tile
+    assert L.kernel_variant_name(((0 * 2 + 0) * 2 + 0) * 3 + 1) == "k_move_table<2, 8, 0, false>"
+    assert L.kernel_variant_name(((5 * 2 + 0) * 2 + 1) * 3 + 1) == "k_move_table<256, 512, 0, true>"
+    assert L.kernel_variant_name(((3 * 2 + 1) * 2 + 0) * 3 + 2) == "k_move_fast<64, 0, false>"
+    assert L.kernel_variant_name(((6 * 2 + 0) * 2 + 0) * 3 + 0) == "k_move_table<256, 2048, 0, false>"
+    with pytest.raises(L.LouvainError):
+        L.kernel_variant_name(((6 * 2 + 0) * 2 + 0) * 3 + 1)      # bucket 6 has one tile

@@ -69,6 +69,7 @@ CASES = [
      dict(mode=O.SORT, sweep_threshold=1e-6, level_threshold=1e-6)),
     ("adaptive_speed caps", dict(mode=L.ADAPTIVE_SPEED, max_sweeps=3, max_levels=2),
      dict(mode=O.ADAPTIVE_SPEED, max_sweeps=3, max_levels=2)),
+    ("hash autotuned", dict(mode=L.HASH, autotune=1), dict(mode=O.HASH)),
 ]
 
 

@@ -253,3 +253,30 @@ def test_sort_sweep_rmat_snapshot():
             tg, eg = c.best_moves(kind, L.params(), act)
             to, _, eo, _ = og.best_moves(comm, act, kind, O.params())
             assert np.array_equal(tg, to) and eg == eo
+
+
+def test_every_tile_choice_and_the_tuner_give_identical_runs():
+    """The best-move kernels' tile (lanes per vertex) changes speed only: each
+    candidate of every bucket, and the autotuner's pick, give the oracle's run."""
+    rng = np.random.default_rng(41)
+    ea, eb = _random_graph(rng, n=30000, m=150000, hubs=2, hub_deg=6000, selfloops=20)
+    w = rng.integers(1, 6, len(ea)).astype(np.uint32)
+    g = L.Graph.from_edges(ea, eb, w)
+    ref = O.Graph.from_edges(ea, eb, w).louvain(O.params(sweep_threshold=1e-6,
+                                                          level_threshold=1e-6))
+    prm = L.params(sweep_threshold=1e-6, level_threshold=1e-6)
+    c = L.Community(g)
+    for k in range(3):
+        # candidate k where the bucket has one, else the default
+        sel = np.array([k if k < n else 0 for n in (3, 3, 3, 3, 2, 2, 1, 1) for _ in (0, 1)],
+                       np.uint8)
+        c.set_tiles(sel)
+        assert np.array_equal(c.tiles(), sel)
+        c.reset()
+        say(f"tiles {sel.tolist()}")
+        compare_runs(c.run(prm, hash_trace=True), ref, f"tile candidate {k}")
+    t = L.Community(g)
+    t.reset()
+    compare_runs(t.run(L.params(sweep_threshold=1e-6, level_threshold=1e-6, autotune=1),
+                       hash_trace=True), ref, "autotuned")
+    say(f"tuned tiles {t.tiles().tolist()}")
