@@ -51,6 +51,8 @@ struct Community {
     // No graph yet: the owner installs each level's graph and calls update()
     // (the serving loop, whose level graphs are assembled from many jobs).
     explicit Community(hipStream_t stream);
+    // From a level-0 graph already on the device (the GPU .edge ingest).
+    Community(GraphDevice &&g0, hipStream_t stream);
     ~Community();
     Community(Community &&) noexcept;
     Community &operator=(Community &&) noexcept;
@@ -72,6 +74,7 @@ struct Community {
 
   private:
     void init(const GraphHost &g, hipStream_t stream);
+    void init_resident(hipStream_t stream);
     void update_level(bool carry, int64_t sigma_in_carry, uint64_t sum_tot_sq_carry);
     bool own_stream_ = false;
 };

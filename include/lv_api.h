@@ -129,6 +129,30 @@ lv_status lv_graph_from_edges(uint64_t n_edges, const int64_t *a, const int64_t 
 lv_status lv_graph_from_csr(uint32_t n, uint64_t E, const uint64_t *row, const uint32_t *col,
                             const uint32_t *w, double total_weight, lv_graph **out);
 void lv_graph_free(lv_graph *g);
+
+/* ---- .edge ingest on the GPU (graph.cu:17-87 built on the device) ----
+ * The same graph lv_graph_load_edge builds (same reads, dedupe, min_id shift,
+ * dense-id check and errors), parsed and assembled in HBM: the file is read
+ * in pinned chunks overlapped with their upload, tokenised, normalised,
+ * radix sorted and compacted into the CSR on the device. */
+typedef struct lv_ingest_info {
+    int64_t min_id;
+    uint32_t n;
+    uint64_t E;
+    double total_weight;
+    double seconds_read_upload;   /* file read + copy to the device (overlapped) */
+    double seconds_tokenize;
+    double seconds_build;
+} lv_ingest_info;
+/* Straight into a community (no host CSR); info may be NULL. */
+lv_status lv_community_load_edge(const char *path, int weighted, int skip_line, void *stream,
+                                 lv_community **out, lv_ingest_info *info);
+/* Into a host graph handle (the device CSR is copied back); info may be NULL. */
+lv_status lv_graph_load_edge_gpu(const char *path, int weighted, int skip_line, lv_graph **out,
+                                 lv_ingest_info *info);
+/* Writes edges as .edge text, "a b" or "a b w" per line (host, multithreaded). */
+lv_status lv_write_edge(const char *path, const int64_t *a, const int64_t *b, const uint32_t *w,
+                        uint64_t m);
 lv_status lv_graph_info(const lv_graph *g, uint32_t *n, uint64_t *E, double *total_weight,
                         int64_t *min_id);
 lv_status lv_graph_copy_csr(const lv_graph *g, uint64_t *row, uint32_t *col, uint32_t *w);

@@ -233,6 +233,18 @@ Community::Community(hipStream_t stream) {
 }
 Community::Community(const GraphHost &g, hipStream_t stream) { init(g, stream); }
 
+Community::Community(GraphDevice &&g0, hipStream_t stream) {
+    if (stream) {
+        streams[0] = stream;
+    } else {
+        LV_HIP(hipStreamCreateWithFlags(&streams[0], hipStreamNonBlocking));
+        own_stream_ = true;
+    }
+    streams[1] = streams[0];
+    graph0 = std::move(g0);
+    init_resident(streams[0]);
+}
+
 void Community::init(const GraphHost &g, hipStream_t stream) {
     if (stream) {
         streams[0] = stream;
@@ -242,6 +254,11 @@ void Community::init(const GraphHost &g, hipStream_t stream) {
     }
     streams[1] = streams[0];
     graph0 = GraphDevice(g, streams[0]);
+    init_resident(streams[0]);
+}
+
+void Community::init_resident(hipStream_t stream) {
+    (void)stream;
     graph = GraphDevice::view_of(graph0);
     n_original = graph.n_nodes;
     n_of_best_communities = graph.n_nodes;

@@ -7,8 +7,11 @@
 // Barabasi-Albert and planted-partition inputs of the measured workloads.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/device_array.h"
@@ -270,6 +273,52 @@ lv_status lv_gen_planted(uint32_t n, double deg_exp, double mean_deg, uint32_t m
     } catch (const std::bad_alloc &) {
         return LV_ERR_OOM;
     }
+}
+
+lv_status lv_write_edge(const char *path, const int64_t *a, const int64_t *b, const uint32_t *w,
+                        uint64_t m) {
+    if (!path || (m && (!a || !b))) return LV_ERR_ARG;
+    auto put = [](char *p, int64_t v) {   // decimal digits of v at p; returns the end
+        char tmp[24];
+        int k = 0;
+        const bool neg = v < 0;
+        uint64_t u = neg ? (uint64_t)(-(v + 1)) + 1 : (uint64_t)v;
+        do { tmp[k++] = (char)('0' + u % 10); u /= 10; } while (u);
+        if (neg) *p++ = '-';
+        while (k) *p++ = tmp[--k];
+        return p;
+    };
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const uint64_t per = (m + nt - 1) / nt;
+    std::vector<std::string> buf(nt);
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            const uint64_t i0 = std::min<uint64_t>(m, t * per), i1 = std::min<uint64_t>(m, i0 + per);
+            std::string &out = buf[t];
+            out.resize((i1 - i0) * 36);
+            char *p = &out[0];
+            for (uint64_t i = i0; i < i1; ++i) {
+                p = put(p, a[i]);
+                *p++ = ' ';
+                p = put(p, b[i]);
+                if (w) {
+                    *p++ = ' ';
+                    p = put(p, w[i]);
+                }
+                *p++ = '\n';
+            }
+            out.resize(p - &out[0]);
+        });
+    for (auto &x : th) x.join();
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return LV_ERR_IO;
+    for (auto &x : buf)
+        if (std::fwrite(x.data(), 1, x.size(), f) != x.size()) {
+            std::fclose(f);
+            return LV_ERR_IO;
+        }
+    return std::fclose(f) == 0 ? LV_OK : LV_ERR_IO;
 }
 
 }  // extern "C"

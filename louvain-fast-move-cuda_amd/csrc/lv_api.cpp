@@ -14,6 +14,7 @@
 #include "../../include/server.h"
 #include "../../include/modularityAlgorithms.h"
 #include "engine_state.h"
+#include "lv_ingest.h"
 #include "lv_kernels.h"
 
 struct lv_graph {
@@ -30,6 +31,7 @@ struct lv_server {
 struct lv_community {
     Community c;
     explicit lv_community(const GraphHost &g, hipStream_t s) : c(g, s) {}
+    lv_community(GraphDevice &&g, hipStream_t s) : c(std::move(g), s) {}
 };
 
 namespace {
@@ -129,6 +131,89 @@ lv_status lv_graph_from_csr(uint32_t n, uint64_t E, const uint64_t *row, const u
 }
 
 void lv_graph_free(lv_graph *g) { delete g; }
+
+static void fill_ingest_info(lv_ingest_info *info, const GraphDevice &g, int64_t min_id,
+                             const lv::IngestTimes &t) {
+    if (!info) return;
+    info->min_id = min_id;
+    info->n = g.n_nodes;
+    info->E = g.n_entries;
+    info->total_weight = g.total_weight;
+    info->seconds_read_upload = t.read_upload;
+    info->seconds_tokenize = t.tokenize;
+    info->seconds_build = t.build;
+}
+
+lv_status lv_community_load_edge(const char *path, int weighted, int skip_line, void *stream,
+                                 lv_community **out, lv_ingest_info *info) {
+    if (!path || !out) return LV_ERR_ARG;
+    *out = nullptr;
+    if (lv_status st = need_device()) return st;
+    return guarded([&] {
+        hipStream_t s = (hipStream_t)stream;
+        hipStream_t own = nullptr;
+        if (!s) {
+            LV_HIP(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+            s = own;
+        }
+        try {
+            int64_t min_id = 0;
+            lv::IngestTimes t;
+            GraphDevice g = lv::ingest_edge_file(path, weighted != 0, skip_line, s, &min_id, &t);
+            fill_ingest_info(info, g, min_id, t);
+            LV_HIP(hipStreamSynchronize(s));
+            *out = new lv_community(std::move(g), (hipStream_t)stream);
+        } catch (...) {
+            if (own) (void)hipStreamDestroy(own);
+            throw;
+        }
+        if (own) {
+            LV_HIP(hipStreamSynchronize(own));
+            (void)hipStreamDestroy(own);
+        }
+    });
+}
+
+lv_status lv_graph_load_edge_gpu(const char *path, int weighted, int skip_line, lv_graph **out,
+                                 lv_ingest_info *info) {
+    if (!path || !out) return LV_ERR_ARG;
+    *out = nullptr;
+    if (lv_status st = need_device()) return st;
+    return guarded([&] {
+        hipStream_t s = nullptr;
+        LV_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        try {
+            int64_t min_id = 0;
+            lv::IngestTimes t;
+            GraphDevice g = lv::ingest_edge_file(path, weighted != 0, skip_line, s, &min_id, &t);
+            fill_ingest_info(info, g, min_id, t);
+            GraphHost h;
+            h.n_nodes = g.n_nodes;
+            h.n_links = g.n_links;
+            h.total_weight = g.total_weight;
+            h.min_id = min_id;
+            h.unit_weights = g.unit_weights;
+            h.row.resize((size_t)g.n_nodes + 1);
+            h.col.resize(g.n_entries);
+            LV_HIP(hipMemcpyAsync(h.row.data(), g.neighboorhood_sum.data(), h.row.size() * 8,
+                                  hipMemcpyDeviceToHost, s));
+            if (g.n_entries)
+                LV_HIP(hipMemcpyAsync(h.col.data(), g.edge_destination.data(), g.n_entries * 4,
+                                      hipMemcpyDeviceToHost, s));
+            if (!g.unit_weights) {
+                h.weights.resize(g.n_entries);
+                LV_HIP(hipMemcpyAsync(h.weights.data(), g.weights.data(), g.n_entries * 4,
+                                      hipMemcpyDeviceToHost, s));
+            }
+            LV_HIP(hipStreamSynchronize(s));
+            *out = new lv_graph{std::move(h)};
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        (void)hipStreamDestroy(s);
+    });
+}
 
 lv_status lv_graph_info(const lv_graph *g, uint32_t *n, uint64_t *E, double *total_weight,
                         int64_t *min_id) {

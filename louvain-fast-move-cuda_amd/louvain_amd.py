@@ -85,6 +85,12 @@ class JobResult(C.Structure):
                 ("seconds_queued", C.c_double), ("seconds_total", C.c_double)]
 
 
+class IngestInfo(C.Structure):
+    _fields_ = [("min_id", C.c_int64), ("n", C.c_uint32), ("E", C.c_uint64),
+                ("total_weight", C.c_double), ("seconds_read_upload", C.c_double),
+                ("seconds_tokenize", C.c_double), ("seconds_build", C.c_double)]
+
+
 class ServerStats(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("admitted", C.c_uint64), ("retired", C.c_uint64),
                 ("uploaded_bytes", C.c_uint64), ("seconds_assemble", C.c_double),
@@ -136,6 +142,11 @@ SIGNATURES = {
     "lv_batch_info": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
                                 C.POINTER(C.c_uint64)]),
     "lv_batch_louvain": (C.c_int, [_vp, C.POINTER(Params), C.POINTER(BatchResult)]),
+    "lv_community_load_edge": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _vp, C.POINTER(_vp),
+                                         C.POINTER(IngestInfo)]),
+    "lv_graph_load_edge_gpu": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.POINTER(_vp),
+                                         C.POINTER(IngestInfo)]),
+    "lv_write_edge": (C.c_int, [C.c_char_p, _vp, _vp, _vp, C.c_uint64]),
     "lv_server_create": (C.c_int, [C.POINTER(Params), C.c_uint64, _vp, C.POINTER(_vp)]),
     "lv_server_destroy": (None, [_vp]),
     "lv_server_submit": (C.c_int, [_vp, _vp, C.POINTER(C.c_uint64)]),
@@ -219,6 +230,15 @@ class Graph:
         return cls(h)
 
     @classmethod
+    def load_edge_gpu(cls, path, weighted=False, skip=0):
+        """The .edge reader run on the GPU; returns (graph, ingest info)."""
+        h = C.c_void_p()
+        info = IngestInfo()
+        _check(lib.lv_graph_load_edge_gpu(str(path).encode(), int(weighted), int(skip),
+                                          C.byref(h), C.byref(info)))
+        return cls(h), _rec(info)
+
+    @classmethod
     def from_edges(cls, a, b, w=None):
         a = np.ascontiguousarray(a, dtype=np.int64)
         b = np.ascontiguousarray(b, dtype=np.int64)
@@ -269,8 +289,19 @@ class Community:
 
     def __init__(self, graph, stream=None):
         h = C.c_void_p()
-        _check(lib.lv_community_create(graph.h, stream, C.byref(h)))
+        if graph is not None:
+            _check(lib.lv_community_create(graph.h, stream, C.byref(h)))
         self.h = h
+
+    @classmethod
+    def load_edge(cls, path, weighted=False, skip=0, stream=None):
+        """A community straight from an .edge file parsed on the GPU (no host
+        CSR); returns (community, ingest info)."""
+        c = cls(None)
+        info = IngestInfo()
+        _check(lib.lv_community_load_edge(str(path).encode(), int(weighted), int(skip), stream,
+                                          C.byref(c.h), C.byref(info)))
+        return c, _rec(info)
 
     def close(self):
         if getattr(self, "h", None):
@@ -461,6 +492,13 @@ class Server:
                 "n_comm": r.n_of_best_communities, "n": r.n, "levels": r.levels,
                 "sweeps": r.sweeps, "seconds_queued": r.seconds_queued,
                 "seconds_total": r.seconds_total, "best": out[:r.n] if best else None}
+
+
+def write_edge(path, a, b, w=None):
+    a = np.ascontiguousarray(a, dtype=np.int64)
+    b = np.ascontiguousarray(b, dtype=np.int64)
+    w = None if w is None else np.ascontiguousarray(w, dtype=np.uint32)
+    _check(lib.lv_write_edge(str(path).encode(), _p(a), _p(b), _p(w), len(a)))
 
 
 def footprint(graph):

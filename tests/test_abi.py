@@ -196,3 +196,22 @@ def test_params_layout_and_tile_variant_names():
     assert L.kernel_variant_name(((6 * 2 + 0) * 2 + 0) * 3 + 0) == "k_move_table<256, 2048, 0, false>"
     with pytest.raises(L.LouvainError):
         L.kernel_variant_name(((6 * 2 + 0) * 2 + 0) * 3 + 1)      # bucket 6 has one tile
+
+
+def test_write_edge_round_trip(tmp_path):
+    rng = np.random.default_rng(17)
+    n = 500
+    a = rng.integers(-50, n - 50, 4000)
+    b = rng.integers(-50, n - 50, 4000)
+    a = np.concatenate([a, np.arange(-50, n - 50)])
+    b = np.concatenate([b, np.roll(np.arange(-50, n - 50), 1)])
+    w = rng.integers(0, 9, len(a)).astype(np.uint32)
+    for weighted in (False, True):
+        p = tmp_path / f"g{int(weighted)}.edge"
+        L.write_edge(p, a, b, w if weighted else None)
+        g = L.Graph.load_edge(p, weighted=weighted)
+        ref = L.Graph.from_edges(a, b, w if weighted else None)
+        r1, c1, w1 = g.csr()[:3]
+        r2, c2, w2 = ref.csr()[:3]
+        assert np.array_equal(r1, r2) and np.array_equal(c1, c2) and np.array_equal(w1, w2)
+        assert g.info()["min_id"] == -50
