@@ -307,6 +307,50 @@ def bench_served(args, L):
     return 0
 
 
+def ingest_result(args, L, ea, eb, prm):
+    """End to end from an .edge file: the host reader (graph.cu:17-87 on the
+    CPU, then upload) against the GPU ingest, each followed by one Louvain run."""
+    import tempfile
+    fd, path = tempfile.mkstemp(suffix=".edge", dir=args.tmpdir)
+    os.close(fd)
+    try:
+        t = time.perf_counter()
+        L.write_edge(path, ea, eb)
+        size = os.path.getsize(path)
+        log(f"ingest: wrote {size / 1e9:.2f} GB .edge ({time.perf_counter() - t:.1f}s)")
+        t0 = time.perf_counter()
+        gh = L.Graph.load_edge(path)
+        t1 = time.perf_counter()
+        ch = L.Community(gh)
+        t2 = time.perf_counter()
+        rh = ch.run(prm, sweeps_cap=1, levels_cap=1, best=False, n0=gh.n)
+        t3 = time.perf_counter()
+        del ch, gh
+        log(f"ingest: host read + CSR {t1 - t0:.2f}s, upload {t2 - t1:.2f}s, run {t3 - t2:.3f}s")
+        gpu = []
+        for _ in range(2):   # the first also pages the file into the OS cache
+            t4 = time.perf_counter()
+            cd, info = L.Community.load_edge(path)
+            t5 = time.perf_counter()
+            rd = cd.run(prm, sweeps_cap=1, levels_cap=1, best=False, n0=info["n"])
+            t6 = time.perf_counter()
+            gpu.append((t5 - t4, t6 - t5, info))
+            del cd
+        g_ing, g_run, info = gpu[-1]
+        log(f"ingest: GPU {g_ing:.3f}s ({info}), run {g_run:.3f}s")
+        return {"file_bytes": size, "host_read_csr_s": t1 - t0, "host_upload_s": t2 - t1,
+                "gpu_ingest_s": g_ing, "gpu_ingest_phases_s": {
+                    "read_upload": info["seconds_read_upload"],
+                    "tokenize": info["seconds_tokenize"], "build": info["seconds_build"]},
+                "speedup": (t1 - t0 + t2 - t1) / g_ing,
+                "end_to_end_host_path_s": t3 - t0, "end_to_end_gpu_path_s": g_ing + g_run,
+                "same_modularity": rh["modularity"] == rd["modularity"],
+                "note": "file in the OS page cache (second read); host path = GraphHost(path) "
+                        "+ upload; GPU path = lv_community_load_edge"}
+    finally:
+        os.unlink(path)
+
+
 def bench_batched(args, L):
     """The batched configuration as its own bench line (value = graphs/s)."""
     b = batched_result(args, L)
@@ -356,6 +400,9 @@ def main():
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-batched", action="store_true",
                     help="rmat: leave out the batched configuration's sub-object")
+    ap.add_argument("--no-ingest", action="store_true",
+                    help="rmat: leave out the .edge ingest sub-object")
+    ap.add_argument("--tmpdir", default=None, help="where the ingest .edge file is written")
     ap.add_argument("--no-served", action="store_true",
                     help="rmat: leave out the served configuration's sub-object")
     ap.add_argument("--served-window", type=float, default=20.0,
@@ -383,7 +430,6 @@ def main():
     log(f"rmat{args.scale}: {len(ea)} undirected edges, n={n} ({time.perf_counter() - t:.1f}s)")
     t = time.perf_counter()
     g = L.Graph.from_edges(ea, eb)
-    del ea, eb
     info = g.info()
     log(f"csr: n={info['n']} E={info['E']} ({time.perf_counter() - t:.1f}s)")
     t = time.perf_counter()
@@ -469,6 +515,10 @@ def main():
         out["cpu_baseline"] = cpu_baseline(row, col, args.cpu_sweeps)
         del row, col
     del c, g
+    if not args.no_ingest:
+        log("ingest ...")
+        out["ingest"] = ingest_result(args, L, ea, eb, prm)
+    del ea, eb
     if not args.no_batched:
         log("batched configuration ...")
         out["batched"] = batched_result(args, L)
