@@ -62,6 +62,10 @@ typedef struct lv_params {
     int32_t autotune;         /* 1: on a community's first fast and first hash sweep, time
                                  each degree bucket's tile candidates and keep the fastest
                                  (decisions, hence results, do not depend on the tile) */
+    int32_t refine;           /* 1: after the level loop, sweep the final partition again on
+                                 the level-0 graph (RefinePhase; the reference's is empty,
+                                 phaseRefine.cuh:1-3, so 0 by default).  Not for batches
+                                 or the server. */
 } lv_params;
 
 typedef struct lv_sweep_rec {
@@ -167,6 +171,9 @@ lv_status lv_optimization_phase(lv_community *c, const lv_params *p);
 lv_status lv_aggregation_phase(lv_community *c, int aggregate, const lv_params *p);
 /* RefinePhase::run() (phaseRefine.cuh:1-3): a no-op, as in the reference */
 lv_status lv_refine_phase(lv_community *c);
+/* The opt-in refinement on a community whose level loop has ended: the final
+ * partition swept again on the level-0 graph, best_communities renumbered. */
+lv_status lv_refine(lv_community *c, const lv_params *p, lv_result *r);
 /* Community::modularity (community.h:26) */
 double lv_modularity(const lv_community *c);
 /* Community::best_communities / n_of_best_communities (community.h:20,25) */

@@ -17,6 +17,10 @@ class OptimizationPhase {
     // with explicit parameters; records one lv_sweep_rec per sweep if rec != nullptr
     static int run(Community &c, const lv_params &p, lv_sweep_rec *rec = nullptr, int rec_cap = 0,
                    int hash_trace = 0, uint64_t *traversed = nullptr);
+    // The sweep loop from sweep index exec0 with the partition and active
+    // lists as they stand; sweep records carry `rec_level`.
+    static int sweep_loop(Community &c, const lv_params &p, int64_t exec0, int32_t rec_level,
+                          lv_sweep_rec *rec, int rec_cap, int hash_trace, uint64_t *traversed);
     // Which sweep kind the schedule assigns to sweep `execution_number`.
     static int sweep_kind(int mode, int64_t execution_number);
 };

@@ -152,6 +152,7 @@ void GraphBatch::run(const lv_params &p, lv_batch_result *r) {
     SweepState &st = *C.state;
     const uint32_t G = B.n_graphs;
     hipStream_t s = C.stream();
+    if (p.refine) throw Error(LV_ERR_ARG, "refinement is not batched");
     if (p.chunk_emulation)
         for (uint32_t g = 0; g < G; ++g)
             if (B.entries[g] > p.chunk_entries)

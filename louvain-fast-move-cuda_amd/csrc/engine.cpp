@@ -14,6 +14,7 @@
 #include "../../include/modularityAlgorithms.h"
 #include "../../include/phaseAggregation.h"
 #include "../../include/phaseOptimization.h"
+#include "../../include/phaseRefine.h"
 #include "engine_state.h"
 #include "lv_kernels.h"
 
@@ -511,12 +512,18 @@ void launch_sweep(Community &c, const lv_params &p, int kind, bool apply, uint32
 
 int OptimizationPhase::run(Community &c, const lv_params &p, lv_sweep_rec *rec, int rec_cap,
                            int hash_trace, uint64_t *traversed) {
+    // phaseOptimization.cuh:52: every vertex starts active
+    c.state->build_static_lists(c.stream());
+    return sweep_loop(c, p, 0, c.level, rec, rec_cap, hash_trace, traversed);
+}
+
+int OptimizationPhase::sweep_loop(Community &c, const lv_params &p, int64_t exec0,
+                                  int32_t rec_level, lv_sweep_rec *rec, int rec_cap,
+                                  int hash_trace, uint64_t *traversed) {
     SweepState &st = *c.state;
     hipStream_t s = c.stream();
     const uint32_t n = c.graph.n_nodes;
-    // phaseOptimization.cuh:52: every vertex starts active
-    st.build_static_lists(s);
-    int64_t exec = 0;
+    int64_t exec = exec0;
     double delta;
     int nrec = 0;
     std::vector<uint32_t> comm_host;
@@ -583,7 +590,7 @@ int OptimizationPhase::run(Community &c, const lv_params &p, lv_sweep_rec *rec, 
         if (rec && nrec < rec_cap) {
             lv_sweep_rec &r = rec[nrec];
             std::memset(&r, 0, sizeof(r));
-            r.level = c.level;
+            r.level = rec_level;
             r.sweep = (int32_t)exec;
             r.kind = kind;
             r.n = n;
@@ -847,6 +854,39 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
     c.graph = GraphDevice::view_of(ng);
 }
 
+// ------------------------------------------------------------ RefinePhase
+
+int RefinePhase::run(Community &c, const lv_params &p, int32_t rec_level, lv_sweep_rec *rec,
+                     int rec_cap, int hash_trace, uint64_t *traversed) {
+    if (c.batch) throw Error(LV_ERR_ARG, "refinement of a batch is not supported");
+    hipStream_t s = c.stream();
+    // the final partition on the level-0 graph (its ids are renumbered, < n)
+    c.graph = GraphDevice::view_of(c.graph0);
+    c.level = 0;
+    const uint32_t n = c.graph.n_nodes;
+    c.communities.ensure(std::max<size_t>(n, 1));
+    c.communities_weight.ensure(std::max<size_t>(n, 1));
+    if (n)
+        LV_HIP(hipMemcpyAsync(c.communities.data(), c.best_communities.data(), n * sizeof(uint32_t),
+                              hipMemcpyDeviceToDevice, s));
+    c.state->setup_level(c.graph, s);
+    if (n)
+        LV_HIP(hipMemcpyAsync(c.state->target.data(), c.communities.data(), n * sizeof(uint32_t),
+                              hipMemcpyDeviceToDevice, s));
+    community_totals(c.communities.data(), c.graph.tot_weight_per_nodes.data(), n,
+                     c.communities_weight.data(), s);
+    c.compute_modularity();
+    c.state->build_static_lists(s);
+    const int ns = OptimizationPhase::sweep_loop(c, p, 1, rec_level, rec, rec_cap, hash_trace,
+                                                 traversed);
+    // renumbering: level-0 vertices are the original ones, so best = map[comm] - 1
+    iota_u32(c.best_communities.data(), n, s);
+    uint32_t n_new = 0;
+    (void)renumber(c, n_new);
+    c.n_of_best_communities = n_new;
+    return ns;
+}
+
 // ---------------------------------------------------- ModularityAlgorithms
 
 Community ModularityAlgorithms::Louvain(GraphHost &graph, MODE mode) {
@@ -905,6 +945,11 @@ void ModularityAlgorithms::run(Community &C, const lv_params &p, lv_result *r) {
         ++nlev;
         if (!cont) C.modularity = q_end;
     } while (cont);
+    if (p.refine) {
+        lv_sweep_rec *rec = (r && r->sweeps && nrec < r->sweeps_cap) ? r->sweeps + nrec : nullptr;
+        const int cap = rec ? r->sweeps_cap - nrec : 0;
+        nrec += RefinePhase::run(C, p, nlev, rec, cap, r ? r->hash_trace : 0, &traversed);
+    }
     LV_HIP(hipStreamSynchronize(s));
     const auto t1 = std::chrono::steady_clock::now();
     g_last_run_stats = C.state->acc;

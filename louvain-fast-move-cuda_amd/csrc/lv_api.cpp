@@ -81,6 +81,7 @@ void lv_default_params(lv_params *p) {
     p->max_levels = 0;
     p->profile = 0;
     p->autotune = 0;
+    p->refine = 0;
 }
 
 const char *lv_status_string(lv_status s) {
@@ -264,6 +265,31 @@ lv_status lv_refine_phase(lv_community *c) {
     if (!c) return LV_ERR_ARG;
     RefinePhase::run();
     return LV_OK;
+}
+
+lv_status lv_refine(lv_community *c, const lv_params *p, lv_result *r) {
+    if (!c) return LV_ERR_ARG;
+    lv_params d;
+    lv_default_params(&d);
+    return guarded([&] {
+        Community &C = c->c;
+        lv_sweep_rec *rec = (r && r->sweeps) ? r->sweeps : nullptr;
+        uint64_t traversed = 0;
+        const int32_t lvl = C.level + 1;
+        const int ns = RefinePhase::run(C, p ? *p : d, lvl, rec, rec ? r->sweeps_cap : 0,
+                                        r ? r->hash_trace : 0, &traversed);
+        LV_HIP(hipStreamSynchronize(C.stream()));
+        if (r) {
+            r->modularity = C.modularity;
+            r->n_of_best_communities = C.n_of_best_communities;
+            r->levels = 0;
+            r->total_sweeps = ns;
+            r->traversed_entries = traversed;
+            if (r->best_communities && C.n_original)
+                LV_HIP(hipMemcpy(r->best_communities, C.best_communities.data(),
+                                 C.n_original * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        }
+    });
 }
 
 double lv_modularity(const lv_community *c) { return c ? c->c.modularity : 0.0; }

@@ -429,6 +429,7 @@ struct LouvainServer::Impl {
 LouvainServer::LouvainServer(const lv_params &p, uint64_t hbm_budget_bytes, hipStream_t stream)
     : d_(std::make_unique<Impl>(p, hbm_budget_bytes, stream)) {
     if (p.chunk_emulation) throw Error(LV_ERR_ARG, "chunk emulation is not served");
+    if (p.refine) throw Error(LV_ERR_ARG, "refinement is not served");
 }
 
 LouvainServer::~LouvainServer() = default;

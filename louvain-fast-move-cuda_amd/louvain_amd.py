@@ -35,7 +35,7 @@ class Params(C.Structure):
                 ("chunk_emulation", C.c_int32), ("chunk_entries", C.c_uint64),
                 ("sweep_threshold", C.c_double), ("level_threshold", C.c_double),
                 ("max_sweeps", C.c_int64), ("max_levels", C.c_int32), ("profile", C.c_int32),
-                ("autotune", C.c_int32)]
+                ("autotune", C.c_int32), ("refine", C.c_int32)]
 
 
 class SweepRec(C.Structure):
@@ -117,6 +117,7 @@ SIGNATURES = {
     "lv_optimization_phase": (C.c_int, [_vp, C.POINTER(Params)]),
     "lv_aggregation_phase": (C.c_int, [_vp, C.c_int, C.POINTER(Params)]),
     "lv_refine_phase": (C.c_int, [_vp]),
+    "lv_refine": (C.c_int, [_vp, C.POINTER(Params), C.POINTER(Result)]),
     "lv_modularity": (C.c_double, [_vp]),
     "lv_best_communities": (C.c_int, [_vp, _vp, C.c_size_t, C.POINTER(C.c_uint32)]),
     "lv_louvain": (C.c_int, [_vp, C.POINTER(Params), C.POINTER(Result)]),
@@ -368,6 +369,11 @@ class Community:
 
     def reset(self):
         _check(lib.lv_community_reset(self.h))
+
+    def refine(self, n0, prm=None, sweeps_cap=100000, hash_trace=False, best=True):
+        """The opt-in refinement after a run (n0 = vertices of the level-0 graph)."""
+        return _run_result(lambda r: lib.lv_refine(self.h, C.byref(prm or params()), r),
+                           n0, sweeps_cap, 1, hash_trace, best)
 
     def run(self, prm=None, sweeps_cap=100000, levels_cap=1000, hash_trace=False, best=True,
             n0=None):

@@ -43,6 +43,7 @@ void lvo_default_params(lvo_params *p) {
     p->max_sweeps = 1000000000ll;            /* EARLY_STOP_LIMIT, constants.h:16 */
     p->max_total_sweeps = 0;
     p->max_levels = 0;
+    p->refine = 0;
 }
 
 uint64_t lvo_fnv1a_u32(const uint32_t *a, uint64_t n) {
@@ -540,6 +541,65 @@ static int sweep_kind(int mode, int64_t exec) {
     return LVO_SWEEP_SORT;
 }
 
+/* internal_run (phaseOptimization.cuh:158-181) from sweep index exec0 with
+ * partition comm (totals tot, active set active); returns the sweep count
+ * reached (exec).  Records sweeps at `level`. */
+static int64_t sweep_loop(const lvo_graph *g, const lvo_params *p, lvo_result *r, int32_t level,
+                          int64_t exec0, uint32_t *comm, uint32_t *target, int64_t *tot,
+                          uint8_t *active, uint8_t *active_next, double *qp,
+                          int32_t *n_sweep_recs, int32_t *total_sweeps, int *stop_all) {
+    const uint32_t n = g->n;
+    double q = *qp;
+    int64_t exec = exec0;
+    double delta;
+    do {
+        const double old_q = q;
+        const int kind = sweep_kind(p->mode, exec);
+        uint32_t n_active = 0;
+        for (uint32_t v = 0; v < n; ++v) n_active += active[v];
+        uint64_t e_active = 0;
+        uint32_t n_considered = 0;
+        int64_t moved = lvo_best_moves(g, comm, tot, active, kind, p, target, &e_active, &n_considered);
+        /* fast_move_update (phaseOptimizationUpdate.cu:135-209): apply all
+         * moves at once, then active[src] |= moved[dst] && comm[dst]!=comm[src] */
+        memset(active_next, 0, n);
+        for (uint32_t v = 0; v < n; ++v) {
+            uint32_t c = target[v];
+            if (c != comm[v]) {
+                tot[c] += (int64_t)g->k[v];
+                tot[comm[v]] -= (int64_t)g->k[v];
+            }
+        }
+        uint8_t *changed = active; /* reuse: its_changed */
+        for (uint32_t v = 0; v < n; ++v) { changed[v] = target[v] != comm[v]; comm[v] = target[v]; }
+        for (uint32_t v = 0; v < n; ++v)
+            for (uint64_t j = g->row[v]; j < g->row[v + 1]; ++j) {
+                uint32_t x = g->col[j];
+                if (changed[x] && comm[x] != comm[v]) { active_next[v] = 1; break; }
+            }
+        memcpy(active, active_next, n);
+        ++exec;
+        int64_t sigma_in; uint64_t stsq;
+        q = lvo_modularity(g, comm, &sigma_in, &stsq);
+        delta = q - old_q;
+        if (r && r->sweeps && *n_sweep_recs < r->sweeps_cap) {
+            lvo_sweep_rec *s = &r->sweeps[*n_sweep_recs];
+            memset(s, 0, sizeof(*s));
+            s->level = level; s->sweep = (int32_t)(exec - 1); s->kind = kind;
+            s->n = n; s->n_active = n_active; s->n_moved = (uint32_t)moved;
+            s->n_considered = n_considered;
+            s->E = g->E; s->e_active = e_active;
+            s->sigma_in = sigma_in; s->sum_tot_sq = stsq; s->q = q;
+            s->comm_hash = lvo_fnv1a_u32(comm, n);
+        }
+        ++*n_sweep_recs;
+        ++*total_sweeps;
+        if (p->max_total_sweeps > 0 && *total_sweeps >= p->max_total_sweeps) { *stop_all = 1; break; }
+    } while (exec <= p->max_sweeps && delta > p->sweep_threshold);
+    *qp = q;
+    return exec;
+}
+
 int lvo_louvain(const lvo_graph *g0, const lvo_params *p, lvo_result *r) {
     const uint32_t n0 = g0->n;
     uint32_t *best = (uint32_t *)malloc((n0 ? n0 : 1) * sizeof(uint32_t));
@@ -569,53 +629,8 @@ int lvo_louvain(const lvo_graph *g0, const lvo_params *p, lvo_result *r) {
         for (uint32_t v = 0; v < n; ++v) { comm[v] = v; tot[v] = (int64_t)g->k[v]; active[v] = 1; }
         q = lvo_modularity(g, comm, NULL, NULL);
         const double q_level_start = q;
-        int64_t exec = 0;
-        double delta;
-        /* internal_run (phaseOptimization.cuh:158-181) */
-        do {
-            const double old_q = q;
-            const int kind = sweep_kind(p->mode, exec);
-            uint32_t n_active = 0;
-            for (uint32_t v = 0; v < n; ++v) n_active += active[v];
-            uint64_t e_active = 0;
-            uint32_t n_considered = 0;
-            int64_t moved = lvo_best_moves(g, comm, tot, active, kind, p, target, &e_active, &n_considered);
-            /* fast_move_update (phaseOptimizationUpdate.cu:135-209): apply all
-             * moves at once, then active[src] |= moved[dst] && comm[dst]!=comm[src] */
-            memset(active_next, 0, n);
-            for (uint32_t v = 0; v < n; ++v) {
-                uint32_t c = target[v];
-                if (c != comm[v]) {
-                    tot[c] += (int64_t)g->k[v];
-                    tot[comm[v]] -= (int64_t)g->k[v];
-                }
-            }
-            uint8_t *changed = active; /* reuse: its_changed */
-            for (uint32_t v = 0; v < n; ++v) { changed[v] = target[v] != comm[v]; comm[v] = target[v]; }
-            for (uint32_t v = 0; v < n; ++v)
-                for (uint64_t j = g->row[v]; j < g->row[v + 1]; ++j) {
-                    uint32_t x = g->col[j];
-                    if (changed[x] && comm[x] != comm[v]) { active_next[v] = 1; break; }
-                }
-            memcpy(active, active_next, n);
-            ++exec;
-            int64_t sigma_in; uint64_t stsq;
-            q = lvo_modularity(g, comm, &sigma_in, &stsq);
-            delta = q - old_q;
-            if (r && r->sweeps && n_sweep_recs < r->sweeps_cap) {
-                lvo_sweep_rec *s = &r->sweeps[n_sweep_recs];
-                memset(s, 0, sizeof(*s));
-                s->level = level; s->sweep = (int32_t)(exec - 1); s->kind = kind;
-                s->n = n; s->n_active = n_active; s->n_moved = (uint32_t)moved;
-                s->n_considered = n_considered;
-                s->E = g->E; s->e_active = e_active;
-                s->sigma_in = sigma_in; s->sum_tot_sq = stsq; s->q = q;
-                s->comm_hash = lvo_fnv1a_u32(comm, n);
-            }
-            ++n_sweep_recs;
-            ++total_sweeps;
-            if (p->max_total_sweeps > 0 && total_sweeps >= p->max_total_sweeps) { stop_all = 1; break; }
-        } while (exec <= p->max_sweeps && delta > p->sweep_threshold);
+        const int64_t exec = sweep_loop(g, p, r, level, 0, comm, target, tot, active, active_next,
+                                        &q, &n_sweep_recs, &total_sweeps, &stop_all);
         /* modularity_routine (modularityAlgorithms.h:52-53) */
         cont = (q - q_level_start) > p->level_threshold;
         if (stop_all) cont = 0;
@@ -645,6 +660,33 @@ int lvo_louvain(const lvo_graph *g0, const lvo_params *p, lvo_result *r) {
         ++level;
     } while (cont);
     if (owned) lvo_graph_free(owned);
+    /* Refinement (opt-in; the reference's RefinePhase is empty, phaseRefine.cuh:1-3):
+     * the final partition on the level-0 graph, swept again from sweep index 1
+     * (no fast sweep: the partition is not the identity), then renumbered. */
+    if (rc == LVO_OK && p->refine && !stop_all) {
+        uint32_t *comm = (uint32_t *)malloc((n0 ? n0 : 1) * sizeof(uint32_t));
+        uint32_t *target = (uint32_t *)malloc((n0 ? n0 : 1) * sizeof(uint32_t));
+        int64_t *tot = (int64_t *)calloc(n0 ? n0 : 1, sizeof(int64_t));
+        uint8_t *active = (uint8_t *)malloc(n0 ? n0 : 1);
+        uint8_t *active_next = (uint8_t *)malloc(n0 ? n0 : 1);
+        uint32_t *new_id = (uint32_t *)malloc((n0 ? n0 : 1) * sizeof(uint32_t));
+        if (!comm || !target || !tot || !active || !active_next || !new_id) {
+            rc = LVO_ERR_OOM;
+        } else {
+            for (uint32_t v = 0; v < n0; ++v) {
+                comm[v] = best[v];
+                tot[best[v]] += (int64_t)g0->k[v];
+                active[v] = 1;
+            }
+            q = lvo_modularity(g0, comm, NULL, NULL);
+            (void)sweep_loop(g0, p, r, level, 1, comm, target, tot, active, active_next, &q,
+                             &n_sweep_recs, &total_sweeps, &stop_all);
+            rc = lvo_aggregate(g0, comm, new_id, &n_best, NULL);
+            if (rc == LVO_OK)
+                for (uint32_t v = 0; v < n0; ++v) best[v] = new_id[comm[v]];
+        }
+        free(comm); free(target); free(tot); free(active); free(active_next); free(new_id);
+    }
     if (r) {
         r->modularity = q;
         r->n_of_best_communities = n_best;

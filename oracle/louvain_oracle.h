@@ -65,6 +65,10 @@ typedef struct lvo_params {
     int64_t max_sweeps;       /* EARLY_STOP_LIMIT (constants.h:16) */
     int64_t max_total_sweeps; /* cap over all levels (bounded CPU samples); <=0: none */
     int32_t max_levels;       /* <=0: none */
+    int32_t refine;           /* 1: after the level loop, project the partition onto the
+                                 level-0 graph and sweep again (no fast sweep) until the
+                                 sweep threshold; the reference's RefinePhase is empty
+                                 (phaseRefine.cuh:1-3), so this is off by default */
 } lvo_params;
 
 typedef struct lvo_sweep_rec {

@@ -26,7 +26,7 @@ class Params(C.Structure):
                 ("chunk_emulation", C.c_int), ("chunk_entries", C.c_uint64),
                 ("sweep_threshold", C.c_double), ("level_threshold", C.c_double),
                 ("max_sweeps", C.c_int64), ("max_total_sweeps", C.c_int64),
-                ("max_levels", C.c_int32)]
+                ("max_levels", C.c_int32), ("refine", C.c_int32)]
 
 
 class SweepRec(C.Structure):
@@ -92,7 +92,10 @@ def _p(a):
 def params(**kw):
     p = Params()
     lib().lvo_default_params(C.byref(p))
+    names = {f[0] for f in Params._fields_}
     for k, v in kw.items():
+        if k not in names:
+            raise TypeError(f"unknown lvo_params field {k!r}")
         setattr(p, k, v)
     return p
 

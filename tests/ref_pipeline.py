@@ -155,11 +155,15 @@ def aggregate(src, dst, w, k, comm):
 
 
 def louvain(row, col, w, tw, mode="hash", gain="exact", tie="lowest",
-            sweep_thr=0.001, level_thr=0.001):
-    """modularityAlgorithms.h:14-81 with phaseOptimization.cuh:116-181."""
+            sweep_thr=0.001, level_thr=0.001, refine=False):
+    """modularityAlgorithms.h:14-81 with phaseOptimization.cuh:116-181.
+    refine: afterwards, the partition on the level-0 graph swept again from
+    sweep index 1 (no fast sweep) and renumbered (the opt-in refinement)."""
     src, dst, ww = csr_to_coo(np.asarray(row), np.asarray(col), np.asarray(w))
+    src0, dst0, ww0 = src, dst, ww
     n0 = len(row) - 1
     k = np.bincount(src, weights=ww, minlength=n0).astype(np.int64)
+    k0 = k
     best = np.arange(n0, dtype=np.int64)
     trace = []
     level = 0
@@ -193,4 +197,23 @@ def louvain(row, col, w, tw, mode="hash", gain="exact", tie="lowest",
             break
         src, dst, ww, k = nsrc, ndst, nw, nk
         level += 1
+    if refine:
+        level += 1
+        comm = best.copy()
+        tot = np.bincount(comm, weights=k0, minlength=n0).astype(np.int64)
+        q = modularity(src0, dst0, ww0, k0, comm, tw)[0]
+        active = np.ones(n0, np.uint8)
+        ex = 1
+        while True:
+            old = q
+            kind = HASH if mode == "hash" else SORT
+            comm, tot, active, moved, sel = sweep(kind, src0, dst0, ww0, k0, comm, tot, active,
+                                                  tw, gain, tie)
+            ex += 1
+            q, si, s, _ = modularity(src0, dst0, ww0, k0, comm, tw)
+            trace.append((level, ex - 1, kind, moved, sel, si, s, q))
+            if not (q - old > sweep_thr):
+                break
+        cmap, n_new, _, _, _, _ = aggregate(src0, dst0, ww0, k0, comm)
+        best = cmap[comm]
     return {"modularity": q, "best": best, "n_comm": n_new, "trace": trace}

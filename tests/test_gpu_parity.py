@@ -280,3 +280,26 @@ def test_every_tile_choice_and_the_tuner_give_identical_runs():
     compare_runs(t.run(L.params(sweep_threshold=1e-6, level_threshold=1e-6, autotune=1),
                        hash_trace=True), ref, "autotuned")
     say(f"tuned tiles {t.tiles().tolist()}")
+
+
+@pytest.mark.parametrize("mode,gain", list(itertools.product(MODES[:2], GAINS)),
+                         ids=lambda x: x[0])
+def test_refine_matches_oracle(fixture_path, mode, gain):
+    """The opt-in refinement (final partition swept again on the level-0
+    graph), trace for trace against the oracle; lv_refine after a plain run
+    gives the same partition as refine=1."""
+    rng = np.random.default_rng(77)
+    ea, eb = _random_graph(rng, n=6000, m=30000, hubs=1, hub_deg=5000, selfloops=10)
+    w = rng.integers(1, 6, len(ea)).astype(np.uint32)
+    for label, g, og in (("fixture", L.Graph.load_edge(fixture_path), O.Graph.load_edge(fixture_path)),
+                         ("random", L.Graph.from_edges(ea, eb, w), O.Graph.from_edges(ea, eb, w))):
+        kw = dict(sweep_threshold=1e-6, level_threshold=1e-6)
+        say(f"refine {label} {mode[0]} {gain[0]}")
+        a = g.louvain(L.params(mode=mode[1], gain=gain[1], refine=1, **kw), hash_trace=True)
+        b = og.louvain(O.params(mode=mode[2], gain=gain[2], refine=1, **kw))
+        compare_runs(a, b, f"refine {label} {mode[0]} {gain[0]}")
+        c = L.Community(g)
+        plain = c.run(L.params(mode=mode[1], gain=gain[1], **kw))
+        assert a["modularity"] >= plain["modularity"]
+        r = c.refine(g.n, L.params(mode=mode[1], gain=gain[1], **kw))
+        assert r["modularity"] == a["modularity"] and np.array_equal(r["best"], a["best"])

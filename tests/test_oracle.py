@@ -270,3 +270,32 @@ def test_edge_parsing(tmp_path):
     empty.write_text("x\n")
     with pytest.raises(RuntimeError, match="parse"):
         O.Graph.load_edge(str(empty))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_oracle_refine_matches_pipeline(fixture_path, seed):
+    """Opt-in refinement: the final partition swept again on the level-0 graph
+    (no fast sweep) and renumbered -- the oracle against the independent numpy
+    pipeline, trace for trace; its modularity equals a from-scratch
+    recomputation and, on these inputs, is no lower than without refinement."""
+    if seed == 0:
+        g = fixture_graph(fixture_path)
+    else:
+        rng = np.random.default_rng(300 + seed)
+        n = int(rng.integers(100, 600))
+        a = np.concatenate([rng.integers(0, n, 4 * n), np.arange(n)])
+        b = np.concatenate([rng.integers(0, n, 4 * n), (np.arange(n) + 1) % n])
+        w = rng.integers(1, 8, len(a)).astype(np.uint32) if seed % 2 else None
+        g = O.Graph.from_edges(a, b, w)
+    row, col, w, _ = g.arrays()
+    for mode, gain, tie in COMBOS[:4]:
+        ra = R.louvain(row, col, w, g.total_weight, mode, gain, tie, 1e-6, 1e-6, refine=True)
+        rb = g.louvain(O.params(mode=ENUM[mode], gain=ENUM[gain], tie=ENUM[tie],
+                                sweep_threshold=1e-6, level_threshold=1e-6, refine=1))
+        assert trace_of(rb) == ra["trace"], (seed, mode, gain, tie)
+        assert np.array_equal(ra["best"], rb["best"]) and ra["n_comm"] == rb["n_comm"]
+        assert rb["modularity"] == g.modularity(rb["best"])[0]
+        plain = g.louvain(O.params(mode=ENUM[mode], gain=ENUM[gain], tie=ENUM[tie],
+                                   sweep_threshold=1e-6, level_threshold=1e-6))
+        assert rb["modularity"] >= plain["modularity"]
+        assert rb["sweeps"][:len(plain["sweeps"])] == plain["sweeps"]
