@@ -734,16 +734,32 @@ __global__ __launch_bounds__(kBlock) void k_hub_scan(HubCtx a, const uint32_t *_
             const uint32_t kv = a.m.kdeg[v];
             const uint32_t tot_own = a.m.tot[own];
             const uint32_t self = a.h.self[slot];
-            for (uint32_t s = part * per + threadIdx.x; s < (part + 1) * per; s += kBlock) {
-                const unsigned long long cur = a.h.slot[off + s];
-                const uint32_t c = (uint32_t)(cur >> 32);
-                if (c == LV_EMPTY) continue;
-                const uint32_t W = (uint32_t)cur;
-                a.h.slot[off + s] = kHubEmpty;
-                if (c == own) continue;
-                ++ncand;
-                const S sc = SC::of(W, self, kv, tot_own, a.m.tot[c], m2, tw);
-                if (better_candidate(sc, c, best, best_c, a.m.highest)) { best = sc; best_c = c; }
+            const uint32_t s_end = (part + 1) * per;
+            for (uint32_t sb = part * per + threadIdx.x; sb < s_end; sb += kBlock * kIlp) {
+                // kIlp slots per lane: slot reads, then the tot gathers, in flight together
+                unsigned long long cur[kIlp];
+                uint32_t ts[kIlp];
+#pragma unroll
+                for (int u = 0; u < kIlp; ++u) {
+                    const uint32_t s2 = sb + (uint32_t)u * kBlock;
+                    cur[u] = s2 < s_end ? a.h.slot[off + s2] : kHubEmpty;
+                }
+#pragma unroll
+                for (int u = 0; u < kIlp; ++u) {
+                    const uint32_t c = (uint32_t)(cur[u] >> 32);
+                    ts[u] = (c != LV_EMPTY && c != own) ? a.m.tot[c] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < kIlp; ++u) {
+                    const uint32_t s2 = sb + (uint32_t)u * kBlock;
+                    const uint32_t c = (uint32_t)(cur[u] >> 32);
+                    if (c == LV_EMPTY) continue;
+                    a.h.slot[off + s2] = kHubEmpty;
+                    if (c == own) continue;
+                    ++ncand;
+                    const S sc = SC::of((uint32_t)cur[u], self, kv, tot_own, ts[u], m2, tw);
+                    if (better_candidate(sc, c, best, best_c, a.m.highest)) { best = sc; best_c = c; }
+                }
             }
         }
     }
