@@ -112,10 +112,11 @@ struct LouvainServer::Impl {
     std::deque<std::pair<uint64_t, uint64_t>> ring_allocs;   // (offset, bytes), oldest first
     DeviceArray<uint8_t> ring;
     uint64_t ring_cap = 0;
-    hipStream_t ustream = nullptr;
+    static constexpr int kUploaders = 4;   // pageable copies scale with host threads
+    hipStream_t ustream[kUploaders] = {};
     int device = 0;
     bool stop = false;
-    std::thread uploader;
+    std::thread uploader[kUploaders];
 
     Impl(const lv_params &pp, uint64_t b, hipStream_t s) : p(pp), budget(b), C(s) {
         C.batch = std::make_unique<BatchState>();
@@ -123,8 +124,10 @@ struct LouvainServer::Impl {
         // an eighth of the budget (at most 16 GiB); larger jobs are uploaded at assembly
         ring_cap = std::min<uint64_t>(std::max<uint64_t>(budget / 8, 1ull << 20), 16ull << 30);
         ring.resize(ring_cap);
-        LV_HIP(hipStreamCreateWithFlags(&ustream, hipStreamNonBlocking));
-        uploader = std::thread([this] { upload_loop(); });
+        for (int t = 0; t < kUploaders; ++t) {
+            LV_HIP(hipStreamCreateWithFlags(&ustream[t], hipStreamNonBlocking));
+            uploader[t] = std::thread([this, t] { upload_loop(ustream[t]); });
+        }
     }
 
     ~Impl() {
@@ -133,8 +136,10 @@ struct LouvainServer::Impl {
             stop = true;
         }
         cv.notify_all();
-        if (uploader.joinable()) uploader.join();
-        if (ustream) (void)hipStreamDestroy(ustream);
+        for (int t = 0; t < kUploaders; ++t) {
+            if (uploader[t].joinable()) uploader[t].join();
+            if (ustream[t]) (void)hipStreamDestroy(ustream[t]);
+        }
     }
 
     // FIFO ring allocation (called with mu held)
@@ -150,7 +155,9 @@ struct LouvainServer::Impl {
         return head + bytes <= tail ? head : kNoSpace;
     }
 
-    void upload_loop() {
+    // Each uploader takes the oldest job waiting, reserves its ring space (in
+    // queue order, so the ring stays FIFO) and copies it on its own stream.
+    void upload_loop(hipStream_t us) {
         (void)hipSetDevice(device);
         for (;;) {
             Job *j = nullptr;
@@ -181,14 +188,14 @@ struct LouvainServer::Impl {
             const GraphHost &h = *j->g;
             uint8_t *base = ring.data() + off;
             bool ok = hipMemcpyAsync(base, h.row.data(), (size_t)j->n0 * 8, hipMemcpyHostToDevice,
-                                     ustream) == hipSuccess;
+                                     us) == hipSuccess;
             if (ok && j->E0)
                 ok = hipMemcpyAsync(base + ring_row_bytes(*j), h.col.data(), j->E0 * 4,
-                                    hipMemcpyHostToDevice, ustream) == hipSuccess;
+                                    hipMemcpyHostToDevice, us) == hipSuccess;
             if (ok && j->E0 && !h.unit_weights)
                 ok = hipMemcpyAsync(base + ring_row_bytes(*j) + ring_col_bytes(*j), h.weights.data(),
-                                    j->E0 * 4, hipMemcpyHostToDevice, ustream) == hipSuccess;
-            ok = (hipStreamSynchronize(ustream) == hipSuccess) && ok;
+                                    j->E0 * 4, hipMemcpyHostToDevice, us) == hipSuccess;
+            ok = (hipStreamSynchronize(us) == hipSuccess) && ok;
             {
                 std::lock_guard<std::mutex> lk(mu);
                 j->ring_off = off;
