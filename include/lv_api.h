@@ -275,6 +275,10 @@ typedef struct lv_job_result {
     double seconds_total;             /* submit -> retirement */
 } lv_job_result;
 
+/* hbm_budget_bytes caps the estimated footprint of the active jobs
+ * (lv_server_footprint); the upload ring (budget / 8, at most 16 GiB) and the
+ * grow-only scratch of the union come on top.  A server whose step returned
+ * an error is left in an unspecified state: destroy it. */
 lv_status lv_server_create(const lv_params *p, uint64_t hbm_budget_bytes, void *stream,
                            lv_server **out);
 void lv_server_destroy(lv_server *s);
