@@ -64,6 +64,8 @@ void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
     }
     // hub tables
     n_hubs = bucket_size[kHubBucket];
+    // hub work items carry the slot in 24 bits (2m < 2^31 keeps hubs < 2^19)
+    if (n_hubs >= (1u << 24)) throw Error(LV_ERR_RANGE, "too many hub vertices");
     if (n_hubs > 0) {
         build_static_lists(s);
         std::vector<uint32_t> hubs(n_hubs);
@@ -76,30 +78,28 @@ void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
         LV_HIP(hipMemcpyAsync(hdeg.data(), hdeg_dev, n_hubs * sizeof(uint64_t),
                               hipMemcpyDeviceToHost, s));
         LV_HIP(hipStreamSynchronize(s));
-        std::vector<uint64_t> toff(n_hubs);
-        std::vector<uint32_t> tlog(n_hubs);
-        uint64_t pool = 0;
-        std::vector<uint32_t> slot_pairs;  // (vertex, slot)
+        std::vector<uint64_t> eoff(n_hubs);
+        uint64_t hub_entries = 0;
         for (uint32_t i = 0; i < n_hubs; ++i) {
-            const uint64_t d = hdeg[i];
-            uint32_t lg = 1;
-            while ((1ull << lg) < 2 * d) ++lg;
-            tlog[i] = lg;
-            toff[i] = pool;
-            pool += 1ull << lg;
+            eoff[i] = hub_entries;
+            hub_entries += hdeg[i];
         }
         hub_slot_of.ensure(n);
         fill_u32(hub_slot_of.data(), LV_EMPTY, n, s);
-        hub_tab_off.ensure(n_hubs);
-        hub_tab_log2.ensure(n_hubs);
-        hub_slot.ensure(pool);
+        hub_ent_off.ensure(n_hubs);
+        hub_cbuf.ensure(hub_entries);
+        hub_pair.ensure(hub_entries);
+        hub_cnt.ensure((size_t)n_hubs * kHubParts);
+        hub_base.ensure((size_t)n_hubs * kHubParts * kHubParts);
+        hub_items.ensure((size_t)n_hubs * kHubParts);
+        hub_n_items.ensure(1);
+        fill_u32(hub_n_items.data(), 0u, 1, s);
         hub_part_s.ensure((size_t)n_hubs * kHubParts);
         hub_part_c.ensure((size_t)n_hubs * kHubParts);
         hub_self.ensure(n_hubs);
-        fill_u64(hub_slot.data(), kHubEmpty, pool, s);
-        LV_HIP(hipMemcpyAsync(hub_tab_off.data(), toff.data(), n_hubs * sizeof(uint64_t),
-                              hipMemcpyHostToDevice, s));
-        LV_HIP(hipMemcpyAsync(hub_tab_log2.data(), tlog.data(), n_hubs * sizeof(uint32_t),
+        fill_u32(hub_cnt.data(), 0u, (size_t)n_hubs * kHubParts, s);
+        fill_u32(hub_self.data(), 0u, n_hubs, s);
+        LV_HIP(hipMemcpyAsync(hub_ent_off.data(), eoff.data(), n_hubs * sizeof(uint64_t),
                               hipMemcpyHostToDevice, s));
         std::vector<uint32_t> slots_host(n_hubs);
         for (uint32_t i = 0; i < n_hubs; ++i) slots_host[i] = i;
@@ -131,9 +131,13 @@ uint32_t *const *SweepState::lists(int which) {
 HubTables SweepState::hub_tables() {
     HubTables h;
     h.slot_of = hub_slot_of.data();
-    h.tab_off = hub_tab_off.data();
-    h.tab_log2 = hub_tab_log2.data();
-    h.slot = hub_slot.data();
+    h.ent_off = hub_ent_off.data();
+    h.cbuf = hub_cbuf.data();
+    h.pair = hub_pair.data();
+    h.cnt = hub_cnt.data();
+    h.base = hub_base.data();
+    h.items = hub_items.data();
+    h.n_items = hub_n_items.data();
     h.part_s = hub_part_s.data();
     h.part_c = hub_part_c.data();
     h.self = hub_self.data();
@@ -689,8 +693,8 @@ uint32_t chunk_vertex_end(Community &c, const lv_params &p, int kind, const uint
 namespace lv {
 enum AggSlot : size_t {
     W_MAP, W_SCAN32, W_NEWC, W_VK0, W_VK1, W_MEMBERS, W_VV1, W_HIST, W_HIST_SCAN, W_MSTART, W_P,
-    W_SCAN64, W_CROW, W_CBUCKET, W_CCOUNT, W_CLISTS, W_TCOL, W_TW, W_RCOUNT, W_HOFF, W_HK0, W_HK1,
-    W_HV0, W_HV1, W_HIDX, W_HS, W_UKEY, W_WS, W_WE, W_USTART
+    W_SCAN64, W_CROW, W_CBUCKET, W_CCOUNT, W_CLISTS, W_TCOL, W_TW, W_RCOUNT, W_HSIZE, W_GINFO,
+    W_GTILES, W_GITEMS, W_GP0, W_GP1, W_GPCNT, W_GPOFF, W_GDCNT, W_GDOFF, W_GBASE
 };
 
 static int bits_for(uint64_t x) {   // smallest b with 2^b >= x (x >= 1)
@@ -791,48 +795,65 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
     x.tw = ws.get<uint32_t>(W_TW, E);
     x.rcount = ws.get<uint32_t>(W_RCOUNT, n_new);
     agg_tables(x, lp, cnt, s);
-    // 4. giant communities: gather their entries, radix sort, reduce runs
+    // 4. giant communities: their entries in destination hash partitions,
+    //    each partition summed per destination in an LDS table
     const uint32_t nh = cnt[kHubBucket];
-    const uint32_t hub_parts = 64;
-    const int bd = std::max(bits_for(n_new), 1);
-    uint32_t *ustart = nullptr;
-    uint64_t *ukey = nullptr, *wsum0 = nullptr, *wsum1 = nullptr;
+    GiantArgs ga{};
+    uint32_t n_parts = 0;
     if (nh) {
-        std::vector<uint32_t> hubs(nh);
-        LV_HIP(hipMemcpyAsync(hubs.data(), lp[kHubBucket], nh * sizeof(uint32_t),
+        std::vector<uint32_t> giants(nh);
+        LV_HIP(hipMemcpyAsync(giants.data(), lp[kHubBucket], nh * sizeof(uint32_t),
                               hipMemcpyDeviceToHost, s));
-        std::vector<uint64_t> hsize(nh);
-        uint64_t *hsize_dev = ws.get<uint64_t>(W_HOFF, (size_t)nh + 1);
-        gather_degrees(crow, lp[kHubBucket], nh, hsize_dev, s);
-        LV_HIP(hipMemcpyAsync(hsize.data(), hsize_dev, nh * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        std::vector<uint64_t> gsize(nh);
+        uint64_t *gsize_dev = ws.get<uint64_t>(W_HSIZE, nh);
+        gather_degrees(crow, lp[kHubBucket], nh, gsize_dev, s);
+        LV_HIP(hipMemcpyAsync(gsize.data(), gsize_dev, nh * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
         LV_HIP(hipStreamSynchronize(s));
-        std::vector<uint64_t> hoff_h((size_t)nh + 1, 0);
-        for (uint32_t i = 0; i < nh; ++i) hoff_h[i + 1] = hoff_h[i] + hsize[i];
-        const uint64_t H = hoff_h[nh];
-        uint64_t *hoff = ws.get<uint64_t>(W_HOFF, (size_t)nh + 1);
-        LV_HIP(hipMemcpyAsync(hoff, hoff_h.data(), ((size_t)nh + 1) * sizeof(uint64_t),
+        std::vector<GiantInfo> info(nh);
+        std::vector<uint64_t> tiles, items;
+        uint64_t off = 0, mbase = 0;
+        uint32_t pbase = 0;
+        for (uint32_t i = 0; i < nh; ++i) {
+            const uint64_t e = gsize[i];
+            uint32_t lgp = 0;
+            while (lgp < (uint32_t)kGiantMaxPartBits && (kGiantPartEntries << lgp) < e) ++lgp;
+            const uint32_t lgt = std::max<uint32_t>(kGiantMinTileBits, lgp + 4);
+            const uint32_t np = 1u << lgp;
+            const uint64_t nt = (e + (1ull << lgt) - 1) >> lgt;
+            info[i] = GiantInfo{off, mbase, giants[i], pbase, lgp, lgt};
+            for (uint64_t t = 0; t < nt; ++t) tiles.push_back(((uint64_t)i << 32) | t);
+            for (uint32_t q = 0; q < np; ++q) items.push_back(((uint64_t)i << 32) | q);
+            off += e;
+            mbase += nt * np;
+            pbase += np;
+        }
+        if (tiles.size() > 0x7FFFFFFFull || items.size() > 0x7FFFFFFFull)
+            throw Error(LV_ERR_RANGE, "giant community plan too large");
+        GiantInfo *info_dev = ws.get<GiantInfo>(W_GINFO, nh);
+        uint64_t *tiles_dev = ws.get<uint64_t>(W_GTILES, tiles.size());
+        uint64_t *items_dev = ws.get<uint64_t>(W_GITEMS, items.size());
+        LV_HIP(hipMemcpyAsync(info_dev, info.data(), nh * sizeof(GiantInfo), hipMemcpyHostToDevice, s));
+        LV_HIP(hipMemcpyAsync(tiles_dev, tiles.data(), tiles.size() * sizeof(uint64_t),
                               hipMemcpyHostToDevice, s));
-        uint64_t *hk0 = ws.get<uint64_t>(W_HK0, H), *hk1 = ws.get<uint64_t>(W_HK1, H);
-        uint32_t *hv0 = ws.get<uint32_t>(W_HV0, H), *hv1 = ws.get<uint32_t>(W_HV1, H);
-        const int hb = std::max(bits_for(nh), 1);
-        agg_hub_gather(x, lp[kHubBucket], nh, hoff, hub_parts, bd, hk0, hv0, s);
-        radix_sort_pairs(hk0, hv0, hk1, hv1, H, bd + hb, ws.get<uint32_t>(W_HIST, radix_hist_elems(H)),
-                         ws.get<uint32_t>(W_HIST_SCAN, scan_scratch_elems(radix_hist_elems(H))), s);
-        uint32_t *idx = ws.get<uint32_t>(W_HIDX, H);
-        uint64_t *S = ws.get<uint64_t>(W_HS, H);
-        heads_u64(hk0, H, idx, s);
-        scan_u32(idx, idx, H, true, ws.get<uint32_t>(W_SCAN32, scan_scratch_elems(H)), s);
-        widen_u32(hv0, (uint32_t)H, S, s);
-        scan_u64(S, S, H, true, ws.get<uint64_t>(W_SCAN64, scan_scratch_elems(H)), s);
-        uint32_t nu = 0;
-        LV_HIP(hipMemcpyAsync(&nu, idx + (H - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        LV_HIP(hipStreamSynchronize(s));
-        ukey = ws.get<uint64_t>(W_UKEY, nu);
-        wsum0 = ws.get<uint64_t>(W_WS, nu);
-        wsum1 = ws.get<uint64_t>(W_WE, nu);
-        agg_runs(hk0, hv0, idx, S, H, ukey, wsum0, wsum1, s);
-        ustart = ws.get<uint32_t>(W_USTART, (size_t)nh + 1);
-        agg_hub_rows(ukey, nu, bd, lp[kHubBucket], nh, ustart, x.rcount, s);
+        LV_HIP(hipMemcpyAsync(items_dev, items.data(), items.size() * sizeof(uint64_t),
+                              hipMemcpyHostToDevice, s));
+        LV_HIP(hipStreamSynchronize(s));   // the plan's host vectors end with this block
+        ga.info = info_dev;
+        ga.tiles = tiles_dev;
+        ga.n_tiles = (uint32_t)tiles.size();
+        ga.items = items_dev;
+        ga.n_items = (uint32_t)items.size();
+        ga.p0 = ws.get<unsigned long long>(W_GP0, off);
+        ga.p1 = ws.get<unsigned long long>(W_GP1, off);
+        ga.pcnt = ws.get<uint32_t>(W_GPCNT, pbase);
+        ga.poff = ws.get<uint32_t>(W_GPOFF, pbase);
+        ga.dcnt = ws.get<uint32_t>(W_GDCNT, pbase);
+        ga.doff = ws.get<uint32_t>(W_GDOFF, pbase);
+        ga.base = ws.get<uint32_t>(W_GBASE, mbase);
+        ga.scan_tmp = ws.get<uint32_t>(W_SCAN32, scan_scratch_elems(pbase));
+        n_parts = pbase;
+        fill_u32(ga.pcnt, 0u, pbase, s);
+        agg_giant(x, ga, n_parts, s);
     }
     // 5. new offsets, then every row copied to place
     uint64_t *orow = ng.neighboorhood_sum.data();
@@ -845,8 +866,8 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
     LV_HIP(hipStreamSynchronize(s));
     ng.edge_destination.ensure(std::max<size_t>(e_new, 1));
     ng.weights.ensure(std::max<size_t>(e_new, 1));
-    agg_copy(x, lp, cnt, hub_parts, orow, ng.edge_destination.data(), ng.weights.data(), ustart,
-             ukey, wsum0, wsum1, (uint32_t)((1ull << bd) - 1), s);
+    agg_copy(x, lp, cnt, orow, ng.edge_destination.data(), ng.weights.data(), s);
+    agg_giant_copy(x, ga, n_parts, orow, ng.edge_destination.data(), ng.weights.data(), s);
     new_degrees(c.communities_weight.data(), map, n, ng.tot_weight_per_nodes.data(), s);
     LV_HIP(hipStreamSynchronize(s));
     ng.n_entries = e_new;

@@ -91,9 +91,13 @@ struct SweepState {
     // hubs
     uint32_t n_hubs = 0;
     DeviceArray<uint32_t> hub_slot_of;
-    DeviceArray<uint64_t> hub_tab_off;
-    DeviceArray<uint32_t> hub_tab_log2;
-    DeviceArray<unsigned long long> hub_slot;
+    DeviceArray<uint64_t> hub_ent_off;
+    DeviceArray<uint32_t> hub_cbuf;
+    DeviceArray<unsigned long long> hub_pair;
+    DeviceArray<uint32_t> hub_cnt;
+    DeviceArray<uint32_t> hub_base;
+    DeviceArray<uint64_t> hub_items;
+    DeviceArray<uint32_t> hub_n_items;
     DeviceArray<int64_t> hub_part_s;
     DeviceArray<uint32_t> hub_part_c;
     DeviceArray<uint32_t> hub_self;

@@ -13,8 +13,8 @@
 //      community in an LDS table, then writes the distinct (dst, w) pairs to a
 //      staging area at crow[c].
 //   3. Giant communities (above 4096 virtual entries) gather their entries as
-//      keys (community << bits | dst), sort them with the LSD radix sort and
-//      reduce runs of equal keys.
+//      (dst, w) pairs, move them into destination hash partitions and sum
+//      each partition per destination in an LDS table.
 //   4. Row sizes are scanned into the new offsets and every row is copied to
 //      place.  Rows of LDS-table communities stay in table order: nothing
 //      downstream depends on the order of a row; exported graphs are sorted
@@ -160,35 +160,52 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
     }
 }
 
-struct AggHub {
+// ---- giant communities (above 4096 virtual entries) ----
+//
+// A giant community's destinations are cut into 2^lgp partitions by the top
+// bits of hub_mix (one per kGiantPartEntries entries, at most 4096) and its
+// virtual entries into tiles of 2^lgt (at least 16 entries per partition, so
+// a tile reserves at most one run per 16 entries).  k_giant_gather writes the
+// (destination, weight) pair of every entry in tile order and reserves each
+// (tile, partition) run; with poff = exclusive scan of the partition counts
+// (every entry is in one partition, so poff is the pair buffers' position of
+// a partition), k_giant_scatter moves the pairs to partition order;
+// k_giant_reduce sums one partition per workgroup in the LDS table
+// (lv_device.h) and writes its distinct pairs back over the tile-order buffer;
+// k_giant_copy puts them at the community's final row.
+
+constexpr int kGiantIlp = 4;
+constexpr uint32_t kGiantMaxParts = 1u << kGiantMaxPartBits;
+
+struct GiantCtx : GiantArgs {
     AggCtx a;
-    const uint64_t *hoff;      // [hub communities + 1] offsets of their entries
-    uint32_t parts;
-    int bits;                  // destination bits
-    uint64_t *hkey;            // (hub index << bits) | destination
-    uint32_t *hval;            // weight
 };
 
-// Gather the virtual entries of the giant communities as sort keys.
+__device__ inline uint32_t giant_part(uint32_t d, uint32_t lgp) {
+    return lgp ? hub_mix(d) >> (32 - lgp) : 0u;
+}
+
 template <bool WEIGHTED>
-__global__ __launch_bounds__(kBlock) void k_agg_hub_gather(AggHub h, const uint32_t *__restrict__ list,
-                                                           uint32_t count) {
-    const uint32_t ci = blockIdx.x / h.parts;
-    const uint32_t part = blockIdx.x % h.parts;
-    if (ci >= count) return;
-    const AggCtx &a = h.a;
-    const uint32_t c = list[ci];
-    const uint64_t k0 = a.crow[c], k1 = a.crow[c + 1];
-    const uint64_t span = (k1 - k0 + h.parts - 1) / h.parts;
-    const uint64_t s0 = k0 + span * part;
-    const uint64_t s1 = s0 + span < k1 ? s0 + span : k1;
-    const uint32_t m0 = a.mstart[c], m1 = a.mstart[c + 1];
-    const uint64_t out = h.hoff[ci] - k0;
-    const uint64_t hi_bits = (uint64_t)ci << h.bits;
+__global__ __launch_bounds__(kBlock) void k_giant_gather(GiantCtx h) {
+    __shared__ uint32_t hist[kGiantMaxParts];
     __shared__ uint64_t s_mrow[kBlock];
     __shared__ uint64_t s_mp[kBlock + 1];
     __shared__ uint32_t s_first;
-    if (threadIdx.x == 0) s_first = s0 < s1 ? member_of(a.P, m0, m1, s0) : m1;
+    const AggCtx &a = h.a;
+    const uint64_t tile = h.tiles[blockIdx.x];
+    const uint32_t lt = (uint32_t)tile;
+    const GiantInfo I = h.info[tile >> 32];
+    const uint32_t np = 1u << I.lgp;
+    for (uint32_t i = threadIdx.x; i < np; i += kBlock) hist[i] = 0u;
+    const uint64_t k0 = a.crow[I.c], k1 = a.crow[I.c + 1];
+    const uint64_t s0 = k0 + ((uint64_t)lt << I.lgt);
+    const uint64_t s1 = s0 + (1ull << I.lgt) < k1 ? s0 + (1ull << I.lgt) : k1;
+    const uint32_t m0 = a.mstart[I.c], m1 = a.mstart[I.c + 1];
+    unsigned long long *out = h.p0 + I.off;
+    if (threadIdx.x == 0) {
+        s_first = member_of(a.P, m0, m1, s0);
+        if (lt == 0) a.rcount[I.c] = 0u;   // the reduce adds the distinct counts
+    }
     __syncthreads();
     for (uint32_t mb = s_first; mb < m1; mb += kBlock) {
         const uint32_t nm = m1 - mb < (uint32_t)kBlock ? m1 - mb : (uint32_t)kBlock;
@@ -201,15 +218,125 @@ __global__ __launch_bounds__(kBlock) void k_agg_hub_gather(AggHub h, const uint3
         __syncthreads();
         const uint64_t e0 = s_mp[0] > s0 ? s_mp[0] : s0;
         const uint64_t e1 = s_mp[nm] < s1 ? s_mp[nm] : s1;
-        for (uint64_t k = e0 + threadIdx.x; k < e1; k += kBlock) {
-            const uint32_t i = lds_member(s_mp, nm, k);
-            const uint64_t j = s_mrow[i] + (k - s_mp[i]);
-            h.hkey[out + k] = hi_bits | a.newc[a.col[j]];
-            h.hval[out + k] = WEIGHTED ? a.wt[j] : 1u;
+        for (uint64_t kb = e0 + threadIdx.x; kb < e1; kb += (uint64_t)kBlock * kGiantIlp) {
+            uint32_t xs[kGiantIlp], ws[kGiantIlp], ds[kGiantIlp];
+#pragma unroll
+            for (int u = 0; u < kGiantIlp; ++u) {
+                const uint64_t k = kb + (uint64_t)u * kBlock;
+                xs[u] = LV_EMPTY;
+                ws[u] = 1u;
+                if (k < e1) {
+                    const uint32_t i = lds_member(s_mp, nm, k);
+                    const uint64_t j = s_mrow[i] + (k - s_mp[i]);
+                    xs[u] = stream_load(&a.col[j]);
+                    if (WEIGHTED) ws[u] = stream_load(&a.wt[j]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kGiantIlp; ++u) ds[u] = xs[u] != LV_EMPTY ? a.newc[xs[u]] : LV_EMPTY;
+#pragma unroll
+            for (int u = 0; u < kGiantIlp; ++u) {
+                if (ds[u] == LV_EMPTY) continue;
+                const uint64_t k = kb + (uint64_t)u * kBlock;
+                out[k - k0] = ((unsigned long long)ds[u] << 32) | ws[u];
+                atomicAdd(&hist[giant_part(ds[u], I.lgp)], 1u);
+            }
         }
         const bool done = s_mp[nm] >= s1;
         __syncthreads();
         if (done) break;
+    }
+    for (uint32_t p = threadIdx.x; p < np; p += kBlock) {
+        const uint32_t c = hist[p];
+        if (c) h.base[I.mbase + (uint64_t)lt * np + p] = atomicAdd(&h.pcnt[I.pbase + p], c);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_giant_scatter(GiantCtx h) {
+    __shared__ uint32_t s_off[kGiantMaxParts];
+    __shared__ uint32_t s_rank[kGiantMaxParts];
+    const uint64_t tile = h.tiles[blockIdx.x];
+    const uint32_t lt = (uint32_t)tile;
+    const GiantInfo I = h.info[tile >> 32];
+    const uint32_t np = 1u << I.lgp;
+    for (uint32_t p = threadIdx.x; p < np; p += kBlock) {
+        // a run start exists only for partitions this tile has entries in
+        s_off[p] = h.poff[I.pbase + p] + h.base[I.mbase + (uint64_t)lt * np + p];
+        s_rank[p] = 0u;
+    }
+    __syncthreads();
+    const uint64_t E = h.a.crow[I.c + 1] - h.a.crow[I.c];
+    const uint64_t t0 = (uint64_t)lt << I.lgt;
+    const uint64_t t1 = t0 + (1ull << I.lgt) < E ? t0 + (1ull << I.lgt) : E;
+    const unsigned long long *in = h.p0 + I.off;
+    for (uint64_t kb = t0 + threadIdx.x; kb < t1; kb += (uint64_t)kBlock * kGiantIlp) {
+        unsigned long long pr[kGiantIlp];
+#pragma unroll
+        for (int u = 0; u < kGiantIlp; ++u) {
+            const uint64_t k = kb + (uint64_t)u * kBlock;
+            pr[u] = k < t1 ? in[k] : ~0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < kGiantIlp; ++u) {
+            if (pr[u] == ~0ull) continue;
+            const uint32_t p = giant_part((uint32_t)(pr[u] >> 32), I.lgp);
+            h.p1[s_off[p] + atomicAdd(&s_rank[p], 1u)] = pr[u];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_giant_reduce(GiantCtx h) {
+    __shared__ SumTable t;
+    const uint64_t item = h.items[blockIdx.x];
+    const uint32_t p = (uint32_t)item;
+    const GiantInfo I = h.info[item >> 32];
+    sum_init(t);
+    const uint32_t beg = h.poff[I.pbase + p];
+    const uint32_t np = h.pcnt[I.pbase + p];
+    const unsigned long long *pair = h.p1 + beg;
+    unsigned long long *out = h.p0 + beg;
+    const uint64_t span_lo = (uint64_t)p << (32 - I.lgp);
+    const uint64_t span_hi = (uint64_t)(p + 1) << (32 - I.lgp);
+    uint64_t width = sum_first_width(span_lo, span_hi, np);
+    uint64_t lo = span_lo;
+    uint32_t n_out = 0;
+    __syncthreads();
+    while (np > 0 && lo < span_hi) {
+        const uint64_t rhi = span_hi - lo < width ? span_hi : lo + width;
+        const uint32_t nu = sum_piece<kGiantIlp>(t, pair, np, lo, rhi);
+        if (nu == 0xFFFFFFFFu) {   // too many destinations: halve the piece
+            width = (rhi - lo) / 2 > 0 ? (rhi - lo) / 2 : 1u;
+            continue;
+        }
+        for (uint32_t i = threadIdx.x; i < nu; i += kBlock) {
+            const uint32_t s = t.used[i];
+            out[n_out + i] = ((unsigned long long)t.keys[s] << 32) | t.vals[s];
+            t.keys[s] = LV_EMPTY;
+            t.vals[s] = 0u;
+        }
+        sum_next_piece(t);
+        n_out += nu;
+        lo = rhi;
+        if (nu < kSumTarget / 2 && width < span_hi - span_lo) width *= 2;   // sparse: widen again
+    }
+    if (threadIdx.x == 0) {
+        h.dcnt[I.pbase + p] = n_out;
+        if (n_out) atomicAdd(&h.a.rcount[I.c], n_out);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_giant_copy(GiantCtx h, const uint64_t *out_row,
+                                                       uint32_t *col, uint32_t *w) {
+    const uint64_t item = h.items[blockIdx.x];
+    const uint32_t p = (uint32_t)item;
+    const GiantInfo I = h.info[item >> 32];
+    const unsigned long long *in = h.p0 + h.poff[I.pbase + p];
+    const uint64_t dst = out_row[I.c] + (h.doff[I.pbase + p] - h.doff[I.pbase]);
+    const uint32_t n = h.dcnt[I.pbase + p];
+    for (uint32_t t = threadIdx.x; t < n; t += kBlock) {
+        const unsigned long long pr = in[t];
+        col[dst + t] = (uint32_t)(pr >> 32);
+        w[dst + t] = (uint32_t)pr;
     }
 }
 
@@ -235,30 +362,12 @@ __global__ void k_runs(const uint64_t *keys, const uint32_t *vals, const uint32_
     }
 }
 
-// Start of each hub community's row among the unique keys.
-__global__ void k_hub_starts(const uint64_t *ukey, uint64_t nu, int bits, uint32_t *ustart) {
-    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < nu;
-         u += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t hc = (uint32_t)(ukey[u] >> bits);
-        if (u == 0 || (uint32_t)(ukey[u - 1] >> bits) != hc) ustart[hc] = (uint32_t)u;
-    }
-}
-
-__global__ void k_hub_counts(const uint32_t *ustart, const uint32_t *list, uint32_t count,
-                             uint32_t *rcount) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
-        rcount[list[i]] = ustart[i + 1] - ustart[i];
-}
-
-// Copy every community's sorted row to its final offset.  Table-bucket rows
-// come from the staging area; giant communities from the unique hub keys.
-template <int G, bool HUB>
+// Copy the staged row of every LDS-table community to its final offset.
+template <int G>
 __global__ __launch_bounds__(kBlock) void k_agg_copy(AggCtx a, const uint32_t *__restrict__ list,
                                                      uint32_t count, uint32_t parts,
                                                      const uint64_t *out_row, uint32_t *col,
-                                                     uint32_t *w, const uint32_t *ustart,
-                                                     const uint64_t *ukey, const uint64_t *ws,
-                                                     const uint64_t *we, uint32_t mask) {
+                                                     uint32_t *w) {
     constexpr int GPB = kBlock / G;
     const int lane = threadIdx.x & (G - 1);
     uint32_t idx, part;
@@ -274,18 +383,10 @@ __global__ __launch_bounds__(kBlock) void k_agg_copy(AggCtx a, const uint32_t *_
     const uint32_t r = a.rcount[c];
     const uint64_t dst = out_row[c];
     const uint32_t stride = G * (G == kBlock ? parts : 1u);
-    if constexpr (HUB) {
-        const uint64_t src = ustart[idx];
-        for (uint32_t t = part * G + lane; t < r; t += stride) {
-            col[dst + t] = (uint32_t)ukey[src + t] & mask;
-            w[dst + t] = (uint32_t)(we[src + t] - ws[src + t]);
-        }
-    } else {
-        const uint64_t src = a.crow[c];
-        for (uint32_t t = part * G + lane; t < r; t += stride) {
-            col[dst + t] = a.tcol[src + t];
-            w[dst + t] = a.tw[src + t];
-        }
+    const uint64_t src = a.crow[c];
+    for (uint32_t t = part * G + lane; t < r; t += stride) {
+        col[dst + t] = a.tcol[src + t];
+        w[dst + t] = a.tw[src + t];
     }
 }
 
@@ -395,19 +496,30 @@ void agg_tables(const AggArgs &x, uint32_t *const *lists, const uint32_t *counts
     }
 }
 
-void agg_hub_gather(const AggArgs &x, const uint32_t *hub_list, uint32_t nh, const uint64_t *hoff,
-                    uint32_t parts, int bits, uint64_t *hkey, uint32_t *hval, hipStream_t s) {
-    if (!nh) return;
-    AggHub h;
+static GiantCtx giant_ctx(const AggArgs &x, const GiantArgs &g) {
+    GiantCtx h;
+    static_cast<GiantArgs &>(h) = g;
     h.a = ctx_of(x);
-    h.hoff = hoff;
-    h.parts = parts;
-    h.bits = bits;
-    h.hkey = hkey;
-    h.hval = hval;
-    const dim3 grid(nh * parts);
-    if (x.g.wt) hipLaunchKernelGGL(k_agg_hub_gather<true>, grid, dim3(kBlock), 0, s, h, hub_list, nh);
-    else hipLaunchKernelGGL(k_agg_hub_gather<false>, grid, dim3(kBlock), 0, s, h, hub_list, nh);
+    return h;
+}
+
+void agg_giant(const AggArgs &x, const GiantArgs &g, uint32_t n_parts, hipStream_t s) {
+    if (!g.n_tiles) return;
+    const GiantCtx h = giant_ctx(x, g);
+    if (x.g.wt) hipLaunchKernelGGL(k_giant_gather<true>, dim3(g.n_tiles), dim3(kBlock), 0, s, h);
+    else hipLaunchKernelGGL(k_giant_gather<false>, dim3(g.n_tiles), dim3(kBlock), 0, s, h);
+    scan_u32(g.pcnt, g.poff, n_parts, false, g.scan_tmp, s);
+    hipLaunchKernelGGL(k_giant_scatter, dim3(g.n_tiles), dim3(kBlock), 0, s, h);
+    hipLaunchKernelGGL(k_giant_reduce, dim3(g.n_items), dim3(kBlock), 0, s, h);
+    scan_u32(g.dcnt, g.doff, n_parts, false, g.scan_tmp, s);
+}
+
+void agg_giant_copy(const AggArgs &x, const GiantArgs &g, uint32_t n_parts, const uint64_t *out_row,
+                    uint32_t *col, uint32_t *w, hipStream_t s) {
+    (void)n_parts;
+    if (!g.n_items) return;
+    hipLaunchKernelGGL(k_giant_copy, dim3(g.n_items), dim3(kBlock), 0, s, giant_ctx(x, g), out_row,
+                       col, w);
 }
 
 void heads_u64(const uint64_t *keys, uint64_t n, uint32_t *heads, hipStream_t s) {
@@ -422,42 +534,29 @@ void agg_runs(const uint64_t *keys, const uint32_t *vals, const uint32_t *idx, c
                        n, ukey, ws, we);
 }
 
-void agg_hub_rows(const uint64_t *ukey, uint64_t nu, int bits, const uint32_t *hub_list,
-                  uint32_t nh, uint32_t *ustart, uint32_t *rcount, hipStream_t s) {
-    if (!nh) return;
-    hipLaunchKernelGGL(k_hub_starts, dim3(grid_for(nu, kBlock * 4)), dim3(kBlock), 0, s, ukey, nu,
-                       bits, ustart);
-    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, ustart + nh, (uint32_t)nu);
-    hipLaunchKernelGGL(k_hub_counts, dim3(grid_for(nh, kBlock)), dim3(kBlock), 0, s,
-                       (const uint32_t *)ustart, hub_list, nh, rcount);
-}
-
-void agg_copy(const AggArgs &x, uint32_t *const *lists, const uint32_t *counts, uint32_t hub_parts,
-              const uint64_t *out_row, uint32_t *col, uint32_t *w, const uint32_t *ustart,
-              const uint64_t *ukey, const uint64_t *ws, const uint64_t *we, uint32_t mask,
-              hipStream_t s) {
+void agg_copy(const AggArgs &x, uint32_t *const *lists, const uint32_t *counts,
+              const uint64_t *out_row, uint32_t *col, uint32_t *w, hipStream_t s) {
     const AggCtx a = ctx_of(x);
-    for (int b = 0; b < kNumBuckets; ++b) {
+    for (int b = 0; b < kHubBucket; ++b) {
         const uint32_t cnt = counts[b];
         if (!cnt) continue;
         const uint32_t *L = lists[b];
-#define LV_CP(GG, PARTS, HUB)                                                                   \
+#define LV_CP(GG, PARTS)                                                                        \
     do {                                                                                        \
         const uint32_t gpb = kBlock / (GG);                                                     \
         const dim3 grid((GG) == kBlock ? cnt * (PARTS) : (cnt + gpb - 1) / gpb);                \
-        hipLaunchKernelGGL((k_agg_copy<GG, HUB>), grid, dim3(kBlock), 0, s, a, L, cnt,          \
-                           (uint32_t)(PARTS), out_row, col, w, ustart, ukey, ws, we, mask);     \
+        hipLaunchKernelGGL((k_agg_copy<GG>), grid, dim3(kBlock), 0, s, a, L, cnt,               \
+                           (uint32_t)(PARTS), out_row, col, w);                                 \
     } while (0)
         switch (b) {
-            case 0: LV_CP(4, 1, false); break;
-            case 1: LV_CP(8, 1, false); break;
-            case 2: LV_CP(16, 1, false); break;
-            case 3: LV_CP(32, 1, false); break;
-            case 4: LV_CP(64, 1, false); break;
-            case 5: LV_CP(64, 1, false); break;
-            case 6: LV_CP(256, 1, false); break;
-            case 7: LV_CP(256, 4, false); break;
-            default: LV_CP(256, hub_parts, true); break;
+            case 0: LV_CP(4, 1); break;
+            case 1: LV_CP(8, 1); break;
+            case 2: LV_CP(16, 1); break;
+            case 3: LV_CP(32, 1); break;
+            case 4: LV_CP(64, 1); break;
+            case 5: LV_CP(64, 1); break;
+            case 6: LV_CP(256, 1); break;
+            default: LV_CP(256, 4); break;
         }
 #undef LV_CP
     }

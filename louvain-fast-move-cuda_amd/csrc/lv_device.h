@@ -135,6 +135,112 @@ __device__ inline uint32_t slot_hash_dyn(uint32_t c, uint32_t log2t) {
 
 constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
 
+// ---- LDS sums of (key << 32 | weight) pairs, range by range of a key mix ----
+//
+// Rows too long for one LDS table (hub rows of the best-move pass, giant
+// communities of the contraction) are cut into partitions by the top bits of
+// hub_mix(key); a workgroup then sums one partition's pairs per key in an LDS
+// table.  A partition may hold more keys than the table: its mix range is then
+// done in pieces, each piece inserted, scanned and cleared in turn, and a
+// piece that overflows is halved and redone.  hub_mix is a bijection, so a
+// piece of one mix value holds one key and the halving ends.
+
+constexpr int kSumLds = 4096;          // table slots
+constexpr uint32_t kSumFill = 3072;    // keys held before a piece counts as overflowing
+constexpr uint32_t kSumTarget = 2048;  // expected keys per piece
+
+// murmur3's 32-bit finaliser (a bijection of the 32-bit keys).
+__device__ inline uint32_t hub_mix(uint32_t c) {
+    c ^= c >> 16;
+    c *= 0x85ebca6bu;
+    c ^= c >> 13;
+    c *= 0xc2b2ae35u;
+    c ^= c >> 16;
+    return c;
+}
+
+struct SumTable {
+    uint32_t keys[kSumLds];    // LV_EMPTY when free
+    uint32_t vals[kSumLds];
+    uint16_t used[kSumFill];   // claimed slots in claim order
+    uint32_t n_used;
+};
+
+// Workgroup-wide: empty table (call once, then the table is left empty by
+// every sum_piece / sum_clear_used).
+__device__ inline void sum_init(SumTable &t) {
+    for (int i = threadIdx.x; i < kSumLds; i += blockDim.x) { t.keys[i] = LV_EMPTY; t.vals[i] = 0u; }
+    if (threadIdx.x == 0) t.n_used = 0;
+}
+
+// Workgroup-wide: adds the pairs whose key mixes into [lo, hi) to the table.
+// Returns the number of distinct keys (uniform), or 0xFFFFFFFF after an
+// overflow, in which case the table is empty again.
+template <int ILP>
+__device__ inline uint32_t sum_piece(SumTable &t, const unsigned long long *__restrict__ pair,
+                                     uint32_t np, uint64_t lo, uint64_t hi) {
+    for (uint32_t ib = threadIdx.x; ib < np; ib += blockDim.x * ILP) {
+        // an overflowing piece is redone: stop reading it
+        if (__hip_atomic_load(&t.n_used, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= kSumFill)
+            break;
+        unsigned long long pr[ILP];
+#pragma unroll
+        for (int u = 0; u < ILP; ++u) {
+            const uint32_t i = ib + (uint32_t)u * blockDim.x;
+            pr[u] = i < np ? pair[i] : ~0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < ILP; ++u) {
+            const uint32_t c = (uint32_t)(pr[u] >> 32);
+            if (c == LV_EMPTY) continue;
+            const uint32_t h = hub_mix(c);
+            if (h < lo || h >= hi) continue;
+            const uint32_t w = (uint32_t)pr[u];
+            // at most one new key per lane once the piece is full, so the
+            // table never fills and every probe ends
+            if (__hip_atomic_load(&t.n_used, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= kSumFill)
+                continue;
+            uint32_t s = h & (kSumLds - 1);
+            for (;;) {
+                const uint32_t prev = atomicCAS(&t.keys[s], LV_EMPTY, c);
+                if (prev == LV_EMPTY) {
+                    atomicAdd(&t.vals[s], w);
+                    const uint32_t k = atomicAdd(&t.n_used, 1u);
+                    if (k < kSumFill) t.used[k] = (uint16_t)s;
+                    break;
+                }
+                if (prev == c) { atomicAdd(&t.vals[s], w); break; }
+                s = (s + 1) & (kSumLds - 1);
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t nu = t.n_used;
+    if (nu < kSumFill) return nu;
+    for (int i = threadIdx.x; i < kSumLds; i += blockDim.x) { t.keys[i] = LV_EMPTY; t.vals[i] = 0u; }
+    __syncthreads();
+    if (threadIdx.x == 0) t.n_used = 0;
+    __syncthreads();
+    return 0xFFFFFFFFu;
+}
+
+// Workgroup-wide, after the caller has read the piece's nu used slots (and
+// cleared them): resets the claim count for the next piece.
+__device__ inline void sum_next_piece(SumTable &t) {
+    __syncthreads();
+    if (threadIdx.x == 0) t.n_used = 0;
+    __syncthreads();
+}
+
+// Width of the first piece of [span_lo, span_hi) for np pairs: as if the
+// pairs were distinct, but at most 4 pieces, since many pairs may share a key
+// (a piece that overflows is halved, one that stays sparse lets the next grow).
+__device__ inline uint64_t sum_first_width(uint64_t span_lo, uint64_t span_hi, uint32_t np) {
+    uint64_t ranges = 1;
+    while (ranges < 4 && ranges < span_hi - span_lo && ranges * kSumTarget < np) ranges <<= 1;
+    return (span_hi - span_lo) / ranges;
+}
+
 template <int GAIN>
 struct Score;
 template <>

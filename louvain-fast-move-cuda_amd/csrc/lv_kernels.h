@@ -20,9 +20,8 @@ namespace lv {
 constexpr int kNumBuckets = 9;        // 8 LDS-table buckets + global-table hubs
 constexpr int kHubBucket = 8;
 constexpr int kStatSlots = 64;        // spread of the per-block statistic atomics
-constexpr int kHubParts = 32;         // workgroups per hub vertex
+constexpr int kHubParts = 32;         // row chunks and community partitions per hub vertex
 constexpr uint32_t kMaxChunks = 4096; // reference chunks per level (STEP_ROUND emulation)
-constexpr unsigned long long kHubEmpty = 0xFFFFFFFF00000000ull;   // empty packed slot
 
 // Maximum degree handled by each LDS bucket; larger goes to the hub bucket.
 constexpr uint32_t kBucketMaxDeg[kNumBuckets - 1] = {4, 8, 16, 32, 64, 256, 1024, 4096};
@@ -59,14 +58,20 @@ struct LevelView {
     uint64_t E;
 };
 
+// Hub rows (> 4096 entries) are reduced by community hash partition: hub h
+// owns entries [ent_off[h], ent_off[h] + degree) of cbuf and pair.
 struct HubTables {
     uint32_t *slot_of;        // [n] hub slot of a vertex (LV_EMPTY if not a hub)
-    uint64_t *tab_off;        // [n_hubs] offset of the hub's table
-    uint32_t *tab_log2;       // [n_hubs] log2 of its table size
-    unsigned long long *slot; // pooled tables: (community << 32) | weight, kept kHubEmpty between uses
+    uint64_t *ent_off;        // [n_hubs] offset of the hub's entries in cbuf / pair
+    uint32_t *cbuf;           // [hub entries] community of each entry (LV_EMPTY: self-loop)
+    unsigned long long *pair; // [hub entries] (community << 32) | weight in partition order
+    uint32_t *cnt;            // [n_hubs * kHubParts] entries per partition, zero between sweeps
+    uint32_t *base;           // [n_hubs * kHubParts * kHubParts] start of (chunk, partition)
+    uint64_t *items;          // [n_hubs * kHubParts] reduce work: vertex << 32 | slot << 8 | log2 groups << 5 | group
+    uint32_t *n_items;        // [1] items posted this sweep, zero between sweeps
     int64_t *part_s;          // [n_hubs * kHubParts] partial best scores
     uint32_t *part_c;         // [n_hubs * kHubParts] partial best communities
-    uint32_t *self;           // [n_hubs] weight to own community
+    uint32_t *self;           // [n_hubs] weight to own community, zero between sweeps
 };
 
 // Several graphs swept as one disjoint union: the graph id of a vertex selects
@@ -198,19 +203,48 @@ void agg_comm_rows(const uint64_t *P, const uint32_t *mstart, uint32_t n_new, ui
                    hipStream_t s);
 // LDS-table communities (buckets 0-7): sorted distinct rows into the staging area
 void agg_tables(const AggArgs &x, uint32_t *const *lists, const uint32_t *counts, hipStream_t s);
-// giant communities: their entries as keys (hub index << bits | destination)
-void agg_hub_gather(const AggArgs &x, const uint32_t *hub_list, uint32_t nh, const uint64_t *hoff,
-                    uint32_t parts, int bits, uint64_t *hkey, uint32_t *hval, hipStream_t s);
+// Giant communities (above 4096 virtual entries): 2^lgp destination hash
+// partitions (one per kGiantPartEntries entries, at most 2^kGiantMaxPartBits)
+// for the LDS reduce, and tiles of 2^lgt entries (at least 16 per partition)
+// for the gather and scatter.
+constexpr uint64_t kGiantPartEntries = 4096;
+constexpr int kGiantMaxPartBits = 12;
+constexpr int kGiantMinTileBits = 12;
+struct GiantInfo {
+    uint64_t off;     // first entry of the community in the pair buffers
+    uint64_t mbase;   // first of its (tile, partition) run starts
+    uint32_t c;       // community id
+    uint32_t pbase;   // first of its partition counters
+    uint32_t lgp;     // log2 of its partition count
+    uint32_t lgt;     // log2 of its tile size
+};
+struct GiantArgs {
+    const GiantInfo *info;          // [giants]
+    const uint64_t *tiles;          // [n_tiles] giant index << 32 | tile
+    uint32_t n_tiles;
+    const uint64_t *items;          // [n_items] giant index << 32 | partition
+    uint32_t n_items;
+    unsigned long long *p0;         // [entries] pairs in tile order, then distinct pairs
+    unsigned long long *p1;         // [entries] pairs in partition order
+    uint32_t *pcnt;                 // [partitions] entries per partition, zeroed
+    uint32_t *poff;                 // [partitions] exclusive scan of pcnt: a partition's pairs
+    uint32_t *dcnt;                 // [partitions] distinct destinations per partition
+    uint32_t *doff;                 // [partitions] exclusive scan of dcnt
+    uint32_t *base;                 // [(tile, partition) runs] start of a tile's run
+    uint32_t *scan_tmp;             // scan scratch for the partition counts
+};
+// distinct (destination, weight) pairs of every giant community and its
+// rcount; then, once the new offsets are known, its row copied to place
+void agg_giant(const AggArgs &x, const GiantArgs &g, uint32_t n_parts, hipStream_t s);
+void agg_giant_copy(const AggArgs &x, const GiantArgs &g, uint32_t n_parts, const uint64_t *out_row,
+                    uint32_t *col, uint32_t *w, hipStream_t s);
 void heads_u64(const uint64_t *keys, uint64_t n, uint32_t *heads, hipStream_t s);
 // runs of sorted keys: idx = inclusive scan of heads, S = inclusive scan of weights
 void agg_runs(const uint64_t *keys, const uint32_t *vals, const uint32_t *idx, const uint64_t *S,
               uint64_t n, uint64_t *ukey, uint64_t *ws, uint64_t *we, hipStream_t s);
-void agg_hub_rows(const uint64_t *ukey, uint64_t nu, int bits, const uint32_t *hub_list,
-                  uint32_t nh, uint32_t *ustart, uint32_t *rcount, hipStream_t s);
-void agg_copy(const AggArgs &x, uint32_t *const *lists, const uint32_t *counts, uint32_t hub_parts,
-              const uint64_t *out_row, uint32_t *col, uint32_t *w, const uint32_t *ustart,
-              const uint64_t *ukey, const uint64_t *ws, const uint64_t *we, uint32_t mask,
-              hipStream_t s);
+// rows of the LDS-table communities copied to their final offsets
+void agg_copy(const AggArgs &x, uint32_t *const *lists, const uint32_t *counts,
+              const uint64_t *out_row, uint32_t *col, uint32_t *w, hipStream_t s);
 void set_u64(uint64_t *p, uint64_t v, hipStream_t s);
 void widen_u32(const uint32_t *in, uint32_t n, uint64_t *out, hipStream_t s);
 // per-bucket sizes of a degree array given by offsets row[n+1]

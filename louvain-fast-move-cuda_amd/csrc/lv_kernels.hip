@@ -7,7 +7,8 @@
 // weights towards each neighbour community in an LDS open-addressing table
 // (2x the degree, so it never overflows), looks up its own community's weight,
 // scores every candidate and reduces to the best one.  Vertices above 4096
-// entries use pooled global-memory tables shared by 32 workgroups.  The first
+// entries are cut into 32 chunks and 32 community hash partitions, each
+// partition summed in an LDS table by a workgroup of its own.  The first
 // sweep of every level runs at the identity partition and needs no table.
 //
 // Commit pass (phaseOptimizationUpdate.cu:14-132): movers update the exact
@@ -24,6 +25,7 @@
 #include "../../include/constants.h"
 #include "../../include/operators.h"
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 
@@ -615,172 +617,271 @@ __global__ __launch_bounds__(kBlock) void k_move_fast(MoveCtx a, const uint32_t 
                     ST_TWRITES, twrite);
 }
 
-// ---- hubs: kHubParts workgroups per vertex, pooled global tables ----
+// ---- hubs: kHubParts workgroups per vertex, partitioned by community hash ----
+//
+// A hub row is cut into kHubParts contiguous chunks (one workgroup each) and
+// its neighbour communities into kHubParts partitions by the top bits of a
+// bijective mix of the community id.  k_hub_count gathers the community of
+// every entry into the hub's c-buffer, sums the weight towards the hub's own
+// community and reserves each (chunk, partition) run; k_hub_scatter packs the
+// (community, weight) pairs in partition order; k_hub_reduce gives every
+// partition to one workgroup, which sums the weights per community in an LDS
+// table and scores them.  Every entry costs streaming reads and writes and
+// LDS atomics only; hub rows see almost only distinct communities, so a
+// global table would take one random atomic per entry.
 
 struct HubCtx {
     MoveCtx m;
     HubTables h;
 };
 
+constexpr int kHubPartBits = ilog2(kHubParts);
+constexpr int kHubSubBits = 32 - kHubPartBits;            // mix bits below the partition
+constexpr uint64_t kHubGroupEntries = 4096;                // entries per reduce group
+constexpr uint32_t kHubReduceBlocks = 2048;                // persistent grid of k_hub_reduce
+constexpr int kHubIlp = 16;                                // pair loads / tot gathers in flight per lane
+static_assert(kHubParts == 32, "items pack the group in 5 bits");
+
+__device__ inline uint32_t ilog2_u32(uint32_t x) { return 31u - (uint32_t)__builtin_clz(x); }
+
+__device__ inline uint32_t hub_groups(uint64_t d) {
+    uint32_t g = 1;
+    while (g < (uint32_t)kHubParts && (uint64_t)g * kHubGroupEntries < d) g <<= 1;
+    return g;
+}
+
+__device__ inline void hub_chunk(uint64_t beg, uint64_t end, uint32_t q, uint64_t &cb,
+                                 uint64_t &ce) {
+    const uint64_t d = end - beg;
+    const uint64_t per = (d + kHubParts - 1) / kHubParts;
+    cb = beg + min(d, per * q);
+    ce = beg + min(d, per * (q + 1));
+}
+
+// The hub of workgroup block/kHubParts when it sweeps: listed, in range and of
+// a live graph.
+__device__ inline bool hub_runs(const HubCtx &a, const uint32_t *list, uint32_t count,
+                                uint32_t hi, uint32_t &v, int64_t &m2, double &tw) {
+    if (hi >= count) return false;
+    v = list[hi];
+    return v < a.m.v_end && vertex_graph(a.m.bt, v, a.m.m2, a.m.tw, m2, tw);
+}
+
 template <bool WEIGHTED>
-__global__ __launch_bounds__(kBlock) void k_hub_insert(HubCtx a, const uint32_t *__restrict__ list,
-                                                       uint32_t count) {
-    const uint32_t hi = blockIdx.x / kHubParts;
-    const uint32_t part = blockIdx.x % kHubParts;
-    uint64_t nself = 0;
-    if (hi < count) {
-        const uint32_t v = list[hi];
-        int64_t m2;
-        double tw;
-        if (v < a.m.v_end && vertex_graph(a.m.bt, v, a.m.m2, a.m.tw, m2, tw)) {
-            const uint32_t slot = a.h.slot_of[v];
-            const uint64_t off = a.h.tab_off[slot];
-            const uint32_t lg = a.h.tab_log2[slot];
-            const uint32_t mask = (1u << lg) - 1u;
-            unsigned long long *S = a.h.slot + off;
-            const uint64_t beg = a.m.row[v], end = a.m.row[v + 1];
-            const uint64_t stride = (uint64_t)kHubParts * kBlock;
-            for (uint64_t jb = beg + (uint64_t)part * kBlock + threadIdx.x; jb < end;
-                 jb += stride * kIlp) {
-                uint32_t xs[kIlp], ws[kIlp], cs[kIlp];
+__global__ __launch_bounds__(kBlock) void k_hub_count(HubCtx a, const uint32_t *__restrict__ list,
+                                                      uint32_t count) {
+    __shared__ uint32_t hist[kHubParts];
+    __shared__ uint64_t r_self[kBlock / 64];
+    const uint32_t q = blockIdx.x % kHubParts;
+    if (threadIdx.x < kHubParts) hist[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t v = 0, slot = 0;
+    int64_t m2;
+    double tw;
+    const bool run = hub_runs(a, list, count, blockIdx.x / kHubParts, v, m2, tw);
+    uint64_t nself = 0, selfw = 0;
+    if (run) {
+        slot = a.h.slot_of[v];
+        const uint32_t own = a.m.comm[v];
+        const uint64_t beg = a.m.row[v], end = a.m.row[v + 1];
+        uint32_t *cbuf = a.h.cbuf + a.h.ent_off[slot];
+        uint64_t cb, ce;
+        hub_chunk(beg, end, q, cb, ce);
+        for (uint64_t jb = cb + threadIdx.x; jb < ce; jb += (uint64_t)kBlock * kIlp) {
+            uint32_t xs[kIlp], cs[kIlp];
 #pragma unroll
-                for (int u = 0; u < kIlp; ++u) {
-                    const uint64_t j = jb + (uint64_t)u * stride;
-                    xs[u] = j < end ? stream_load(&a.m.col[j]) : v;
-                    ws[u] = (WEIGHTED && j < end) ? stream_load(&a.m.wt[j]) : 1u;
-                }
+            for (int u = 0; u < kIlp; ++u) {
+                const uint64_t j = jb + (uint64_t)u * kBlock;
+                xs[u] = j < ce ? stream_load(&a.m.col[j]) : v;
+            }
 #pragma unroll
-                for (int u = 0; u < kIlp; ++u) cs[u] = xs[u] != v ? a.m.comm[xs[u]] : 0u;
+            for (int u = 0; u < kIlp; ++u) cs[u] = xs[u] != v ? a.m.comm[xs[u]] : LV_EMPTY;
 #pragma unroll
-                for (int u = 0; u < kIlp; ++u) {
-                    if (xs[u] == v) continue;
-                    ++nself;
-                    const uint32_t c = cs[u];
-                    const uint32_t w = ws[u];
-                    // one atomic per entry: add to the packed slot holding c, or
-                    // claim an empty slot with (c << 32 | w)
-                    uint32_t h = slot_hash_dyn(c, lg);
-                    for (;;) {
-                        const unsigned long long cur =
-                            __hip_atomic_load(&S[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        const uint32_t kk = (uint32_t)(cur >> 32);
-                        if (kk == c) { atomicAdd(&S[h], (unsigned long long)w); break; }
-                        if (kk == LV_EMPTY) {
-                            const unsigned long long prev =
-                                atomicCAS(&S[h], kHubEmpty, ((unsigned long long)c << 32) | w);
-                            if (prev == kHubEmpty) break;
-                            if ((uint32_t)(prev >> 32) == c) {
-                                atomicAdd(&S[h], (unsigned long long)w);
-                                break;
-                            }
-                        }
-                        h = (h + 1) & mask;
-                    }
-                }
+            for (int u = 0; u < kIlp; ++u) {
+                const uint64_t j = jb + (uint64_t)u * kBlock;
+                if (j >= ce) continue;
+                cbuf[j - beg] = cs[u];
+                if (xs[u] == v) continue;
+                ++nself;
+                if (cs[u] == own) selfw += WEIGHTED ? a.m.wt[j] : 1u;
+                else atomicAdd(&hist[hub_mix(cs[u]) >> kHubSubBits], 1u);
+            }
+        }
+    }
+    const uint64_t wsum = group_sum_u64<64>(selfw);
+    if ((threadIdx.x & 63) == 0) r_self[threadIdx.x >> 6] = wsum;
+    __syncthreads();
+    if (run) {
+        if (threadIdx.x < kHubParts) {
+            const uint32_t h = hist[threadIdx.x];
+            if (h)
+                a.h.base[((size_t)slot * kHubParts + q) * kHubParts + threadIdx.x] =
+                    atomicAdd(&a.h.cnt[(size_t)slot * kHubParts + threadIdx.x], h);
+        }
+        if (threadIdx.x == 0) {
+            uint64_t t = 0;
+            for (int i = 0; i < kBlock / 64; ++i) t += r_self[i];
+            if (t) atomicAdd(&a.h.self[slot], (uint32_t)t);
+            if (q == 0) {   // post the hub's reduce groups
+                const uint32_t groups = hub_groups(a.m.row[v + 1] - a.m.row[v]);
+                const uint32_t b0 = atomicAdd(a.h.n_items, groups);
+                const uint64_t head = ((uint64_t)v << 32) | ((uint64_t)slot << 8) |
+                                      ((uint64_t)ilog2_u32(groups) << kHubPartBits);
+                for (uint32_t g = 0; g < groups; ++g) a.h.items[b0 + g] = head | g;
             }
         }
     }
     block_add_stats(a.m.stats, ST_E_ACTIVE, nself, -1, 0, -1, 0);
 }
 
-__global__ void k_hub_self(HubCtx a, const uint32_t *__restrict__ list, uint32_t count) {
-    const uint32_t hi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (hi >= count) return;
-    const uint32_t v = list[hi];
+template <bool WEIGHTED>
+__global__ __launch_bounds__(kBlock) void k_hub_scatter(HubCtx a,
+                                                        const uint32_t *__restrict__ list,
+                                                        uint32_t count) {
+    __shared__ uint32_t s_off[kHubParts];
+    __shared__ uint32_t s_rank[kHubParts];
+    const uint32_t q = blockIdx.x % kHubParts;
+    uint32_t v = 0;
+    int64_t m2;
+    double tw;
+    const bool run = hub_runs(a, list, count, blockIdx.x / kHubParts, v, m2, tw);
+    if (!run) return;   // uniform over the workgroup
     const uint32_t slot = a.h.slot_of[v];
-    uint32_t self = 0;
-    if (v < a.m.v_end) {
-        const uint32_t own = a.m.comm[v];
-        const uint64_t off = a.h.tab_off[slot];
-        const uint32_t lg = a.h.tab_log2[slot];
-        const uint32_t mask = (1u << lg) - 1u;
-        uint32_t h = slot_hash_dyn(own, lg);
-        for (;;) {
-            const unsigned long long cur = a.h.slot[off + h];
-            const uint32_t c = (uint32_t)(cur >> 32);
-            if (c == own) { self = (uint32_t)cur; break; }
-            if (c == LV_EMPTY) break;
-            h = (h + 1) & mask;
+    if (threadIdx.x < 64) {
+        // start of each partition in the hub's pairs, plus this chunk's run
+        const int lane = threadIdx.x;
+        uint32_t c = lane < kHubParts ? a.h.cnt[(size_t)slot * kHubParts + lane] : 0u;
+        uint32_t incl = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)incl, off);
+            if (lane >= off) incl += y;
+        }
+        if (lane < kHubParts) {
+            // base is only written for runs this chunk has; others stay unused
+            s_off[lane] = (incl - c) +
+                          a.h.base[((size_t)slot * kHubParts + q) * kHubParts + lane];
+            s_rank[lane] = 0;
         }
     }
-    a.h.self[slot] = self;
+    __syncthreads();
+    const uint32_t own = a.m.comm[v];
+    const uint64_t beg = a.m.row[v], end = a.m.row[v + 1];
+    const uint64_t eo = a.h.ent_off[slot];
+    const uint32_t *cbuf = a.h.cbuf + eo;
+    unsigned long long *pair = a.h.pair + eo;
+    uint64_t cb, ce;
+    hub_chunk(beg, end, q, cb, ce);
+    for (uint64_t jb = cb + threadIdx.x; jb < ce; jb += (uint64_t)kBlock * kIlp) {
+        uint32_t cs[kIlp], ws[kIlp];
+#pragma unroll
+        for (int u = 0; u < kIlp; ++u) {
+            const uint64_t j = jb + (uint64_t)u * kBlock;
+            cs[u] = j < ce ? __builtin_nontemporal_load(&cbuf[j - beg]) : LV_EMPTY;
+            ws[u] = (WEIGHTED && j < ce) ? stream_load(&a.m.wt[j]) : 1u;
+        }
+#pragma unroll
+        for (int u = 0; u < kIlp; ++u) {
+            if (cs[u] == LV_EMPTY || cs[u] == own) continue;
+            const uint32_t p = hub_mix(cs[u]) >> kHubSubBits;
+            const uint32_t r = atomicAdd(&s_rank[p], 1u);
+            pair[s_off[p] + r] = ((unsigned long long)cs[u] << 32) | ws[u];
+        }
+    }
 }
 
-// Scan one part of the hub's table; record the part's best candidate; clear
-// the scanned slots so the pool stays empty between sweeps.
+// Persistent over the posted (hub, group) items: a group's pairs are summed
+// per community in the LDS table, piece by piece of its partitions' mix range
+// (lv_device.h), and every community is scored.
 template <int GAIN>
-__global__ __launch_bounds__(kBlock) void k_hub_scan(HubCtx a, const uint32_t *__restrict__ list,
-                                                     uint32_t count) {
+__global__ __launch_bounds__(kBlock) void k_hub_reduce(HubCtx a) {
     using SC = Score<GAIN>;
     using S = typename SC::T;
-    const uint32_t hi = blockIdx.x / kHubParts;
-    const uint32_t part = blockIdx.x % kHubParts;
-    S best = SC::none();
-    uint32_t best_c = a.m.highest ? 0u : LV_EMPTY;
-    uint64_t ncand = 0;
-    uint32_t slot = 0;
-    const bool ok = hi < count;
-    if (ok) {
-        const uint32_t v = list[hi];
-        slot = a.h.slot_of[v];
-        int64_t m2;
-        double tw;
-        (void)vertex_graph(a.m.bt, v, a.m.m2, a.m.tw, m2, tw);   // a halted graph's table is empty
-        if (v < a.m.v_end) {
-            const uint32_t own = a.m.comm[v];
-            const uint64_t off = a.h.tab_off[slot];
-            const uint32_t size = 1u << a.h.tab_log2[slot];
-            const uint32_t per = size / kHubParts;
-            const uint32_t kv = a.m.kdeg[v];
-            const uint32_t tot_own = a.m.tot[own];
-            const uint32_t self = a.h.self[slot];
-            const uint32_t s_end = (part + 1) * per;
-            for (uint32_t sb = part * per + threadIdx.x; sb < s_end; sb += kBlock * kIlp) {
-                // kIlp slots per lane: slot reads, then the tot gathers, in flight together
-                unsigned long long cur[kIlp];
-                uint32_t ts[kIlp];
-#pragma unroll
-                for (int u = 0; u < kIlp; ++u) {
-                    const uint32_t s2 = sb + (uint32_t)u * kBlock;
-                    cur[u] = s2 < s_end ? a.h.slot[off + s2] : kHubEmpty;
-                }
-#pragma unroll
-                for (int u = 0; u < kIlp; ++u) {
-                    const uint32_t c = (uint32_t)(cur[u] >> 32);
-                    ts[u] = (c != LV_EMPTY && c != own) ? a.m.tot[c] : 0u;
-                }
-#pragma unroll
-                for (int u = 0; u < kIlp; ++u) {
-                    const uint32_t s2 = sb + (uint32_t)u * kBlock;
-                    const uint32_t c = (uint32_t)(cur[u] >> 32);
-                    if (c == LV_EMPTY) continue;
-                    a.h.slot[off + s2] = kHubEmpty;
-                    if (c == own) continue;
-                    ++ncand;
-                    const S sc = SC::of((uint32_t)cur[u], self, kv, tot_own, ts[u], m2, tw);
-                    if (better_candidate(sc, c, best, best_c, a.m.highest)) { best = sc; best_c = c; }
-                }
-            }
-        }
-    }
+    __shared__ SumTable t;
+    __shared__ uint32_t s_beg, s_cnt;
     __shared__ S r_s[kBlock / 64];
     __shared__ uint32_t r_c[kBlock / 64];
-    group_best<64, S>(best, best_c, a.m.highest);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { r_s[w] = best; r_c[w] = best_c; }
-    __syncthreads();
-    if (threadIdx.x == 0 && ok) {
-        for (int i = 0; i < kBlock / 64; ++i)
-            if (better_candidate(r_s[i], r_c[i], best, best_c, a.m.highest)) {
-                best = r_s[i];
-                best_c = r_c[i];
-            }
-        if constexpr (GAIN == 0) {
-            a.h.part_s[(size_t)slot * kHubParts + part] = best;
-        } else {
-            a.h.part_s[(size_t)slot * kHubParts + part] = (int64_t)__float_as_int(best);
+    sum_init(t);
+    uint64_t ncand = 0;
+    const uint32_t n_items = *a.h.n_items;
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const uint64_t item = a.h.items[it];
+        const uint32_t g = (uint32_t)item & (kHubParts - 1);
+        const uint32_t v = (uint32_t)(item >> 32);
+        const uint32_t slot = ((uint32_t)item >> 8) & 0xFFFFFFu;
+        const uint32_t per = (uint32_t)kHubParts >> (((uint32_t)item >> kHubPartBits) & 7u);
+        int64_t m2;
+        double tw;
+        (void)vertex_graph(a.m.bt, v, a.m.m2, a.m.tw, m2, tw);   // items are sweeping hubs only
+        __syncthreads();   // the previous item's shared scalars are consumed
+        if (threadIdx.x < 64) {
+            const uint32_t lane = threadIdx.x;
+            const uint32_t c = lane < (uint32_t)kHubParts ? a.h.cnt[(size_t)slot * kHubParts + lane] : 0u;
+            const uint64_t before = group_sum_u64<64>(lane < g * per ? c : 0u);
+            const uint64_t mine = group_sum_u64<64>(lane >= g * per && lane < (g + 1) * per ? c : 0u);
+            if (lane == 0) { s_beg = (uint32_t)before; s_cnt = (uint32_t)mine; }
         }
-        a.h.part_c[(size_t)slot * kHubParts + part] = best_c;
+        __syncthreads();
+        const uint32_t own = a.m.comm[v];
+        const uint32_t kv = a.m.kdeg[v];
+        const uint32_t tot_own = a.m.tot[own];
+        const uint32_t self = a.h.self[slot];
+        const unsigned long long *pair = a.h.pair + a.h.ent_off[slot] + s_beg;
+        const uint32_t np = s_cnt;
+        const uint64_t span_lo = (uint64_t)(g * per) << kHubSubBits;
+        const uint64_t span_hi = (uint64_t)((g + 1) * per) << kHubSubBits;
+        uint64_t width = sum_first_width(span_lo, span_hi, np);
+        uint64_t lo = span_lo;
+        S best = SC::none();
+        uint32_t best_c = a.m.highest ? 0u : LV_EMPTY;
+        while (np > 0 && lo < span_hi) {
+            const uint64_t rhi = span_hi - lo < width ? span_hi : lo + width;
+            const uint32_t nu = sum_piece<kHubIlp>(t, pair, np, lo, rhi);
+            if (nu == 0xFFFFFFFFu) {   // too many communities: halve the piece
+                width = (rhi - lo) / 2 > 0 ? (rhi - lo) / 2 : 1u;
+                continue;
+            }
+            for (uint32_t ib = threadIdx.x; ib < nu; ib += kBlock * kHubIlp) {
+                uint32_t ss[kHubIlp], cs[kHubIlp], ts[kHubIlp];
+#pragma unroll
+                for (int u = 0; u < kHubIlp; ++u) {
+                    const uint32_t i = ib + (uint32_t)u * kBlock;
+                    ss[u] = i < nu ? t.used[i] : 0u;
+                    cs[u] = i < nu ? t.keys[ss[u]] : LV_EMPTY;
+                }
+#pragma unroll
+                for (int u = 0; u < kHubIlp; ++u) ts[u] = cs[u] != LV_EMPTY ? a.m.tot[cs[u]] : 0u;
+#pragma unroll
+                for (int u = 0; u < kHubIlp; ++u) {
+                    if (cs[u] == LV_EMPTY) continue;
+                    ++ncand;
+                    const S sc = SC::of(t.vals[ss[u]], self, kv, tot_own, ts[u], m2, tw);
+                    if (better_candidate(sc, cs[u], best, best_c, a.m.highest)) { best = sc; best_c = cs[u]; }
+                    t.keys[ss[u]] = LV_EMPTY;
+                    t.vals[ss[u]] = 0u;
+                }
+            }
+            sum_next_piece(t);
+            lo = rhi;
+            if (nu < kSumTarget / 2 && width < span_hi - span_lo) width *= 2;   // sparse: widen again
+        }
+        group_best<64, S>(best, best_c, a.m.highest);
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) { r_s[w] = best; r_c[w] = best_c; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int i = 0; i < kBlock / 64; ++i)
+                if (better_candidate(r_s[i], r_c[i], best, best_c, a.m.highest)) {
+                    best = r_s[i];
+                    best_c = r_c[i];
+                }
+            if constexpr (GAIN == 0) {
+                a.h.part_s[(size_t)slot * kHubParts + g] = best;
+            } else {
+                a.h.part_s[(size_t)slot * kHubParts + g] = (int64_t)__float_as_int(best);
+            }
+            a.h.part_c[(size_t)slot * kHubParts + g] = best_c;
+        }
     }
     block_add_stats(a.m.stats, ST_CANDIDATES, ncand, -1, 0, -1, 0);
 }
@@ -853,9 +954,10 @@ __global__ void k_hub_decide(HubCtx a, const uint32_t *__restrict__ list, uint32
     const uint32_t slot = a.h.slot_of[v];
     const int lane = threadIdx.x;
     const bool highest = fast ? false : a.m.highest;
+    const uint32_t parts = fast ? kHubParts : hub_groups(a.m.row[v + 1] - a.m.row[v]);
     S best = SC::none();
     uint32_t best_c = highest ? 0u : LV_EMPTY;
-    if (lane < kHubParts) {
+    if (lane < (int)parts) {
         const int64_t raw = a.h.part_s[(size_t)slot * kHubParts + lane];
         if constexpr (GAIN == 0) best = raw;
         else best = __int_as_float((int)raw);
@@ -875,6 +977,11 @@ __global__ void k_hub_decide(HubCtx a, const uint32_t *__restrict__ list, uint32
         // a hub row holds > 4096 entries and at most two self-loop entries, so
         // it always has candidates
         atomicAdd(&a.m.stats[ST_CONSIDERED * kStatSlots + (hi & (kStatSlots - 1))], 1ull);
+    }
+    if (!fast) {   // partition counts, own weight and items start the next sweep at zero
+        if (lane < kHubParts) a.h.cnt[(size_t)slot * kHubParts + lane] = 0u;
+        if (lane == 0) a.h.self[slot] = 0u;
+        if (hi == 0 && lane == 0) *a.h.n_items = 0u;
     }
 }
 
@@ -1308,14 +1415,20 @@ void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
                 default: {
                     HubCtx h{m, hub};
                     const dim3 grid(cnt * kHubParts), blk(kBlock);
-                    if (wtd) hipLaunchKernelGGL(k_hub_insert<true>, grid, blk, 0, s, h, L, cnt);
-                    else hipLaunchKernelGGL(k_hub_insert<false>, grid, blk, 0, s, h, L, cnt);
-                    hipLaunchKernelGGL(k_hub_self, dim3((cnt + 63) / 64), dim3(64), 0, s, h, L, cnt);
+                    if (wtd) {
+                        hipLaunchKernelGGL(k_hub_count<true>, grid, blk, 0, s, h, L, cnt);
+                        hipLaunchKernelGGL(k_hub_scatter<true>, grid, blk, 0, s, h, L, cnt);
+                    } else {
+                        hipLaunchKernelGGL(k_hub_count<false>, grid, blk, 0, s, h, L, cnt);
+                        hipLaunchKernelGGL(k_hub_scatter<false>, grid, blk, 0, s, h, L, cnt);
+                    }
+                    // persistent: the items posted by k_hub_count
+                    const dim3 rgrid(std::min<uint32_t>(cnt * kHubParts, kHubReduceBlocks));
                     if (a.gain == 0) {
-                        hipLaunchKernelGGL(k_hub_scan<0>, grid, blk, 0, s, h, L, cnt);
+                        hipLaunchKernelGGL(k_hub_reduce<0>, rgrid, blk, 0, s, h);
                         hipLaunchKernelGGL(k_hub_decide<0>, dim3(cnt), dim3(64), 0, s, h, L, cnt, 0);
                     } else {
-                        hipLaunchKernelGGL(k_hub_scan<1>, grid, blk, 0, s, h, L, cnt);
+                        hipLaunchKernelGGL(k_hub_reduce<1>, rgrid, blk, 0, s, h);
                         hipLaunchKernelGGL(k_hub_decide<1>, dim3(cnt), dim3(64), 0, s, h, L, cnt, 0);
                     }
                     break;

@@ -133,6 +133,66 @@ def test_best_moves_snapshots(kind):
         assert (si, s) == (sio, so) and q == qo
 
 
+def _fmix32(x):
+    """murmur3's 32-bit finaliser, the community mix of the hub kernels."""
+    x = np.asarray(x, np.uint64) & 0xFFFFFFFF
+    x ^= x >> 16
+    x = (x * 0x85EBCA6B) & 0xFFFFFFFF
+    x ^= x >> 13
+    x = (x * 0xC2B2AE35) & 0xFFFFFFFF
+    x ^= x >> 16
+    return x
+
+
+def test_hub_partition_ranges():
+    """Hub rows are summed per community hash partition in 4096-slot LDS
+    tables, range by range of the mix bits below the partition; a range with
+    more than 3072 communities is halved and redone.  Hub A's 7000 neighbour
+    communities all mix into partition 0 and the lowest 1/8 of its range, so
+    its workgroup halves the range repeatedly; hub B spreads 20000 communities
+    over one partition's many ranges; hub C sees 12 communities; hub D has
+    self-loops and neighbours in its own community."""
+    rng = np.random.default_rng(41)
+    ids = np.arange(1, 4_000_000, dtype=np.uint64)
+    mix = _fmix32(ids)
+    part, sub = mix >> 27, mix & ((1 << 27) - 1)
+    packed = ids[(part == 0) & (sub < (1 << 24))][:7000].astype(np.int64)
+    spread = ids[part == 5][:20000].astype(np.int64)
+    assert len(packed) == 7000 and len(spread) == 20000
+    n = int(max(packed.max(), spread.max())) + 2
+    hubs = [n - 1, n - 2, n - 3, n - 4]
+    leaves = rng.permutation(np.arange(8, n - 8))[:40000]
+    la, lb, lc, ld = leaves[:7000], leaves[7000:27000], leaves[27000:33000], leaves[33000:39000]
+    chain = np.arange(n - 1)   # every id in use (the library wants dense ids)
+    ea = np.concatenate([np.full(7000, hubs[0]), np.full(20000, hubs[1]), np.full(6000, hubs[2]),
+                         np.full(6000, hubs[3]), [hubs[3]] * 2, la[:500], chain])
+    eb = np.concatenate([la, lb, lc, ld, [hubs[3]] * 2, lb[:500], chain + 1])
+    w = rng.integers(1, 7, len(ea)).astype(np.uint32)
+    g = L.Graph.from_edges(ea, eb, w)
+    og = O.Graph.from_edges(ea, eb, w)
+    assert g.n == n
+    comm = np.arange(n, dtype=np.uint32)
+    comm[la] = packed
+    comm[lb] = spread
+    comm[lc] = rng.integers(0, 12, len(lc)) + 100
+    comm[ld[:2500]] = hubs[3]
+    comm[ld[2500:]] = rng.integers(0, n, 3500)
+    # hub A's communities: one partition, the lowest eighth of its mix range,
+    # far above the 3072 an LDS range holds
+    mix_a = _fmix32(comm[la])
+    assert np.all(mix_a >> 27 == 0) and len(np.unique(comm[la])) > 2 * 3072
+    c = L.Community(g)
+    c.set_partition(comm)
+    active = np.ones(n, np.uint8)
+    for (gn, gg, og_), (tn, gt, ot) in itertools.product(GAINS, TIES):
+        say(f"hub partitions {gn} {tn}")
+        tg, ea_g = c.best_moves(L.SWEEP_HASH, L.params(gain=gg, tie=gt), active)
+        to, moved, ea_o, _ = og.best_moves(comm, active, O.SWEEP_HASH, O.params(gain=og_, tie=ot))
+        assert np.array_equal(tg[hubs], to[hubs]), (gn, tn, tg[hubs], to[hubs])
+        assert np.array_equal(tg, to), f"{gn} {tn}: {np.flatnonzero(tg != to)[:10]}"
+        assert ea_g == ea_o
+
+
 def test_fast_sweep_identity():
     rng = np.random.default_rng(11)
     ea, eb = _random_graph(rng, n=15000, m=60000, hubs=2, hub_deg=7000)
@@ -173,6 +233,59 @@ def test_aggregation_matches_oracle():
         # Q is preserved by contraction
         q_new, _, _ = c.modularity_parts()
         assert q_new == og.modularity(comm)[0]
+        c.close()
+
+
+def _packed_hub_graph(rng):
+    """Four hubs whose leaves are ids chosen by their community mix: hub A's
+    7000 leaves all mix into the lowest 1/128 of the top-bit-0 half (one
+    partition for up to 2 partitions, and far above one LDS piece), hub B's
+    20000 leaves into one of 32 partitions; a chain keeps every id in use."""
+    ids = np.arange(1, 4_000_000, dtype=np.uint64)
+    mix = _fmix32(ids)
+    part, sub = mix >> 27, mix & ((1 << 27) - 1)
+    packed = ids[(part == 0) & (sub < (1 << 24))][:7000].astype(np.int64)
+    spread = ids[part == 5][:20000].astype(np.int64)
+    n = int(max(packed.max(), spread.max())) + 2
+    hubs = [n - 1, n - 2, n - 3, n - 4]
+    others = rng.permutation(np.arange(8, n - 8))[:12000]
+    chain = np.arange(n - 1)
+    ea = np.concatenate([np.full(7000, hubs[0]), np.full(20000, hubs[1]), np.full(6000, hubs[2]),
+                         np.full(6000, hubs[3]), [hubs[3]] * 2, chain])
+    eb = np.concatenate([packed, spread, others[:6000], others[6000:], [hubs[3]] * 2, chain + 1])
+    return ea, eb, n, hubs, packed
+
+
+def test_aggregation_giant_partitions():
+    """Contraction of giant communities (above 4096 virtual entries) through
+    destination hash partitions and LDS sums: the identity partition (hub A's
+    row overflows an LDS piece again and again), blocks of 1000 vertices,
+    one community, three communities, and all even vertices merged (a giant
+    with ~900 K distinct destinations over 256 partitions)."""
+    rng = np.random.default_rng(43)
+    ea, eb, n, hubs, packed = _packed_hub_graph(rng)
+    w = rng.integers(1, 5, len(ea)).astype(np.uint32)
+    g = L.Graph.from_edges(ea, eb, w)
+    og = O.Graph.from_edges(ea, eb, w)
+    assert g.n == n
+    v = np.arange(n, dtype=np.uint32)
+    evens = np.where(v % 2 == 0, 0, v).astype(np.uint32)
+    parts = {"identity": v, "blocks": v // 1000, "one": np.zeros(n, np.uint32),
+             "three": v % 3, "evens": evens}
+    for name, comm in parts.items():
+        say(f"aggregate giants {name}")
+        c = L.Community(g)
+        c.set_partition(comm)
+        c.aggregation_phase(True)
+        new_id, n_new, h = og.aggregate(comm)
+        row, col, ww, k = c.graph_csr()
+        r2, c2, w2, k2 = h.arrays()
+        assert len(row) - 1 == n_new, name
+        assert np.array_equal(row, r2), name
+        assert np.array_equal(col, c2) and np.array_equal(ww, w2), name
+        assert np.array_equal(k.astype(np.uint64), k2), name
+        q_new, _, _ = c.modularity_parts()
+        assert q_new == og.modularity(comm)[0], name
         c.close()
 
 
