@@ -166,16 +166,21 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
 // bits of hub_mix (one per kGiantPartEntries entries, at most 4096) and its
 // virtual entries into tiles of 2^lgt (at least 16 entries per partition, so
 // a tile reserves at most one run per 16 entries).  k_giant_gather writes the
-// (destination, weight) pair of every entry in tile order and reserves each
-// (tile, partition) run; with poff = exclusive scan of the partition counts
-// (every entry is in one partition, so poff is the pair buffers' position of
-// a partition), k_giant_scatter moves the pairs to partition order;
+// (destination, weight) pair of every entry in tile order, folding repeats of
+// the destinations it caches (a community's own id and the giants it is
+// bound to repeat hundreds of thousands of times) into one pair per tile, and
+// reserves each (tile, partition) run; with poff = exclusive scan of the
+// partition counts (every pair is in one partition, so poff is the pair
+// buffers' position of a partition), k_giant_scatter moves the pairs to
+// partition order;
 // k_giant_reduce sums one partition per workgroup in the LDS table
 // (lv_device.h) and writes its distinct pairs back over the tile-order buffer;
 // k_giant_copy puts them at the community's final row.
 
 constexpr int kGiantIlp = 4;
 constexpr uint32_t kGiantMaxParts = 1u << kGiantMaxPartBits;
+constexpr uint32_t kGiantCache = 1024;                 // destinations cached per tile
+constexpr unsigned long long kGiantHole = ~0ull;       // entry folded into a cached one
 
 struct GiantCtx : GiantArgs {
     AggCtx a;
@@ -191,12 +196,18 @@ __global__ __launch_bounds__(kBlock) void k_giant_gather(GiantCtx h) {
     __shared__ uint64_t s_mrow[kBlock];
     __shared__ uint64_t s_mp[kBlock + 1];
     __shared__ uint32_t s_first;
+    // destinations cached for the tile: a repeat adds its weight and leaves a
+    // hole; each cached destination is written once, at its first entry
+    __shared__ uint32_t c_key[kGiantCache];
+    __shared__ uint32_t c_val[kGiantCache];
+    __shared__ uint32_t c_pos[kGiantCache];
     const AggCtx &a = h.a;
     const uint64_t tile = h.tiles[blockIdx.x];
     const uint32_t lt = (uint32_t)tile;
     const GiantInfo I = h.info[tile >> 32];
     const uint32_t np = 1u << I.lgp;
     for (uint32_t i = threadIdx.x; i < np; i += kBlock) hist[i] = 0u;
+    for (uint32_t i = threadIdx.x; i < kGiantCache; i += kBlock) { c_key[i] = LV_EMPTY; c_val[i] = 0u; }
     const uint64_t k0 = a.crow[I.c], k1 = a.crow[I.c + 1];
     const uint64_t s0 = k0 + ((uint64_t)lt << I.lgt);
     const uint64_t s1 = s0 + (1ull << I.lgt) < k1 ? s0 + (1ull << I.lgt) : k1;
@@ -236,16 +247,40 @@ __global__ __launch_bounds__(kBlock) void k_giant_gather(GiantCtx h) {
             for (int u = 0; u < kGiantIlp; ++u) ds[u] = xs[u] != LV_EMPTY ? a.newc[xs[u]] : LV_EMPTY;
 #pragma unroll
             for (int u = 0; u < kGiantIlp; ++u) {
-                if (ds[u] == LV_EMPTY) continue;
+                const uint32_t d = ds[u];
+                if (d == LV_EMPTY) continue;
                 const uint64_t k = kb + (uint64_t)u * kBlock;
-                out[k - k0] = ((unsigned long long)ds[u] << 32) | ws[u];
-                atomicAdd(&hist[giant_part(ds[u], I.lgp)], 1u);
+                const uint32_t cs = hub_mix(d) & (kGiantCache - 1);
+                uint32_t cur = c_key[cs];
+                if (cur == LV_EMPTY) {
+                    cur = atomicCAS(&c_key[cs], LV_EMPTY, d);
+                    if (cur == LV_EMPTY) {   // cached here; written at the flush
+                        c_pos[cs] = (uint32_t)(k - k0);
+                        atomicAdd(&c_val[cs], ws[u]);
+                        continue;
+                    }
+                }
+                if (cur == d) {
+                    atomicAdd(&c_val[cs], ws[u]);
+                    out[k - k0] = kGiantHole;
+                    continue;
+                }
+                out[k - k0] = ((unsigned long long)d << 32) | ws[u];
+                atomicAdd(&hist[giant_part(d, I.lgp)], 1u);
             }
         }
         const bool done = s_mp[nm] >= s1;
         __syncthreads();
         if (done) break;
     }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kGiantCache; i += kBlock) {
+        const uint32_t d = c_key[i];
+        if (d == LV_EMPTY) continue;
+        out[c_pos[i]] = ((unsigned long long)d << 32) | c_val[i];
+        atomicAdd(&hist[giant_part(d, I.lgp)], 1u);
+    }
+    __syncthreads();
     for (uint32_t p = threadIdx.x; p < np; p += kBlock) {
         const uint32_t c = hist[p];
         if (c) h.base[I.mbase + (uint64_t)lt * np + p] = atomicAdd(&h.pcnt[I.pbase + p], c);
@@ -274,11 +309,11 @@ __global__ __launch_bounds__(kBlock) void k_giant_scatter(GiantCtx h) {
 #pragma unroll
         for (int u = 0; u < kGiantIlp; ++u) {
             const uint64_t k = kb + (uint64_t)u * kBlock;
-            pr[u] = k < t1 ? in[k] : ~0ull;
+            pr[u] = k < t1 ? in[k] : kGiantHole;
         }
 #pragma unroll
         for (int u = 0; u < kGiantIlp; ++u) {
-            if (pr[u] == ~0ull) continue;
+            if (pr[u] == kGiantHole) continue;   // past the tile, or folded by the gather
             const uint32_t p = giant_part((uint32_t)(pr[u] >> 32), I.lgp);
             h.p1[s_off[p] + atomicAdd(&s_rank[p], 1u)] = pr[u];
         }
