@@ -39,6 +39,8 @@ void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
     bucket.ensure(std::max<size_t>(n, 1));
     mark.ensure(std::max<size_t>(n, 1));
     target.ensure(std::max<size_t>(n, 1));
+    movers.ensure(std::max<size_t>(n, 1));
+    n_movers.ensure(kBigBuckets);
     list_buf[0].ensure(std::max<size_t>(n, 1));
     list_buf[1].ensure(std::max<size_t>(n, 1));
     counts_dev.ensure(2 * kNumBuckets);
@@ -451,6 +453,8 @@ static SweepArgs sweep_args(Community &c, const lv_params &p, int kind, uint32_t
     a.gain = p.gain;
     a.highest = (p.tie == LV_TIE_REFERENCE && kind == LV_SWEEP_HASH);
     a.tiles = st.tiles;
+    a.movers = st.movers.data();
+    a.n_movers = st.n_movers.data();
     if (c.batch) a.bt = c.batch->view();
     return a;
 }

@@ -79,6 +79,8 @@ struct SweepState {
     DeviceArray<uint8_t> bucket;
     DeviceArray<uint8_t> mark;
     DeviceArray<uint32_t> target;
+    DeviceArray<uint32_t> movers;               // commit scratch of the large-row buckets
+    DeviceArray<uint32_t> n_movers;
     DeviceArray<uint32_t> list_buf[2];
     DeviceArray<uint32_t> counts_dev;
     DeviceArray<unsigned long long> stats;

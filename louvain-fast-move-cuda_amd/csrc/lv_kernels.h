@@ -21,6 +21,7 @@ constexpr int kNumBuckets = 9;        // 8 LDS-table buckets + global-table hubs
 constexpr int kHubBucket = 8;
 constexpr int kStatSlots = 64;        // spread of the per-block statistic atomics
 constexpr int kHubParts = 32;         // row chunks and community partitions per hub vertex
+constexpr int kBigBuckets = 3;        // large-row buckets (the last three) committed over movers
 constexpr uint32_t kMaxChunks = 4096; // reference chunks per level (STEP_ROUND emulation)
 
 // Maximum degree handled by each LDS bucket; larger goes to the hub bucket.
@@ -104,6 +105,8 @@ struct SweepArgs {
     int gain;                    // 0 exact, 1 ref_f32
     bool highest;                // ties to the highest community id
     TileChoice tiles;            // best-move kernel tile per bucket
+    uint32_t *movers;            // [n] commit scratch: movers of the large-row buckets
+    uint32_t *n_movers;          // [kBigBuckets] their counts
 };
 
 // One LDS bucket's best-move launch with tile candidate `sel` (the sweep kind

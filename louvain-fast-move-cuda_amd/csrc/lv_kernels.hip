@@ -1001,62 +1001,133 @@ struct CommitCtx {
     unsigned long long *g_dsig;   // batches: sigma_in change per graph
 };
 
-// Group (G <= 64) or `parts` workgroups (G == 256) per listed vertex.
-template <int G, bool WEIGHTED>
+// Commit of mover v (target != comm) over the entries part, part + stride, ...
+// of its row in steps of `stride` lanes: tot moves with v's degree (lane 0 of
+// part 0), neighbours left in another community are marked, and the change of
+// sigma_in from v's entries is returned.
+// IDENT: the sweep ran at the identity partition (every first sweep of a
+// level), so a neighbour's old community is the neighbour itself.
+template <bool WEIGHTED, bool IDENT>
+__device__ inline int64_t commit_row(const CommitCtx &a, uint32_t v, uint32_t own, uint32_t c,
+                                     uint32_t first, uint32_t stride) {
+    if (first == 0) {
+        const uint32_t kv = a.kdeg[v];
+        atomicAdd(&a.tot[c], kv);
+        atomicSub(&a.tot[own], kv);
+    }
+    int64_t dsig = 0;
+    const uint64_t beg = a.row[v], end = a.row[v + 1];
+    for (uint64_t jb = beg + first; jb < end; jb += (uint64_t)stride * kIlp) {
+        uint32_t xs[kIlp], ws[kIlp], ox[kIlp], nx[kIlp];
+#pragma unroll
+        for (int u = 0; u < kIlp; ++u) {
+            const uint64_t j = jb + (uint64_t)u * stride;
+            xs[u] = j < end ? stream_load(&a.col[j]) : v;
+            ws[u] = (WEIGHTED && j < end) ? stream_load(&a.wt[j]) : 1u;
+        }
+#pragma unroll
+        for (int u = 0; u < kIlp; ++u) {
+            ox[u] = xs[u] != v ? (IDENT ? xs[u] : a.comm[xs[u]]) : 0u;
+            nx[u] = xs[u] != v ? a.target[xs[u]] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < kIlp; ++u) {
+            if (xs[u] == v) continue;
+            const int d = (int)(nx[u] == c) - (int)(ox[u] == own);
+            if (d) dsig += (int64_t)ws[u] * d * (ox[u] != nx[u] ? 1 : 2);
+            if (nx[u] != c) a.mark[xs[u]] = 1;
+        }
+    }
+    return dsig;
+}
+
+// A group of G (<= 64) lanes per listed vertex.
+template <int G, bool WEIGHTED, bool IDENT>
 __global__ __launch_bounds__(kBlock) void k_commit(CommitCtx a, const uint32_t *__restrict__ list,
-                                                   uint32_t count, uint32_t parts) {
+                                                   uint32_t count) {
     constexpr int GPB = kBlock / G;
     const int lane = threadIdx.x & (G - 1);
-    uint32_t idx, part;
-    if constexpr (G == kBlock) {
-        idx = blockIdx.x / parts;
-        part = blockIdx.x % parts;
-    } else {
-        idx = blockIdx.x * GPB + threadIdx.x / G;
-        part = 0;
-    }
+    const uint32_t idx = blockIdx.x * GPB + threadIdx.x / G;
     int64_t dsig = 0;
     uint32_t v = 0;
     if (idx < count) {
         v = list[idx];
         const uint32_t own = a.comm[v];
         const uint32_t c = a.target[v];
-        if (c != own) {
-            if (lane == 0 && part == 0) {
-                const uint32_t kv = a.kdeg[v];
-                atomicAdd(&a.tot[c], kv);
-                atomicSub(&a.tot[own], kv);
-            }
-            const uint64_t beg = a.row[v], end = a.row[v + 1];
-            const uint64_t stride = (uint64_t)G * (G == kBlock ? parts : 1u);
-            for (uint64_t jb = beg + (uint64_t)part * G + lane; jb < end; jb += stride * kIlp) {
-                uint32_t xs[kIlp], ws[kIlp], ox[kIlp], nx[kIlp];
-#pragma unroll
-                for (int u = 0; u < kIlp; ++u) {
-                    const uint64_t j = jb + (uint64_t)u * stride;
-                    xs[u] = j < end ? stream_load(&a.col[j]) : v;
-                    ws[u] = (WEIGHTED && j < end) ? stream_load(&a.wt[j]) : 1u;
-                }
-#pragma unroll
-                for (int u = 0; u < kIlp; ++u) {
-                    ox[u] = xs[u] != v ? a.comm[xs[u]] : 0u;
-                    nx[u] = xs[u] != v ? a.target[xs[u]] : 0u;
-                }
-#pragma unroll
-                for (int u = 0; u < kIlp; ++u) {
-                    if (xs[u] == v) continue;
-                    const int d = (int)(nx[u] == c) - (int)(ox[u] == own);
-                    if (d) dsig += (int64_t)ws[u] * d * (ox[u] != nx[u] ? 1 : 2);
-                    if (nx[u] != c) a.mark[xs[u]] = 1;
-                }
-            }
-        }
+        if (c != own) dsig = commit_row<WEIGHTED, IDENT>(a, v, own, c, (uint32_t)lane, (uint32_t)G);
     }
     if (a.gid) wave_keyed_add(a.g_dsig, idx < count ? a.gid[v] : 0u, (uint64_t)dsig, dsig != 0);
     block_add_stats(a.stats, ST_DSIGMA, (uint64_t)dsig, -1, 0, -1, 0);
 }
 
+// Large rows (buckets 6 and 7, hubs): most listed vertices stay after the
+// first sweep, so the movers are compacted first (per bucket, one atomic per
+// wave) and the commit runs over (mover, part) items on a persistent grid.
+constexpr uint32_t kBigParts[kBigBuckets] = {1u, 4u, (uint32_t)kHubParts};
+constexpr uint32_t kCommitBlocks = 2048;
+
+struct BigLists {
+    const uint32_t *list[kBigBuckets];
+    uint32_t count[kBigBuckets];
+    uint32_t off[kBigBuckets];       // region of each bucket's movers in `movers`
+};
+
+__global__ __launch_bounds__(kBlock) void k_big_movers(BigLists bl, const uint32_t *comm,
+                                                       const uint32_t *target, uint32_t *movers,
+                                                       uint32_t *n_movers) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t b = 0, k = i;
+    while (b < kBigBuckets && k >= bl.count[b]) { k -= bl.count[b]; ++b; }
+    uint32_t v = 0;
+    bool mv = false;
+    if (b < kBigBuckets) {
+        v = bl.list[b][k];
+        mv = target[v] != comm[v];
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < (uint32_t)kBigBuckets; ++q) {
+        const bool sel = mv && b == q;
+        const unsigned long long m = __ballot(sel);
+        if (m == 0) continue;
+        const int leader = __ffsll((long long)m) - 1;
+        uint32_t base = 0;
+        if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(&n_movers[q], (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)base, leader);
+        if (sel) movers[bl.off[q] + base + (uint32_t)__popcll(m & lanemask_lt())] = v;
+    }
+}
+
+template <bool WEIGHTED, bool IDENT>
+__global__ __launch_bounds__(kBlock) void k_commit_big(CommitCtx a, BigLists bl,
+                                                       const uint32_t *__restrict__ movers,
+                                                       const uint32_t *__restrict__ n_movers) {
+    uint32_t nm[kBigBuckets];
+    uint32_t total = 0;
+#pragma unroll
+    for (int q = 0; q < kBigBuckets; ++q) {
+        nm[q] = n_movers[q];
+        total += nm[q] * kBigParts[q];
+    }
+    int64_t dsum = 0;
+    for (uint32_t it = blockIdx.x; it < total; it += gridDim.x) {
+        uint32_t q = 0, j = it;
+        while (j >= nm[q] * kBigParts[q]) { j -= nm[q] * kBigParts[q]; ++q; }
+        const uint32_t v = movers[bl.off[q] + j / kBigParts[q]];
+        const uint32_t part = j % kBigParts[q];
+        const int64_t dsig = commit_row<WEIGHTED, IDENT>(a, v, a.comm[v], a.target[v],
+                                                  part * kBlock + threadIdx.x,
+                                                  kBlock * kBigParts[q]);
+        if (a.gid) wave_keyed_add(a.g_dsig, a.gid[v], (uint64_t)dsig, dsig != 0);
+        dsum += dsig;
+    }
+    block_add_stats(a.stats, ST_DSIGMA, (uint64_t)dsum, -1, 0, -1, 0);
+}
+
 // ----------------------------------------------------------- finalize pass
+
+// Each workgroup takes kFinTiles consecutive tiles of kBlock vertices and
+// reserves its run of every next list with one atomic per bucket.
+constexpr int kFinTiles = 16;
 
 __global__ __launch_bounds__(kBlock) void k_finalize(uint32_t *comm, const uint32_t *target,
                                                      const uint32_t *tot, uint8_t *mark,
@@ -1065,12 +1136,19 @@ __global__ __launch_bounds__(kBlock) void k_finalize(uint32_t *comm, const uint3
                                                      unsigned long long *stats,
                                                      const uint32_t *gid,
                                                      unsigned long long *g_sumsq) {
+    __shared__ uint32_t s_cnt[kNumBuckets];
+    __shared__ uint32_t s_base[kNumBuckets];
+    if (threadIdx.x < kNumBuckets) s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
     uint64_t moved = 0, sq = 0;
-    const uint32_t ntiles = (n + kBlock - 1) / kBlock;
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const uint32_t v = t * kBlock + threadIdx.x;
-        bool sel = false;
-        int b = 0;
+    const uint32_t v0 = blockIdx.x * (uint32_t)(kBlock * kFinTiles);
+    uint32_t rank[kFinTiles];
+    uint8_t bk[kFinTiles];
+#pragma unroll
+    for (int i = 0; i < kFinTiles; ++i) {
+        const uint32_t v = v0 + (uint32_t)i * kBlock + threadIdx.x;
+        rank[i] = LV_EMPTY;
+        bk[i] = 0;
         uint64_t q = 0;
         uint32_t g = 0;
         if (v < n) {
@@ -1081,13 +1159,22 @@ __global__ __launch_bounds__(kBlock) void k_finalize(uint32_t *comm, const uint3
             if (gid) g = gid[v];
             if (mark[v]) {
                 mark[v] = 0;
-                sel = true;
-                b = bucket[v];
+                bk[i] = bucket[v];
+                rank[i] = atomicAdd(&s_cnt[bk[i]], 1u);
             }
         }
         if (gid) wave_keyed_add(g_sumsq, g, q, q != 0);   // gid is uniform: whole waves call it
-        append_to_bucket(sel, v, b, next, next_counts);
     }
+    __syncthreads();
+    if (threadIdx.x < kNumBuckets) {
+        const uint32_t c = s_cnt[threadIdx.x];
+        s_base[threadIdx.x] = c ? atomicAdd(&next_counts[threadIdx.x], c) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kFinTiles; ++i)
+        if (rank[i] != LV_EMPTY)
+            next.p[bk[i]][s_base[bk[i]] + rank[i]] = v0 + (uint32_t)i * kBlock + threadIdx.x;
     block_add_stats(stats, ST_MOVED, moved, ST_SUMSQ, sq, -1, 0);
 }
 
@@ -1446,30 +1533,53 @@ void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *cou
     c.comm = a.comm; c.target = a.target; c.tot = a.tot; c.mark = a.mark; c.stats = a.stats;
     c.gid = a.bt.gid; c.g_dsig = a.bt.dsig;
     const bool wtd = a.g.wt != nullptr;
-    for (int b = 0; b < kNumBuckets; ++b) {
+    const bool ident = a.kind == 0;   // fast sweeps run at the identity partition
+    const dim3 blk(kBlock);
+    for (int b = 0; b < kNumBuckets - kBigBuckets; ++b) {
         const uint32_t cnt = counts[b];
         if (!cnt) continue;
         const uint32_t *L = lists[b];
-        const dim3 blk(kBlock);
-#define LV_COMMIT(GG, PARTS)                                                                   \
+#define LV_COMMIT(GG)                                                                          \
     do {                                                                                       \
-        const uint32_t gpb = kBlock / (GG);                                                    \
-        const dim3 grid((GG) == kBlock ? cnt * (PARTS) : (cnt + gpb - 1) / gpb);               \
-        if (wtd) hipLaunchKernelGGL((k_commit<GG, true>), grid, blk, 0, s, c, L, cnt, (uint32_t)(PARTS)); \
-        else hipLaunchKernelGGL((k_commit<GG, false>), grid, blk, 0, s, c, L, cnt, (uint32_t)(PARTS)); \
+        const dim3 grid((cnt + kBlock / (GG) - 1) / (kBlock / (GG)));                          \
+        if (ident) {                                                                           \
+            if (wtd) hipLaunchKernelGGL((k_commit<GG, true, true>), grid, blk, 0, s, c, L, cnt);  \
+            else hipLaunchKernelGGL((k_commit<GG, false, true>), grid, blk, 0, s, c, L, cnt);     \
+        } else {                                                                               \
+            if (wtd) hipLaunchKernelGGL((k_commit<GG, true, false>), grid, blk, 0, s, c, L, cnt); \
+            else hipLaunchKernelGGL((k_commit<GG, false, false>), grid, blk, 0, s, c, L, cnt);    \
+        }                                                                                      \
     } while (0)
         switch (b) {
-            case 0: LV_COMMIT(4, 1); break;
-            case 1: LV_COMMIT(8, 1); break;
-            case 2: LV_COMMIT(16, 1); break;
-            case 3: LV_COMMIT(32, 1); break;
-            case 4: LV_COMMIT(64, 1); break;
-            case 5: LV_COMMIT(64, 1); break;
-            case 6: LV_COMMIT(256, 1); break;
-            case 7: LV_COMMIT(256, 4); break;
-            default: LV_COMMIT(256, kHubParts); break;
+            case 0: LV_COMMIT(4); break;
+            case 1: LV_COMMIT(8); break;
+            case 2: LV_COMMIT(16); break;
+            case 3: LV_COMMIT(32); break;
+            default: LV_COMMIT(64); break;   // buckets 4 and 5
         }
 #undef LV_COMMIT
+    }
+    BigLists bl;
+    uint32_t listed = 0, items = 0;
+    for (int q = 0; q < kBigBuckets; ++q) {
+        const int b = kNumBuckets - kBigBuckets + q;
+        bl.list[q] = lists[b];
+        bl.count[q] = counts[b];
+        bl.off[q] = listed;
+        listed += counts[b];
+        items += counts[b] * kBigParts[q];
+    }
+    if (!listed) return;
+    fill_u32(a.n_movers, 0u, kBigBuckets, s);
+    hipLaunchKernelGGL(k_big_movers, dim3((listed + kBlock - 1) / kBlock), blk, 0, s, bl, a.comm,
+                       (const uint32_t *)a.target, a.movers, a.n_movers);
+    const dim3 grid(std::min<uint32_t>(items, kCommitBlocks));
+    if (ident) {
+        if (wtd) hipLaunchKernelGGL((k_commit_big<true, true>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
+        else hipLaunchKernelGGL((k_commit_big<false, true>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
+    } else {
+        if (wtd) hipLaunchKernelGGL((k_commit_big<true, false>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
+        else hipLaunchKernelGGL((k_commit_big<false, false>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
     }
 }
 
@@ -1480,7 +1590,8 @@ void finalize_pass(const SweepArgs &a, uint32_t *const *next_lists, uint32_t *ne
     fill_u32(next_counts, 0u, kNumBuckets, s);
     const uint32_t n = a.g.n;
     if (!n) return;
-    hipLaunchKernelGGL(k_finalize, dim3(grid_for(n, kBlock * 4, 16384u)), dim3(kBlock), 0, s,
+    const uint32_t per = (uint32_t)(kBlock * kFinTiles);
+    hipLaunchKernelGGL(k_finalize, dim3((n + per - 1) / per), dim3(kBlock), 0, s,
                        const_cast<uint32_t *>(a.comm), (const uint32_t *)a.target,
                        (const uint32_t *)a.tot, a.mark, a.bucket, n, lp, next_counts, a.stats,
                        a.bt.gid, a.bt.sumsq);
