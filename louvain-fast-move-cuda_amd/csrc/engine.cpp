@@ -40,6 +40,7 @@ void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
     mark.ensure(std::max<size_t>(n, 1));
     target.ensure(std::max<size_t>(n, 1));
     movers.ensure(std::max<size_t>(n, 1));
+    ct.ensure(std::max<size_t>(n, 1));
     n_movers.ensure(kBigBuckets);
     list_buf[0].ensure(std::max<size_t>(n, 1));
     list_buf[1].ensure(std::max<size_t>(n, 1));
@@ -455,6 +456,7 @@ static SweepArgs sweep_args(Community &c, const lv_params &p, int kind, uint32_t
     a.tiles = st.tiles;
     a.movers = st.movers.data();
     a.n_movers = st.n_movers.data();
+    a.ct = st.ct.data();
     if (c.batch) a.bt = c.batch->view();
     return a;
 }

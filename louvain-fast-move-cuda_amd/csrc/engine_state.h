@@ -81,6 +81,7 @@ struct SweepState {
     DeviceArray<uint32_t> target;
     DeviceArray<uint32_t> movers;               // commit scratch of the large-row buckets
     DeviceArray<uint32_t> n_movers;
+    DeviceArray<unsigned long long> ct;         // commit scratch: comm | target << 32
     DeviceArray<uint32_t> list_buf[2];
     DeviceArray<uint32_t> counts_dev;
     DeviceArray<unsigned long long> stats;

@@ -107,6 +107,7 @@ struct SweepArgs {
     TileChoice tiles;            // best-move kernel tile per bucket
     uint32_t *movers;            // [n] commit scratch: movers of the large-row buckets
     uint32_t *n_movers;          // [kBigBuckets] their counts
+    unsigned long long *ct;      // [n] commit scratch: comm | target << 32
 };
 
 // One LDS bucket's best-move launch with tile candidate `sel` (the sweep kind

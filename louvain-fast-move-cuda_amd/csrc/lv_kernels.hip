@@ -999,15 +999,28 @@ struct CommitCtx {
     unsigned long long *stats;
     const uint32_t *gid;          // batches: graph of each vertex (nullptr: one graph)
     unsigned long long *g_dsig;   // batches: sigma_in change per graph
+    const unsigned long long *ct; // comm | target << 32 of every vertex (after k_pack_ct)
 };
+
+__global__ void k_pack_ct(const uint32_t *comm, const uint32_t *target, uint32_t n,
+                          unsigned long long *ct) {
+    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n;
+         v += (size_t)gridDim.x * blockDim.x)
+        ct[v] = (unsigned long long)comm[v] | ((unsigned long long)target[v] << 32);
+}
 
 // Commit of mover v (target != comm) over the entries part, part + stride, ...
 // of its row in steps of `stride` lanes: tot moves with v's degree (lane 0 of
 // part 0), neighbours left in another community are marked, and the change of
 // sigma_in from v's entries is returned.
-// IDENT: the sweep ran at the identity partition (every first sweep of a
-// level), so a neighbour's old community is the neighbour itself.
-template <bool WEIGHTED, bool IDENT>
+// Neighbour communities: CM == kCommitIdent when the sweep ran at the
+// identity partition (every first sweep of a level), so a neighbour's old
+// community is the neighbour itself; CM == kCommitPacked reads both
+// communities of a neighbour from one 8-byte word (ct = comm | target << 32).
+constexpr int kCommitIdent = 1;
+constexpr int kCommitPacked = 2;
+
+template <bool WEIGHTED, int CM>
 __device__ inline int64_t commit_row(const CommitCtx &a, uint32_t v, uint32_t own, uint32_t c,
                                      uint32_t first, uint32_t stride) {
     if (first == 0) {
@@ -1027,8 +1040,14 @@ __device__ inline int64_t commit_row(const CommitCtx &a, uint32_t v, uint32_t ow
         }
 #pragma unroll
         for (int u = 0; u < kIlp; ++u) {
-            ox[u] = xs[u] != v ? (IDENT ? xs[u] : a.comm[xs[u]]) : 0u;
-            nx[u] = xs[u] != v ? a.target[xs[u]] : 0u;
+            if (CM == kCommitIdent) {
+                ox[u] = xs[u];
+                nx[u] = xs[u] != v ? a.target[xs[u]] : 0u;
+            } else {
+                const unsigned long long p = xs[u] != v ? a.ct[xs[u]] : 0ull;
+                ox[u] = (uint32_t)p;
+                nx[u] = (uint32_t)(p >> 32);
+            }
         }
 #pragma unroll
         for (int u = 0; u < kIlp; ++u) {
@@ -1042,7 +1061,7 @@ __device__ inline int64_t commit_row(const CommitCtx &a, uint32_t v, uint32_t ow
 }
 
 // A group of G (<= 64) lanes per listed vertex.
-template <int G, bool WEIGHTED, bool IDENT>
+template <int G, bool WEIGHTED, int CM>
 __global__ __launch_bounds__(kBlock) void k_commit(CommitCtx a, const uint32_t *__restrict__ list,
                                                    uint32_t count) {
     constexpr int GPB = kBlock / G;
@@ -1054,7 +1073,7 @@ __global__ __launch_bounds__(kBlock) void k_commit(CommitCtx a, const uint32_t *
         v = list[idx];
         const uint32_t own = a.comm[v];
         const uint32_t c = a.target[v];
-        if (c != own) dsig = commit_row<WEIGHTED, IDENT>(a, v, own, c, (uint32_t)lane, (uint32_t)G);
+        if (c != own) dsig = commit_row<WEIGHTED, CM>(a, v, own, c, (uint32_t)lane, (uint32_t)G);
     }
     if (a.gid) wave_keyed_add(a.g_dsig, idx < count ? a.gid[v] : 0u, (uint64_t)dsig, dsig != 0);
     block_add_stats(a.stats, ST_DSIGMA, (uint64_t)dsig, -1, 0, -1, 0);
@@ -1097,7 +1116,7 @@ __global__ __launch_bounds__(kBlock) void k_big_movers(BigLists bl, const uint32
     }
 }
 
-template <bool WEIGHTED, bool IDENT>
+template <bool WEIGHTED, int CM>
 __global__ __launch_bounds__(kBlock) void k_commit_big(CommitCtx a, BigLists bl,
                                                        const uint32_t *__restrict__ movers,
                                                        const uint32_t *__restrict__ n_movers) {
@@ -1114,7 +1133,7 @@ __global__ __launch_bounds__(kBlock) void k_commit_big(CommitCtx a, BigLists bl,
         while (j >= nm[q] * kBigParts[q]) { j -= nm[q] * kBigParts[q]; ++q; }
         const uint32_t v = movers[bl.off[q] + j / kBigParts[q]];
         const uint32_t part = j % kBigParts[q];
-        const int64_t dsig = commit_row<WEIGHTED, IDENT>(a, v, a.comm[v], a.target[v],
+        const int64_t dsig = commit_row<WEIGHTED, CM>(a, v, a.comm[v], a.target[v],
                                                   part * kBlock + threadIdx.x,
                                                   kBlock * kBigParts[q]);
         if (a.gid) wave_keyed_add(a.g_dsig, a.gid[v], (uint64_t)dsig, dsig != 0);
@@ -1535,6 +1554,10 @@ void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *cou
     const bool wtd = a.g.wt != nullptr;
     const bool ident = a.kind == 0;   // fast sweeps run at the identity partition
     const dim3 blk(kBlock);
+    c.ct = a.ct;
+    if (!ident && a.g.n)
+        hipLaunchKernelGGL(k_pack_ct, dim3(grid_for(a.g.n, kBlock * 4)), blk, 0, s, a.comm,
+                           (const uint32_t *)a.target, a.g.n, a.ct);
     for (int b = 0; b < kNumBuckets - kBigBuckets; ++b) {
         const uint32_t cnt = counts[b];
         if (!cnt) continue;
@@ -1543,11 +1566,11 @@ void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *cou
     do {                                                                                       \
         const dim3 grid((cnt + kBlock / (GG) - 1) / (kBlock / (GG)));                          \
         if (ident) {                                                                           \
-            if (wtd) hipLaunchKernelGGL((k_commit<GG, true, true>), grid, blk, 0, s, c, L, cnt);  \
-            else hipLaunchKernelGGL((k_commit<GG, false, true>), grid, blk, 0, s, c, L, cnt);     \
+            if (wtd) hipLaunchKernelGGL((k_commit<GG, true, kCommitIdent>), grid, blk, 0, s, c, L, cnt); \
+            else hipLaunchKernelGGL((k_commit<GG, false, kCommitIdent>), grid, blk, 0, s, c, L, cnt); \
         } else {                                                                               \
-            if (wtd) hipLaunchKernelGGL((k_commit<GG, true, false>), grid, blk, 0, s, c, L, cnt); \
-            else hipLaunchKernelGGL((k_commit<GG, false, false>), grid, blk, 0, s, c, L, cnt);    \
+            if (wtd) hipLaunchKernelGGL((k_commit<GG, true, kCommitPacked>), grid, blk, 0, s, c, L, cnt); \
+            else hipLaunchKernelGGL((k_commit<GG, false, kCommitPacked>), grid, blk, 0, s, c, L, cnt); \
         }                                                                                      \
     } while (0)
         switch (b) {
@@ -1575,11 +1598,11 @@ void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *cou
                        (const uint32_t *)a.target, a.movers, a.n_movers);
     const dim3 grid(std::min<uint32_t>(items, kCommitBlocks));
     if (ident) {
-        if (wtd) hipLaunchKernelGGL((k_commit_big<true, true>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
-        else hipLaunchKernelGGL((k_commit_big<false, true>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
+        if (wtd) hipLaunchKernelGGL((k_commit_big<true, kCommitIdent>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
+        else hipLaunchKernelGGL((k_commit_big<false, kCommitIdent>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
     } else {
-        if (wtd) hipLaunchKernelGGL((k_commit_big<true, false>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
-        else hipLaunchKernelGGL((k_commit_big<false, false>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
+        if (wtd) hipLaunchKernelGGL((k_commit_big<true, kCommitPacked>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
+        else hipLaunchKernelGGL((k_commit_big<false, kCommitPacked>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
     }
 }
 
