@@ -288,24 +288,4 @@ struct ListPtrs {
     uint32_t *p[kNumBuckets];
 };
 
-// Append the vertex of each thread (if selected) to the list of its bucket,
-// with one global atomic per bucket per tile.
-__device__ inline void append_to_bucket(bool sel, uint32_t v, int b, const ListPtrs &lists,
-                                        uint32_t *counts) {
-    __shared__ uint32_t s_cnt[kNumBuckets];
-    __shared__ uint32_t s_base[kNumBuckets];
-    if (threadIdx.x < kNumBuckets) s_cnt[threadIdx.x] = 0;
-    __syncthreads();
-    uint32_t rank = 0;
-    if (sel) rank = atomicAdd(&s_cnt[b], 1u);
-    __syncthreads();
-    if (threadIdx.x < kNumBuckets) {
-        const uint32_t c = s_cnt[threadIdx.x];
-        s_base[threadIdx.x] = c ? atomicAdd(&counts[threadIdx.x], c) : 0u;
-    }
-    __syncthreads();
-    if (sel) lists.p[b][s_base[b] + rank] = v;
-    __syncthreads();
-}
-
 }  // namespace lv

@@ -316,14 +316,36 @@ __global__ __launch_bounds__(kBlock) void k_bucket_counts(const uint8_t *bucket,
     if (threadIdx.x < kNumBuckets && h[threadIdx.x]) atomicAdd(&counts[threadIdx.x], h[threadIdx.x]);
 }
 
+// Every vertex appended to the list of its bucket; each workgroup takes
+// kListTiles consecutive tiles of kBlock vertices and reserves its run of
+// every list with one atomic per bucket.
+constexpr int kListTiles = 16;
+
 __global__ __launch_bounds__(kBlock) void k_build_lists_all(const uint8_t *bucket, uint32_t n,
                                                             ListPtrs lists, uint32_t *counts) {
-    const uint32_t ntiles = (n + kBlock - 1) / kBlock;
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const uint32_t v = t * kBlock + threadIdx.x;
-        const bool sel = v < n;
-        append_to_bucket(sel, v, sel ? bucket[v] : 0, lists, counts);
+    __shared__ uint32_t s_cnt[kNumBuckets];
+    __shared__ uint32_t s_base[kNumBuckets];
+    if (threadIdx.x < kNumBuckets) s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t v0 = blockIdx.x * (uint32_t)(kBlock * kListTiles);
+    uint32_t rank[kListTiles];
+    uint8_t bk[kListTiles];
+#pragma unroll
+    for (int i = 0; i < kListTiles; ++i) {
+        const uint32_t v = v0 + (uint32_t)i * kBlock + threadIdx.x;
+        bk[i] = v < n ? bucket[v] : 0;
+        rank[i] = v < n ? atomicAdd(&s_cnt[bk[i]], 1u) : LV_EMPTY;
     }
+    __syncthreads();
+    if (threadIdx.x < kNumBuckets) {
+        const uint32_t c = s_cnt[threadIdx.x];
+        s_base[threadIdx.x] = c ? atomicAdd(&counts[threadIdx.x], c) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kListTiles; ++i)
+        if (rank[i] != LV_EMPTY)
+            lists.p[bk[i]][s_base[bk[i]] + rank[i]] = v0 + (uint32_t)i * kBlock + threadIdx.x;
 }
 
 // First entry index handled by a thread chunk, and its source vertex.
@@ -1383,8 +1405,9 @@ void build_lists_all(const uint8_t *bucket, uint32_t n, uint32_t *const *lists, 
     for (int b = 0; b < kNumBuckets; ++b) lp.p[b] = lists[b];
     fill_u32(counts, 0u, kNumBuckets, s);
     if (!n) return;
-    hipLaunchKernelGGL(k_build_lists_all, dim3(grid_for(n, kBlock * 4, 16384u)), dim3(kBlock), 0,
-                       s, bucket, n, lp, counts);
+    const uint32_t per = (uint32_t)(kBlock * kListTiles);
+    hipLaunchKernelGGL(k_build_lists_all, dim3((n + per - 1) / per), dim3(kBlock), 0, s, bucket, n,
+                       lp, counts);
 }
 
 void identity_parts(const LevelView &g, unsigned long long *stats, hipStream_t s) {
