@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstddef>
 #include <cstdint>
 #include <stdexcept>
@@ -33,6 +34,23 @@ inline void hip_check(hipError_t e, const char *what) {
 }
 
 #define LV_HIP(call) ::lv::hip_check((call), #call)
+
+// Waits for the work queued on stream s.  A run's host thread has nothing to
+// do while it waits for list sizes or level sizes (~30 times per run, for
+// tens of microseconds each), so it polls the stream rather than sleeping in
+// a blocking wait; after 20 ms of polling it blocks.
+inline void wait_stream(hipStream_t s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        const hipError_t e = hipStreamQuery(s);
+        if (e == hipSuccess) return;
+        if (e != hipErrorNotReady) hip_check(e, "hipStreamQuery");
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) {
+            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+            return;
+        }
+    }
+}
 
 template <typename T>
 class DeviceArray {

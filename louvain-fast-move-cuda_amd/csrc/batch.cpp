@@ -135,7 +135,7 @@ GraphBatch::GraphBatch(const std::vector<const GraphHost *> &graphs, hipStream_t
     LV_HIP(hipMemcpyAsync(B->gid0.data(), gid.data(), n0 * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     LV_HIP(hipMemcpyAsync(B->m2_dev.data(), B->m2.data(), G * sizeof(int64_t), hipMemcpyHostToDevice, s));
     LV_HIP(hipMemcpyAsync(B->tw_dev.data(), B->tw.data(), G * sizeof(double), hipMemcpyHostToDevice, s));
-    LV_HIP(hipStreamSynchronize(s));
+    wait_stream(s);
     B->gid = B->gid0.data();
     C_->batch = std::move(B);
 }
@@ -157,7 +157,7 @@ void GraphBatch::run(const lv_params &p, lv_batch_result *r) {
         for (uint32_t g = 0; g < G; ++g)
             if (B.entries[g] > p.chunk_entries)
                 throw Error(LV_ERR_ARG, "chunk emulation in a batch needs graphs of one chunk");
-    LV_HIP(hipStreamSynchronize(s));
+    wait_stream(s);
     const auto t0 = std::chrono::steady_clock::now();
     C.reset();
     B.gid = B.gid0.data();
@@ -189,7 +189,7 @@ void GraphBatch::run(const lv_params &p, lv_batch_result *r) {
             launch_sweep(C, p, kind, true, C.graph.n_nodes);
             LV_HIP(hipMemcpyAsync(B.host_acc, B.acc.data(), 2 * (size_t)G * sizeof(unsigned long long),
                                   hipMemcpyDeviceToHost, s));
-            LV_HIP(hipStreamSynchronize(s));
+            wait_stream(s);
             const unsigned long long *h = st.host_stats;
             traversed += all_bucket_slots(h, ST_E_ACTIVE);
             ++exec;
@@ -240,7 +240,7 @@ void GraphBatch::run(const lv_params &p, lv_batch_result *r) {
     }
     LV_HIP(hipMemcpyAsync(cnt, B.acc.data() + 2 * (size_t)G, G * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, s));
-    LV_HIP(hipStreamSynchronize(s));
+    wait_stream(s);
     for (uint32_t g = 0; g < G; ++g) ncomm[g] = (uint32_t)cnt[g];
     const auto t1 = std::chrono::steady_clock::now();
     if (!r) return;

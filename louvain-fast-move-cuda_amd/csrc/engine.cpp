@@ -57,7 +57,7 @@ void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
     uint32_t cnt[kNumBuckets];
     LV_HIP(hipMemcpyAsync(host_stats, counts_dev.data(), kNumBuckets * sizeof(uint32_t),
                           hipMemcpyDeviceToHost, s));
-    LV_HIP(hipStreamSynchronize(s));
+    wait_stream(s);
     std::memcpy(cnt, host_stats, sizeof(cnt));
     uint64_t off = 0;
     for (int b = 0; b < kNumBuckets; ++b) {
@@ -80,7 +80,7 @@ void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
                               n_hubs * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         LV_HIP(hipMemcpyAsync(hdeg.data(), hdeg_dev, n_hubs * sizeof(uint64_t),
                               hipMemcpyDeviceToHost, s));
-        LV_HIP(hipStreamSynchronize(s));
+        wait_stream(s);
         std::vector<uint64_t> eoff(n_hubs);
         uint64_t hub_entries = 0;
         for (uint32_t i = 0; i < n_hubs; ++i) {
@@ -114,7 +114,7 @@ void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
         LV_HIP(hipMemcpyAsync(hub_slots, slots_host.data(), n_hubs * sizeof(uint32_t),
                               hipMemcpyHostToDevice, s));
         scatter_u32(hub_slot_of.data(), hub_ids, hub_slots, n_hubs, s);
-        LV_HIP(hipStreamSynchronize(s));
+        wait_stream(s);
     }
 }
 
@@ -222,7 +222,7 @@ GraphDevice::GraphDevice(const GraphHost &g, hipStream_t s) {
     if (n_nodes)
         LV_HIP(hipMemcpyAsync(tot_weight_per_nodes.data(), k.data(), n_nodes * sizeof(uint32_t),
                               hipMemcpyHostToDevice, s));
-    LV_HIP(hipStreamSynchronize(s));
+    wait_stream(s);
 }
 
 // --------------------------------------------------------------- Community
@@ -388,7 +388,7 @@ void Community::update_level(bool carry, int64_t sigma_in_carry, uint64_t sum_to
     identity_parts(level_view(graph), state->stats.data() + kGlobalSection, s);
     LV_HIP(hipMemcpyAsync(state->host_stats, state->stats.data(),
                           kStatWordsOffset * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-    LV_HIP(hipStreamSynchronize(s));
+    wait_stream(s);
     sigma_in = (int64_t)sum_slots(state->host_stats, ST_DSIGMA);
     sum_tot_sq = sum_slots(state->host_stats, ST_SUMSQ);
     modularity = q_from_parts();
@@ -406,7 +406,7 @@ void Community::compute_modularity() {
                      state->stats.data() + kGlobalSection, s);
     LV_HIP(hipMemcpyAsync(state->host_stats, state->stats.data(),
                           kStatWordsOffset * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-    LV_HIP(hipStreamSynchronize(s));
+    wait_stream(s);
     sigma_in = (int64_t)sum_slots(state->host_stats, ST_DSIGMA);
     sum_tot_sq = sum_slots(state->host_stats, ST_SUMSQ);
     modularity = q_from_parts();
@@ -545,7 +545,7 @@ int OptimizationPhase::sweep_loop(Community &c, const lv_params &p, int64_t exec
         uint32_t v_end = n;
         if (p.chunk_emulation && kind != LV_SWEEP_HASH) v_end = chunk_vertex_end(c, p, kind, nullptr);
         launch_sweep(c, p, kind, true, v_end);
-        LV_HIP(hipStreamSynchronize(s));
+        wait_stream(s);
         const unsigned long long *h = st.host_stats;
         const uint64_t e_active = all_bucket_slots(h, ST_E_ACTIVE);
         const uint64_t considered = all_bucket_slots(h, ST_CONSIDERED);
@@ -645,7 +645,7 @@ uint32_t chunk_vertex_end(Community &c, const lv_params &p, int kind, const uint
         std::vector<uint64_t> row((size_t)n + 1);
         LV_HIP(hipMemcpyAsync(row.data(), c.graph.neighboorhood_sum.data(),
                               ((size_t)n + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-        LV_HIP(hipStreamSynchronize(s));
+        wait_stream(s);
         // limit = round + STEP; limit = limit >= E ? E : row[src(limit) - 1]
         // (phaseOptimizationSort.cu:46-60); a bound that would not advance moves
         // to the next vertex boundary instead of looping forever
@@ -683,7 +683,7 @@ uint32_t chunk_vertex_end(Community &c, const lv_params &p, int kind, const uint
                        st.ws.get<uint32_t>(kChunkSlot, nc), nc, csel, s);
     std::vector<unsigned long long> h(nc);
     LV_HIP(hipMemcpyAsync(h.data(), csel, nc * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-    LV_HIP(hipStreamSynchronize(s));
+    wait_stream(s);
     // the fast/sort sweeps return at the first chunk that selects nothing
     // (phaseOptimizationSort.cu:81-86, :235-240)
     for (uint32_t k = 0; k < nc; ++k)
@@ -723,7 +723,7 @@ uint32_t *renumber(Community &c, uint32_t &n_new) {
     c.update_best(map);
     n_new = 0;
     if (n) LV_HIP(hipMemcpyAsync(&n_new, map + (n - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    LV_HIP(hipStreamSynchronize(s));
+    wait_stream(s);
     return map;
 }
 }  // namespace lv
@@ -781,7 +781,7 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
     bucket_counts(cb, n_new, cnt_dev, s);
     uint32_t cnt[kNumBuckets];
     LV_HIP(hipMemcpyAsync(cnt, cnt_dev, sizeof(cnt), hipMemcpyDeviceToHost, s));
-    LV_HIP(hipStreamSynchronize(s));
+    wait_stream(s);
     uint32_t *clists = ws.get<uint32_t>(W_CLISTS, n_new);
     uint32_t *lp[kNumBuckets];
     {
@@ -814,7 +814,7 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
         uint64_t *gsize_dev = ws.get<uint64_t>(W_HSIZE, nh);
         gather_degrees(crow, lp[kHubBucket], nh, gsize_dev, s);
         LV_HIP(hipMemcpyAsync(gsize.data(), gsize_dev, nh * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-        LV_HIP(hipStreamSynchronize(s));
+        wait_stream(s);
         std::vector<GiantInfo> info(nh);
         std::vector<uint64_t> tiles, items;
         uint64_t off = 0, mbase = 0;
@@ -843,7 +843,7 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
                               hipMemcpyHostToDevice, s));
         LV_HIP(hipMemcpyAsync(items_dev, items.data(), items.size() * sizeof(uint64_t),
                               hipMemcpyHostToDevice, s));
-        LV_HIP(hipStreamSynchronize(s));   // the plan's host vectors end with this block
+        wait_stream(s);   // the plan's host vectors end with this block
         ga.info = info_dev;
         ga.tiles = tiles_dev;
         ga.n_tiles = (uint32_t)tiles.size();
@@ -869,13 +869,13 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
              ws.get<uint64_t>(W_SCAN64, scan_scratch_elems((size_t)n_new + 1)), s);
     uint64_t e_new = 0;
     LV_HIP(hipMemcpyAsync(&e_new, orow + n_new, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    LV_HIP(hipStreamSynchronize(s));
+    wait_stream(s);
     ng.edge_destination.ensure(std::max<size_t>(e_new, 1));
     ng.weights.ensure(std::max<size_t>(e_new, 1));
     agg_copy(x, lp, cnt, orow, ng.edge_destination.data(), ng.weights.data(), s);
     agg_giant_copy(x, ga, n_parts, orow, ng.edge_destination.data(), ng.weights.data(), s);
     new_degrees(c.communities_weight.data(), map, n, ng.tot_weight_per_nodes.data(), s);
-    LV_HIP(hipStreamSynchronize(s));
+    wait_stream(s);
     ng.n_entries = e_new;
     ng.n_links = e_new / 2;                                           // :97
     c.graph = GraphDevice::view_of(ng);
@@ -932,7 +932,7 @@ Community ModularityAlgorithms::Louvain(const GraphHost &graph, const lv_params 
 
 void ModularityAlgorithms::run(Community &C, const lv_params &p, lv_result *r) {
     hipStream_t s = C.stream();
-    LV_HIP(hipStreamSynchronize(s));
+    wait_stream(s);
     C.state->acc = KernelStatsAcc();
     const auto t0 = std::chrono::steady_clock::now();
     int nrec = 0, nlev = 0;
@@ -977,7 +977,7 @@ void ModularityAlgorithms::run(Community &C, const lv_params &p, lv_result *r) {
         const int cap = rec ? r->sweeps_cap - nrec : 0;
         nrec += RefinePhase::run(C, p, nlev, rec, cap, r ? r->hash_trace : 0, &traversed);
     }
-    LV_HIP(hipStreamSynchronize(s));
+    wait_stream(s);
     const auto t1 = std::chrono::steady_clock::now();
     g_last_run_stats = C.state->acc;
     if (r) {
