@@ -500,7 +500,8 @@ void launch_sweep(Community &c, const lv_params &p, int kind, bool apply, uint32
     hipStream_t s = c.stream();
     if (p.autotune) autotune_tiles(c, p, kind, v_end);
     SweepArgs a = sweep_args(c, p, kind, v_end);
-    fill_u64(st.stats.data(), 0ull, st.stats.size(), s);
+    sweep_reset(st.stats.data(), st.stats.size(), a.n_movers, st.counts_dev.data() + kNumBuckets, s);
+    a.counters_zeroed = true;
     uint32_t *const *cur = st.lists(st.cur);
     if (p.profile) LV_HIP(hipEventRecord(st.ev[0], s));
     if (kind == LV_SWEEP_SORT) sort_sweep(a, cur, st.count, st.bucket_off, st.ws, s);

@@ -108,6 +108,7 @@ struct SweepArgs {
     uint32_t *movers;            // [n] commit scratch: movers of the large-row buckets
     uint32_t *n_movers;          // [kBigBuckets] their counts
     unsigned long long *ct;      // [n] commit scratch: comm | target << 32
+    bool counters_zeroed = false; // n_movers and the next-list counts were zeroed by sweep_reset
 };
 
 // One LDS bucket's best-move launch with tile candidate `sel` (the sweep kind
@@ -117,6 +118,9 @@ void move_bucket(const SweepArgs &a, int bucket, int sel, const uint32_t *list, 
 
 // ---- primitives ----
 void fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
+// one launch at the start of a sweep: stats, movers counts and next-list counts to zero
+void sweep_reset(unsigned long long *stats, size_t n_stats, uint32_t *n_movers,
+                 uint32_t *next_counts, hipStream_t s);
 void fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
 void fill_u64(unsigned long long *p, unsigned long long v, size_t n, hipStream_t s);
 void iota_u32(uint32_t *p, size_t n, hipStream_t s);

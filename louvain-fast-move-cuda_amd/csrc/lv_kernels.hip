@@ -52,6 +52,18 @@ __global__ void k_fill_u64(unsigned long long *p, unsigned long long v, size_t n
          i += (size_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
+// Start of a sweep: statistics, movers counts and next-list counts to zero.
+__global__ void k_sweep_reset(unsigned long long *stats, size_t n_stats, uint32_t *a, uint32_t na,
+                              uint32_t *b, uint32_t nb) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n_stats;
+         i += (size_t)gridDim.x * blockDim.x)
+        stats[i] = 0ull;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < na) a[threadIdx.x] = 0u;
+        if (threadIdx.x < nb) b[threadIdx.x] = 0u;
+    }
+}
+
 __global__ void k_iota_u32(uint32_t *p, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
@@ -1082,22 +1094,48 @@ __device__ inline int64_t commit_row(const CommitCtx &a, uint32_t v, uint32_t ow
     return dsig;
 }
 
-// A group of G (<= 64) lanes per listed vertex.
+// A group of G (<= 64) lanes per listed vertex of buckets 0-5, all six
+// buckets in one launch: workgroup b takes bucket q with wg[q] <= b < wg[q+1].
+constexpr int kSmallBuckets = kNumBuckets - kBigBuckets;
+constexpr int kSmallG[kSmallBuckets] = {4, 8, 16, 32, 64, 64};
+
+struct SmallLists {
+    const uint32_t *list[kSmallBuckets];
+    uint32_t count[kSmallBuckets];
+    uint32_t wg[kSmallBuckets + 1];   // first workgroup of each bucket
+};
+
 template <int G, bool WEIGHTED, int CM>
-__global__ __launch_bounds__(kBlock) void k_commit(CommitCtx a, const uint32_t *__restrict__ list,
-                                                   uint32_t count) {
+__device__ inline int64_t commit_group(const CommitCtx &a, const uint32_t *__restrict__ list,
+                                       uint32_t count, uint32_t wb, uint32_t &v) {
     constexpr int GPB = kBlock / G;
     const int lane = threadIdx.x & (G - 1);
-    const uint32_t idx = blockIdx.x * GPB + threadIdx.x / G;
+    const uint32_t idx = wb * GPB + threadIdx.x / G;
     int64_t dsig = 0;
-    uint32_t v = 0;
     if (idx < count) {
         v = list[idx];
         const uint32_t own = a.comm[v];
         const uint32_t c = a.target[v];
         if (c != own) dsig = commit_row<WEIGHTED, CM>(a, v, own, c, (uint32_t)lane, (uint32_t)G);
     }
-    if (a.gid) wave_keyed_add(a.g_dsig, idx < count ? a.gid[v] : 0u, (uint64_t)dsig, dsig != 0);
+    return dsig;
+}
+
+template <bool WEIGHTED, int CM>
+__global__ __launch_bounds__(kBlock) void k_commit(CommitCtx a, SmallLists sl) {
+    uint32_t q = 0;
+    while (q + 1 < (uint32_t)kSmallBuckets && blockIdx.x >= sl.wg[q + 1]) ++q;
+    const uint32_t wb = blockIdx.x - sl.wg[q];
+    uint32_t v = LV_EMPTY;
+    int64_t dsig = 0;
+    switch (kSmallG[q]) {   // uniform over the workgroup
+        case 4: dsig = commit_group<4, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v); break;
+        case 8: dsig = commit_group<8, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v); break;
+        case 16: dsig = commit_group<16, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v); break;
+        case 32: dsig = commit_group<32, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v); break;
+        default: dsig = commit_group<64, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v); break;
+    }
+    if (a.gid) wave_keyed_add(a.g_dsig, v != LV_EMPTY ? a.gid[v] : 0u, (uint64_t)dsig, dsig != 0);
     block_add_stats(a.stats, ST_DSIGMA, (uint64_t)dsig, -1, 0, -1, 0);
 }
 
@@ -1277,6 +1315,14 @@ void batch_next_gid(const uint32_t *tot, const uint32_t *map, const uint32_t *gi
     if (!n) return;
     hipLaunchKernelGGL(k_batch_next_gid, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, tot,
                        map, gid, n, gid_new);
+}
+
+void sweep_reset(unsigned long long *stats, size_t n_stats, uint32_t *n_movers,
+                 uint32_t *next_counts, hipStream_t s) {
+    static_assert(kBigBuckets <= kBlock && kNumBuckets <= kBlock, "one counter per thread");
+    hipLaunchKernelGGL(k_sweep_reset, dim3(grid_for(n_stats, kBlock * 4)), dim3(kBlock), 0, s,
+                       stats, n_stats, n_movers, (uint32_t)kBigBuckets, next_counts,
+                       (uint32_t)kNumBuckets);
 }
 
 void fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
@@ -1581,29 +1627,24 @@ void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *cou
     if (!ident && a.g.n)
         hipLaunchKernelGGL(k_pack_ct, dim3(grid_for(a.g.n, kBlock * 4)), blk, 0, s, a.comm,
                            (const uint32_t *)a.target, a.g.n, a.ct);
-    for (int b = 0; b < kNumBuckets - kBigBuckets; ++b) {
-        const uint32_t cnt = counts[b];
-        if (!cnt) continue;
-        const uint32_t *L = lists[b];
-#define LV_COMMIT(GG)                                                                          \
-    do {                                                                                       \
-        const dim3 grid((cnt + kBlock / (GG) - 1) / (kBlock / (GG)));                          \
-        if (ident) {                                                                           \
-            if (wtd) hipLaunchKernelGGL((k_commit<GG, true, kCommitIdent>), grid, blk, 0, s, c, L, cnt); \
-            else hipLaunchKernelGGL((k_commit<GG, false, kCommitIdent>), grid, blk, 0, s, c, L, cnt); \
-        } else {                                                                               \
-            if (wtd) hipLaunchKernelGGL((k_commit<GG, true, kCommitPacked>), grid, blk, 0, s, c, L, cnt); \
-            else hipLaunchKernelGGL((k_commit<GG, false, kCommitPacked>), grid, blk, 0, s, c, L, cnt); \
-        }                                                                                      \
-    } while (0)
-        switch (b) {
-            case 0: LV_COMMIT(4); break;
-            case 1: LV_COMMIT(8); break;
-            case 2: LV_COMMIT(16); break;
-            case 3: LV_COMMIT(32); break;
-            default: LV_COMMIT(64); break;   // buckets 4 and 5
+    SmallLists sl;
+    uint32_t wgs = 0;
+    for (int q = 0; q < kSmallBuckets; ++q) {
+        sl.list[q] = lists[q];
+        sl.count[q] = counts[q];
+        sl.wg[q] = wgs;
+        const uint32_t gpb = kBlock / kSmallG[q];
+        wgs += (counts[q] + gpb - 1) / gpb;
+    }
+    sl.wg[kSmallBuckets] = wgs;
+    if (wgs) {
+        if (ident) {
+            if (wtd) hipLaunchKernelGGL((k_commit<true, kCommitIdent>), dim3(wgs), blk, 0, s, c, sl);
+            else hipLaunchKernelGGL((k_commit<false, kCommitIdent>), dim3(wgs), blk, 0, s, c, sl);
+        } else {
+            if (wtd) hipLaunchKernelGGL((k_commit<true, kCommitPacked>), dim3(wgs), blk, 0, s, c, sl);
+            else hipLaunchKernelGGL((k_commit<false, kCommitPacked>), dim3(wgs), blk, 0, s, c, sl);
         }
-#undef LV_COMMIT
     }
     BigLists bl;
     uint32_t listed = 0, items = 0;
@@ -1616,7 +1657,7 @@ void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *cou
         items += counts[b] * kBigParts[q];
     }
     if (!listed) return;
-    fill_u32(a.n_movers, 0u, kBigBuckets, s);
+    if (!a.counters_zeroed) fill_u32(a.n_movers, 0u, kBigBuckets, s);
     hipLaunchKernelGGL(k_big_movers, dim3((listed + kBlock - 1) / kBlock), blk, 0, s, bl, a.comm,
                        (const uint32_t *)a.target, a.movers, a.n_movers);
     const dim3 grid(std::min<uint32_t>(items, kCommitBlocks));
@@ -1633,7 +1674,7 @@ void finalize_pass(const SweepArgs &a, uint32_t *const *next_lists, uint32_t *ne
                    hipStream_t s) {
     ListPtrs lp;
     for (int b = 0; b < kNumBuckets; ++b) lp.p[b] = next_lists[b];
-    fill_u32(next_counts, 0u, kNumBuckets, s);
+    if (!a.counters_zeroed) fill_u32(next_counts, 0u, kNumBuckets, s);
     const uint32_t n = a.g.n;
     if (!n) return;
     const uint32_t per = (uint32_t)(kBlock * kFinTiles);
