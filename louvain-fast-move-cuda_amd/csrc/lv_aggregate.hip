@@ -375,7 +375,7 @@ __global__ __launch_bounds__(kBlock) void k_giant_copy(GiantCtx h, const uint64_
     }
 }
 
-// Runs of equal keys in the sorted hub entries: head flags.
+// Runs of equal keys in sorted keys (the sort sweep, .edge ingest): head flags.
 __global__ void k_heads(const uint64_t *keys, uint64_t n, uint32_t *heads) {
     for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n;
          j += (uint64_t)gridDim.x * blockDim.x)
