@@ -177,7 +177,7 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
 // (lv_device.h) and writes its distinct pairs back over the tile-order buffer;
 // k_giant_copy puts them at the community's final row.
 
-constexpr int kGiantIlp = 4;
+constexpr int kGiantIlp = 8;
 constexpr uint32_t kGiantMaxParts = 1u << kGiantMaxPartBits;
 constexpr uint32_t kGiantCache = 1024;                 // destinations cached per tile
 constexpr unsigned long long kGiantHole = ~0ull;       // entry folded into a cached one
