@@ -13,6 +13,7 @@ import pytest
 
 import louvain_amd as L
 from oracle import oracle as O
+from partition_mix import fmix32
 
 pytestmark = pytest.mark.gpu
 
@@ -133,17 +134,6 @@ def test_best_moves_snapshots(kind):
         assert (si, s) == (sio, so) and q == qo
 
 
-def _fmix32(x):
-    """murmur3's 32-bit finaliser, the community mix of the hub kernels."""
-    x = np.asarray(x, np.uint64) & 0xFFFFFFFF
-    x ^= x >> 16
-    x = (x * 0x85EBCA6B) & 0xFFFFFFFF
-    x ^= x >> 13
-    x = (x * 0xC2B2AE35) & 0xFFFFFFFF
-    x ^= x >> 16
-    return x
-
-
 def test_hub_partition_ranges():
     """Hub rows are summed per community hash partition in 4096-slot LDS
     tables, range by range of the mix bits below the partition; a range with
@@ -154,7 +144,7 @@ def test_hub_partition_ranges():
     self-loops and neighbours in its own community."""
     rng = np.random.default_rng(41)
     ids = np.arange(1, 4_000_000, dtype=np.uint64)
-    mix = _fmix32(ids)
+    mix = fmix32(ids)
     part, sub = mix >> 27, mix & ((1 << 27) - 1)
     packed = ids[(part == 0) & (sub < (1 << 24))][:7000].astype(np.int64)
     spread = ids[part == 5][:20000].astype(np.int64)
@@ -179,7 +169,7 @@ def test_hub_partition_ranges():
     comm[ld[2500:]] = rng.integers(0, n, 3500)
     # hub A's communities: one partition, the lowest eighth of its mix range,
     # far above the 3072 an LDS range holds
-    mix_a = _fmix32(comm[la])
+    mix_a = fmix32(comm[la])
     assert np.all(mix_a >> 27 == 0) and len(np.unique(comm[la])) > 2 * 3072
     c = L.Community(g)
     c.set_partition(comm)
@@ -242,7 +232,7 @@ def _packed_hub_graph(rng):
     partition for up to 2 partitions, and far above one LDS piece), hub B's
     20000 leaves into one of 32 partitions; a chain keeps every id in use."""
     ids = np.arange(1, 4_000_000, dtype=np.uint64)
-    mix = _fmix32(ids)
+    mix = fmix32(ids)
     part, sub = mix >> 27, mix & ((1 << 27) - 1)
     packed = ids[(part == 0) & (sub < (1 << 24))][:7000].astype(np.int64)
     spread = ids[part == 5][:20000].astype(np.int64)
