@@ -657,10 +657,12 @@ __global__ __launch_bounds__(kBlock) void k_move_fast(MoveCtx a, const uint32_t 
 // its neighbour communities into kHubParts partitions by the top bits of a
 // bijective mix of the community id.  k_hub_count gathers the community of
 // every entry into the hub's c-buffer, sums the weight towards the hub's own
-// community and reserves each (chunk, partition) run; k_hub_scatter packs the
-// (community, weight) pairs in partition order; k_hub_reduce gives every
-// partition to one workgroup, which sums the weights per community in an LDS
-// table and scores them.  Every entry costs streaming reads and writes and
+// community, reserves each (chunk, partition) run and posts the hub's reduce
+// groups (one per kHubGroupEntries entries, each a run of consecutive
+// partitions); k_hub_scatter packs the (community, weight) pairs in partition
+// order; k_hub_reduce, a persistent grid over the posted groups, sums a
+// group's weights per community in an LDS table and scores them; k_hub_decide
+// reduces the groups' bests.  Every entry costs streaming reads and writes and
 // LDS atomics only; hub rows see almost only distinct communities, so a
 // global table would take one random atomic per entry.
 
@@ -678,6 +680,8 @@ static_assert(kHubParts == 32, "items pack the group in 5 bits");
 
 __device__ inline uint32_t ilog2_u32(uint32_t x) { return 31u - (uint32_t)__builtin_clz(x); }
 
+// Reduce groups of a hub row of d entries: a power of two <= kHubParts, one
+// per kHubGroupEntries; group g takes kHubParts / groups consecutive partitions.
 __device__ inline uint32_t hub_groups(uint64_t d) {
     uint32_t g = 1;
     while (g < (uint32_t)kHubParts && (uint64_t)g * kHubGroupEntries < d) g <<= 1;
