@@ -147,7 +147,25 @@ HubTables SweepState::hub_tables() {
     return h;
 }
 
+const SideStreams *SweepState::side_streams() {
+    if (!side.ready()) {
+        for (int i = 0; i < 3; ++i) {
+            LV_HIP(hipStreamCreateWithFlags(&side.s[i], hipStreamNonBlocking));
+            LV_HIP(hipEventCreateWithFlags(&side.join[i], hipEventDisableTiming));
+        }
+        LV_HIP(hipEventCreateWithFlags(&side.fork, hipEventDisableTiming));
+    }
+    return &side;
+}
+
 SweepState::~SweepState() {
+    for (int i = 0; i < 3; ++i) {
+        if (side.join[i]) (void)hipEventDestroy(side.join[i]);
+        if (side.s[i]) (void)hipStreamDestroy(side.s[i]);
+    }
+    if (side.fork) (void)hipEventDestroy(side.fork);
+    for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+    for (auto &e : bev) if (e) (void)hipEventDestroy(e);
     if (host_stats) (void)hipHostFree(host_stats);
 }
 
@@ -505,7 +523,7 @@ void launch_sweep(Community &c, const lv_params &p, int kind, bool apply, uint32
     uint32_t *const *cur = st.lists(st.cur);
     if (p.profile) LV_HIP(hipEventRecord(st.ev[0], s));
     if (kind == LV_SWEEP_SORT) sort_sweep(a, cur, st.count, st.bucket_off, st.ws, s);
-    else move_pass(a, st.hub_tables(), cur, st.count, s);
+    else move_pass(a, st.hub_tables(), cur, st.count, s, st.side_streams());
     if (p.profile) LV_HIP(hipEventRecord(st.ev[1], s));
     if (apply) {
         commit_pass(a, cur, st.count, s);

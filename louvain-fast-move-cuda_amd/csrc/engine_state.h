@@ -115,6 +115,9 @@ struct SweepState {
     // tiles of the best-move kernels (tuned once per community when asked)
     TileChoice tiles;
     bool tuned[2] = {false, false};   // [fast sweep]
+    // side streams of the best-move pass (created on first use)
+    SideStreams side;
+    const SideStreams *side_streams();
     // profiling
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     hipEvent_t bev[2 * kNumBuckets] = {};

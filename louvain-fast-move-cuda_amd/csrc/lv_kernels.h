@@ -165,8 +165,20 @@ void modularity_parts(const LevelView &g, const uint32_t *comm, const uint32_t *
 void identity_parts(const LevelView &g, unsigned long long *stats, hipStream_t s);
 
 // ---- sweep ----
+// Side streams of a pass whose launches are independent: the pass forks
+// after `fork` is recorded on the main stream and joins before it returns.
+struct SideStreams {
+    hipStream_t s[3] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[3] = {};
+    bool ready() const { return s[0] != nullptr; }
+};
+
+// side: when ready, buckets 5, 7 and the hubs run on side streams beside
+// buckets 0-4, then bucket 6 alone (a bucket's profiling events on its own
+// stream).
 void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
-               const uint32_t *counts, hipStream_t s);
+               const uint32_t *counts, hipStream_t s, const SideStreams *side = nullptr);
 void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *counts,
                  hipStream_t s);
 // comm := target for movers, sum tot^2, compact marked vertices into next lists

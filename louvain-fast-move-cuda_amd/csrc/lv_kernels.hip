@@ -23,6 +23,7 @@
 #include "lv_kernels.h"
 
 #include "../../include/constants.h"
+#include "../../include/device_array.h"
 #include "../../include/operators.h"
 
 #include <algorithm>
@@ -1555,14 +1556,35 @@ void move_bucket(const SweepArgs &a, int b, int sel, const uint32_t *list, uint3
 }
 
 void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
-               const uint32_t *counts, hipStream_t s) {
+               const uint32_t *counts, hipStream_t s0, const SideStreams *side) {
     MoveCtx m = move_ctx(a);
     const bool wtd = a.g.wt != nullptr;
     const bool fast = a.kind == 0;
-    for (int b = 0; b < kNumBuckets; ++b) {
+    // Buckets 5, 7 and the hubs run on side streams beside buckets 0-4 (their
+    // tails and launch gaps overlap); bucket 6, the largest launch, runs after
+    // the join with the device to itself.  A bucket's profiling events go on
+    // the stream it runs on.
+    const bool fork = side && side->ready();
+    if (fork) {
+        LV_HIP(hipEventRecord(side->fork, s0));
+        for (int i = 0; i < 3; ++i) LV_HIP(hipStreamWaitEvent(side->s[i], side->fork, 0));
+    }
+    static constexpr int kOrder[kNumBuckets] = {0, 1, 2, 3, 4, 5, 7, kHubBucket, 6};
+    for (int k = 0; k < kNumBuckets; ++k) {
+        const int b = kOrder[k];
+        if (fork && b == 6)
+            for (int i = 0; i < 3; ++i) {
+                LV_HIP(hipEventRecord(side->join[i], side->s[i]));
+                LV_HIP(hipStreamWaitEvent(s0, side->join[i], 0));
+            }
         const uint32_t cnt = counts[b];
         if (!cnt) continue;
         const uint32_t *L = lists[b];
+        const hipStream_t s = !fork ? s0
+                              : b == 5 ? side->s[2]
+                              : b == 7 ? side->s[0]
+                              : b == kHubBucket ? side->s[1]
+                              : s0;
         m.stats = a.bstats + (size_t)b * kNumStats * kStatSlots;
         if (a.bev) (void)hipEventRecord(a.bev[2 * b], s);
         if (fast) {
