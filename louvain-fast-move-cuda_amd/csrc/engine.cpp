@@ -526,7 +526,7 @@ void launch_sweep(Community &c, const lv_params &p, int kind, bool apply, uint32
     else move_pass(a, st.hub_tables(), cur, st.count, s, st.side_streams());
     if (p.profile) LV_HIP(hipEventRecord(st.ev[1], s));
     if (apply) {
-        commit_pass(a, cur, st.count, s);
+        commit_pass(a, cur, st.count, s, st.side_streams());
         finalize_pass(a, st.lists(st.cur ^ 1), st.counts_dev.data() + kNumBuckets, s);
         LV_HIP(hipMemcpyAsync(st.host_stats + kStatWordsOffset,
                               st.counts_dev.data() + kNumBuckets, kNumBuckets * sizeof(uint32_t),

@@ -179,8 +179,9 @@ struct SideStreams {
 // stream).
 void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
                const uint32_t *counts, hipStream_t s, const SideStreams *side = nullptr);
+// side: when ready, buckets 0-5 commit on a side stream beside the large rows.
 void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *counts,
-                 hipStream_t s);
+                 hipStream_t s, const SideStreams *side = nullptr);
 // comm := target for movers, sum tot^2, compact marked vertices into next lists
 void finalize_pass(const SweepArgs &a, uint32_t *const *next_lists, uint32_t *next_counts,
                    hipStream_t s);

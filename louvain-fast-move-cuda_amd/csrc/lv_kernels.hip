@@ -1640,8 +1640,11 @@ void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
     }
 }
 
+static void commit_big_rows(const SweepArgs &a, const CommitCtx &c, uint32_t *const *lists,
+                            const uint32_t *counts, bool wtd, bool ident, hipStream_t s);
+
 void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *counts,
-                 hipStream_t s) {
+                 hipStream_t s, const SideStreams *side) {
     CommitCtx c;
     c.row = a.g.row; c.col = a.g.col; c.wt = a.g.wt; c.kdeg = a.g.kdeg;
     c.comm = a.comm; c.target = a.target; c.tot = a.tot; c.mark = a.mark; c.stats = a.stats;
@@ -1663,15 +1666,34 @@ void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *cou
         wgs += (counts[q] + gpb - 1) / gpb;
     }
     sl.wg[kSmallBuckets] = wgs;
+    // buckets 0-5 on a side stream beside the large rows (both after k_pack_ct;
+    // their writes are atomics or the same mark bytes)
+    const bool fork = wgs && side && side->ready();
+    const hipStream_t ss = fork ? side->s[0] : s;
+    if (fork) {
+        LV_HIP(hipEventRecord(side->fork, s));
+        LV_HIP(hipStreamWaitEvent(ss, side->fork, 0));
+    }
     if (wgs) {
         if (ident) {
-            if (wtd) hipLaunchKernelGGL((k_commit<true, kCommitIdent>), dim3(wgs), blk, 0, s, c, sl);
-            else hipLaunchKernelGGL((k_commit<false, kCommitIdent>), dim3(wgs), blk, 0, s, c, sl);
+            if (wtd) hipLaunchKernelGGL((k_commit<true, kCommitIdent>), dim3(wgs), blk, 0, ss, c, sl);
+            else hipLaunchKernelGGL((k_commit<false, kCommitIdent>), dim3(wgs), blk, 0, ss, c, sl);
         } else {
-            if (wtd) hipLaunchKernelGGL((k_commit<true, kCommitPacked>), dim3(wgs), blk, 0, s, c, sl);
-            else hipLaunchKernelGGL((k_commit<false, kCommitPacked>), dim3(wgs), blk, 0, s, c, sl);
+            if (wtd) hipLaunchKernelGGL((k_commit<true, kCommitPacked>), dim3(wgs), blk, 0, ss, c, sl);
+            else hipLaunchKernelGGL((k_commit<false, kCommitPacked>), dim3(wgs), blk, 0, ss, c, sl);
         }
     }
+    commit_big_rows(a, c, lists, counts, wtd, ident, s);
+    if (fork) {
+        LV_HIP(hipEventRecord(side->join[0], ss));
+        LV_HIP(hipStreamWaitEvent(s, side->join[0], 0));
+    }
+}
+
+// Large rows (buckets 6, 7, hubs): movers compacted, then the persistent commit.
+static void commit_big_rows(const SweepArgs &a, const CommitCtx &c, uint32_t *const *lists,
+                            const uint32_t *counts, bool wtd, bool ident, hipStream_t s) {
+    const dim3 blk(kBlock);
     BigLists bl;
     uint32_t listed = 0, items = 0;
     for (int q = 0; q < kBigBuckets; ++q) {
