@@ -819,9 +819,10 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
     x.tcol = ws.get<uint32_t>(W_TCOL, E);
     x.tw = ws.get<uint32_t>(W_TW, E);
     x.rcount = ws.get<uint32_t>(W_RCOUNT, n_new);
-    agg_tables(x, lp, cnt, s);
     // 4. giant communities: their entries in destination hash partitions,
-    //    each partition summed per destination in an LDS table
+    //    each partition summed per destination in an LDS table (planned on the
+    //    host first, then run on a side stream beside step 3's kernels: the
+    //    two touch disjoint communities and disjoint buffers)
     const uint32_t nh = cnt[kHubBucket];
     GiantArgs ga{};
     uint32_t n_parts = 0;
@@ -877,8 +878,18 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
         ga.base = ws.get<uint32_t>(W_GBASE, mbase);
         ga.scan_tmp = ws.get<uint32_t>(W_SCAN32, scan_scratch_elems(pbase));
         n_parts = pbase;
-        fill_u32(ga.pcnt, 0u, pbase, s);
-        agg_giant(x, ga, n_parts, s);
+    }
+    const SideStreams *side = nh ? c.state->side_streams() : nullptr;
+    if (side) {
+        LV_HIP(hipEventRecord(side->fork, s));
+        LV_HIP(hipStreamWaitEvent(side->s[0], side->fork, 0));
+        fill_u32(ga.pcnt, 0u, n_parts, side->s[0]);
+        agg_giant(x, ga, n_parts, side->s[0]);
+    }
+    agg_tables(x, lp, cnt, s);
+    if (side) {
+        LV_HIP(hipEventRecord(side->join[0], side->s[0]));
+        LV_HIP(hipStreamWaitEvent(s, side->join[0], 0));
     }
     // 5. new offsets, then every row copied to place
     uint64_t *orow = ng.neighboorhood_sum.data();
