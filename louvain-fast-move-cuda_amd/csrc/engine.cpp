@@ -71,17 +71,15 @@ void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
     if (n_hubs >= (1u << 24)) throw Error(LV_ERR_RANGE, "too many hub vertices");
     if (n_hubs > 0) {
         build_static_lists(s);
-        std::vector<uint32_t> hubs(n_hubs);
         std::vector<uint64_t> hdeg(n_hubs);
         uint64_t *hdeg_dev = ws.get<uint64_t>(kSetupSlot, n_hubs);
         gather_degrees(g.neighboorhood_sum.data(), list_buf[0].data() + bucket_off[kHubBucket],
                        n_hubs, hdeg_dev, s);
-        LV_HIP(hipMemcpyAsync(hubs.data(), list_buf[0].data() + bucket_off[kHubBucket],
-                              n_hubs * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         LV_HIP(hipMemcpyAsync(hdeg.data(), hdeg_dev, n_hubs * sizeof(uint64_t),
                               hipMemcpyDeviceToHost, s));
         wait_stream(s);
-        std::vector<uint64_t> eoff(n_hubs);
+        std::vector<uint64_t> &eoff = hub_eoff_host;   // kept: read by the async upload below
+        eoff.resize(n_hubs);
         uint64_t hub_entries = 0;
         for (uint32_t i = 0; i < n_hubs; ++i) {
             eoff[i] = hub_entries;
@@ -104,17 +102,11 @@ void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
         fill_u32(hub_self.data(), 0u, n_hubs, s);
         LV_HIP(hipMemcpyAsync(hub_ent_off.data(), eoff.data(), n_hubs * sizeof(uint64_t),
                               hipMemcpyHostToDevice, s));
-        std::vector<uint32_t> slots_host(n_hubs);
-        for (uint32_t i = 0; i < n_hubs; ++i) slots_host[i] = i;
-        // slot_of[hubs[i]] = i
-        uint32_t *hub_ids = ws.get<uint32_t>(kSetupSlot + 1, n_hubs);
+        // slot_of[hubs[i]] = i, from the hub list already on the device
         uint32_t *hub_slots = ws.get<uint32_t>(kSetupSlot + 2, n_hubs);
-        LV_HIP(hipMemcpyAsync(hub_ids, hubs.data(), n_hubs * sizeof(uint32_t),
-                              hipMemcpyHostToDevice, s));
-        LV_HIP(hipMemcpyAsync(hub_slots, slots_host.data(), n_hubs * sizeof(uint32_t),
-                              hipMemcpyHostToDevice, s));
-        scatter_u32(hub_slot_of.data(), hub_ids, hub_slots, n_hubs, s);
-        wait_stream(s);
+        iota_u32(hub_slots, n_hubs, s);
+        scatter_u32(hub_slot_of.data(), list_buf[0].data() + bucket_off[kHubBucket], hub_slots,
+                    n_hubs, s);
     }
 }
 
@@ -835,8 +827,12 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
         gather_degrees(crow, lp[kHubBucket], nh, gsize_dev, s);
         LV_HIP(hipMemcpyAsync(gsize.data(), gsize_dev, nh * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
         wait_stream(s);
-        std::vector<GiantInfo> info(nh);
-        std::vector<uint64_t> tiles, items;
+        // kept in the state: read by the async uploads below
+        std::vector<GiantInfo> &info = c.state->giant_info;
+        std::vector<uint64_t> &tiles = c.state->giant_tiles, &items = c.state->giant_items;
+        info.resize(nh);
+        tiles.clear();
+        items.clear();
         uint64_t off = 0, mbase = 0;
         uint32_t pbase = 0;
         for (uint32_t i = 0; i < nh; ++i) {
@@ -863,7 +859,6 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
                               hipMemcpyHostToDevice, s));
         LV_HIP(hipMemcpyAsync(items_dev, items.data(), items.size() * sizeof(uint64_t),
                               hipMemcpyHostToDevice, s));
-        wait_stream(s);   // the plan's host vectors end with this block
         ga.info = info_dev;
         ga.tiles = tiles_dev;
         ga.n_tiles = (uint32_t)tiles.size();
@@ -905,7 +900,6 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
     agg_copy(x, lp, cnt, orow, ng.edge_destination.data(), ng.weights.data(), s);
     agg_giant_copy(x, ga, n_parts, orow, ng.edge_destination.data(), ng.weights.data(), s);
     new_degrees(c.communities_weight.data(), map, n, ng.tot_weight_per_nodes.data(), s);
-    wait_stream(s);
     ng.n_entries = e_new;
     ng.n_links = e_new / 2;                                           // :97
     c.graph = GraphDevice::view_of(ng);

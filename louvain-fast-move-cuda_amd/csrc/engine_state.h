@@ -104,6 +104,10 @@ struct SweepState {
     DeviceArray<int64_t> hub_part_s;
     DeviceArray<uint32_t> hub_part_c;
     DeviceArray<uint32_t> hub_self;
+    std::vector<uint64_t> hub_eoff_host;        // source of the async hub_ent_off upload
+    // host plan of the last contraction's giant communities (sources of async uploads)
+    std::vector<GiantInfo> giant_info;
+    std::vector<uint64_t> giant_tiles, giant_items;
     Workspace ws;
     // chunk emulation: level the bounds were computed for, chunk start vertices
     ChunkKey chunk_key;
