@@ -1093,7 +1093,10 @@ __device__ inline int64_t commit_row(const CommitCtx &a, uint32_t v, uint32_t ow
             if (xs[u] == v) continue;
             const int d = (int)(nx[u] == c) - (int)(ox[u] == own);
             if (d) dsig += (int64_t)ws[u] * d * (ox[u] != nx[u] ? 1 : 2);
-            if (nx[u] != c) a.mark[xs[u]] = 1;
+            // most neighbours are marked by several movers: store only the
+            // first mark, so lines of `mark` are not dirtied and written back
+            // again and again
+            if (nx[u] != c && !a.mark[xs[u]]) a.mark[xs[u]] = 1;
         }
     }
     return dsig;
