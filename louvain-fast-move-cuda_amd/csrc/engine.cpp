@@ -834,13 +834,14 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
         tiles.clear();
         items.clear();
         uint64_t off = 0, mbase = 0;
-        uint32_t pbase = 0;
+        uint32_t pbase = 0, max_parts = 1;
         for (uint32_t i = 0; i < nh; ++i) {
             const uint64_t e = gsize[i];
             uint32_t lgp = 0;
             while (lgp < (uint32_t)kGiantMaxPartBits && (kGiantPartEntries << lgp) < e) ++lgp;
             const uint32_t lgt = std::max<uint32_t>(kGiantMinTileBits, lgp + 4);
             const uint32_t np = 1u << lgp;
+            max_parts = std::max(max_parts, np);
             const uint64_t nt = (e + (1ull << lgt) - 1) >> lgt;
             info[i] = GiantInfo{off, mbase, giants[i], pbase, lgp, lgt};
             for (uint64_t t = 0; t < nt; ++t) tiles.push_back(((uint64_t)i << 32) | t);
@@ -872,6 +873,7 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
         ga.doff = ws.get<uint32_t>(W_GDOFF, pbase);
         ga.base = ws.get<uint32_t>(W_GBASE, mbase);
         ga.scan_tmp = ws.get<uint32_t>(W_SCAN32, scan_scratch_elems(pbase));
+        ga.max_parts = max_parts;
         n_parts = pbase;
     }
     const SideStreams *side = nh ? c.state->side_streams() : nullptr;

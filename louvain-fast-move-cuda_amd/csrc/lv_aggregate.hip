@@ -178,7 +178,6 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
 // k_giant_copy puts them at the community's final row.
 
 constexpr int kGiantIlp = 8;
-constexpr uint32_t kGiantMaxParts = 1u << kGiantMaxPartBits;
 constexpr uint32_t kGiantCache = 1024;                 // destinations cached per tile
 constexpr unsigned long long kGiantHole = ~0ull;       // entry folded into a cached one
 
@@ -192,7 +191,7 @@ __device__ inline uint32_t giant_part(uint32_t d, uint32_t lgp) {
 
 template <bool WEIGHTED>
 __global__ __launch_bounds__(kBlock) void k_giant_gather(GiantCtx h) {
-    __shared__ uint32_t hist[kGiantMaxParts];
+    extern __shared__ uint32_t hist[];   // [h.max_parts]
     __shared__ uint64_t s_mrow[kBlock];
     __shared__ uint64_t s_mp[kBlock + 1];
     __shared__ uint32_t s_first;
@@ -288,8 +287,9 @@ __global__ __launch_bounds__(kBlock) void k_giant_gather(GiantCtx h) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_giant_scatter(GiantCtx h) {
-    __shared__ uint32_t s_off[kGiantMaxParts];
-    __shared__ uint32_t s_rank[kGiantMaxParts];
+    extern __shared__ uint32_t s_dyn[];   // [2 * h.max_parts]
+    uint32_t *s_off = s_dyn;
+    uint32_t *s_rank = s_dyn + h.max_parts;
     const uint64_t tile = h.tiles[blockIdx.x];
     const uint32_t lt = (uint32_t)tile;
     const GiantInfo I = h.info[tile >> 32];
@@ -541,10 +541,12 @@ static GiantCtx giant_ctx(const AggArgs &x, const GiantArgs &g) {
 void agg_giant(const AggArgs &x, const GiantArgs &g, uint32_t n_parts, hipStream_t s) {
     if (!g.n_tiles) return;
     const GiantCtx h = giant_ctx(x, g);
-    if (x.g.wt) hipLaunchKernelGGL(k_giant_gather<true>, dim3(g.n_tiles), dim3(kBlock), 0, s, h);
-    else hipLaunchKernelGGL(k_giant_gather<false>, dim3(g.n_tiles), dim3(kBlock), 0, s, h);
+    // the per-partition LDS arrays are as large as the plan's largest community needs
+    const size_t lds1 = (size_t)g.max_parts * sizeof(uint32_t);
+    if (x.g.wt) hipLaunchKernelGGL(k_giant_gather<true>, dim3(g.n_tiles), dim3(kBlock), lds1, s, h);
+    else hipLaunchKernelGGL(k_giant_gather<false>, dim3(g.n_tiles), dim3(kBlock), lds1, s, h);
     scan_u32(g.pcnt, g.poff, n_parts, false, g.scan_tmp, s);
-    hipLaunchKernelGGL(k_giant_scatter, dim3(g.n_tiles), dim3(kBlock), 0, s, h);
+    hipLaunchKernelGGL(k_giant_scatter, dim3(g.n_tiles), dim3(kBlock), 2 * lds1, s, h);
     hipLaunchKernelGGL(k_giant_reduce, dim3(g.n_items), dim3(kBlock), 0, s, h);
     scan_u32(g.dcnt, g.doff, n_parts, false, g.scan_tmp, s);
 }

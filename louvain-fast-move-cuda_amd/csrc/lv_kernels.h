@@ -253,6 +253,7 @@ struct GiantArgs {
     uint32_t *doff;                 // [partitions] exclusive scan of dcnt
     uint32_t *base;                 // [(tile, partition) runs] start of a tile's run
     uint32_t *scan_tmp;             // scan scratch for the partition counts
+    uint32_t max_parts;             // largest partition count of the plan (LDS per workgroup)
 };
 // distinct (destination, weight) pairs of every giant community and its
 // rcount; then, once the new offsets are known, its row copied to place
