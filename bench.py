@@ -168,23 +168,38 @@ def batched_result(args, L):
 
 def served_result(args, L):
     """Poisson arrivals (rate args.rate for args.window s, seed 5) served with
-    continuous batching under an HBM budget; 95% planted-partition graphs
-    (the planted configuration's degree and community-size laws at
-    args.served_n vertices, weights 1..10), 5% R-MAT scale 20.  Requests draw
-    from a pool of distinct seeded graphs; every request is its own job (its
-    own upload, device state and result)."""
+    continuous batching under an HBM budget; 95% of the requests draw from the
+    batched configuration's size/degree distribution (Barabasi-Albert, n
+    log-uniform in [200, 200000], m uniform in {2..8}, unit weights; or, with
+    --served-mix planted, planted-partition graphs with the planted
+    configuration's laws at args.served_n vertices, weights 1..10), 5% are
+    R-MAT scale 20.  Requests draw from a pool of distinct seeded graphs; every
+    request is its own job (its own upload, device state and result)."""
     import numpy as np
     t = time.perf_counter()
     pool_p = []
-    for i in range(args.pool):
-        ea, eb, ew, _ = L.gen_planted(n=args.served_n, n_comm=max(2, args.served_n // 500),
-                                      seed=5000 + i)
-        pool_p.append(L.Graph.from_edges(ea, eb, ew))
+    if args.served_mix == "ba":
+        rng_p = np.random.default_rng(55)
+        ns = np.exp(rng_p.uniform(np.log(200), np.log(200_000), args.pool)).astype(np.int64)
+        ms = rng_p.integers(2, 9, args.pool)
+        for i in range(args.pool):
+            ea, eb = L.gen_ba(int(ns[i]), int(ms[i]), seed=5_000_000 + i)
+            pool_p.append(L.Graph.from_edges(ea, eb))
+        mix_desc = ("Barabasi-Albert graphs of the batched configuration's distribution (n "
+                    "log-uniform in [200, 200000], m uniform in {2..8}, unit weights)")
+    else:
+        for i in range(args.pool):
+            ea, eb, ew, _ = L.gen_planted(n=args.served_n, n_comm=max(2, args.served_n // 500),
+                                          seed=5000 + i)
+            pool_p.append(L.Graph.from_edges(ea, eb, ew))
+        mix_desc = (f"planted-partition graphs (n={args.served_n}, degree exponent 2.5, mean 20, "
+                    f"max 5000, community sizes exponent 1.5 in [50, 5000], mixing 0.3, weights "
+                    f"1..10)")
     pool_r = []
-    for i in range(max(1, args.pool // 4)):
+    for i in range(4):
         ea, eb, _ = L.gen_rmat(20, 16, 0.57, 0.19, 0.19, seed=500 + i)
         pool_r.append(L.Graph.from_edges(ea, eb))
-    log(f"served pool: {len(pool_p)} planted (n={args.served_n}, mean E="
+    log(f"served pool: {len(pool_p)} {args.served_mix} (mean E="
         f"{np.mean([g.E for g in pool_p]):.0f}), {len(pool_r)} R-MAT-20 (mean E="
         f"{np.mean([g.E for g in pool_r]):.0f}) ({time.perf_counter() - t:.1f}s)")
     rng = np.random.default_rng(5)
@@ -263,16 +278,14 @@ def served_result(args, L):
     S.close()
     return {
         "config": {"workload": f"served Louvain: Poisson arrivals at {args.rate:g} req/s for "
-                               f"{args.window:g} s (seed 5); 95% planted-partition graphs "
-                               f"(n={args.served_n}, degree exponent 2.5, mean 20, max 5000, "
-                               f"community sizes exponent 1.5 in [50, 5000], mixing 0.3, weights "
-                               f"1..10), 5% R-MAT scale 20; HBM budget {args.hbm_budget_gb:g} GB",
-                   "requests": int(len(arr)), "pool_planted": len(pool_p),
+                               f"{args.window:g} s (seed 5); 95% {mix_desc}, 5% R-MAT scale 20; "
+                               f"HBM budget {args.hbm_budget_gb:g} GB",
+                   "requests": int(len(arr)), f"pool_{args.served_mix}": len(pool_p),
                    "pool_rmat20": len(pool_r)},
         "graphs_per_s": len(arr) / elapsed,
         "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),
-        "p50_latency_planted_s": float(np.percentile(lat[~kind_of], 50)),
+        f"p50_latency_{args.served_mix}_s": float(np.percentile(lat[~kind_of], 50)),
         "p99_latency_rmat20_s": float(np.percentile(lat[kind_of], 99)) if kind_of.any() else None,
         "elapsed_s": elapsed,
         "union_levels": steps,
@@ -393,9 +406,12 @@ def main():
                     help="1: tune the best-move tiles during the warm-up run")
     ap.add_argument("--rate", type=float, default=200.0, help="served: arrivals per second")
     ap.add_argument("--window", type=float, default=60.0, help="served: arrival window (s)")
+    ap.add_argument("--served-mix", default="ba", choices=["ba", "planted"],
+                    help="served: the 95% requests (ba: the batched configuration's "
+                         "distribution; planted: planted-partition graphs of --served-n vertices)")
     ap.add_argument("--served-n", type=int, default=1_000_000,
                     help="served: vertices of the planted-partition requests")
-    ap.add_argument("--pool", type=int, default=16, help="served: distinct planted graphs")
+    ap.add_argument("--pool", type=int, default=64, help="served: distinct graphs of the 95% mix")
     ap.add_argument("--burst", type=int, default=400, help="served: saturation burst size")
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-batched", action="store_true",
