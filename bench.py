@@ -452,7 +452,7 @@ def main():
     c = L.Community(g)
     log(f"upload + level setup ({time.perf_counter() - t:.2f}s)")
     prm = L.params(mode=args.mode, sweep_threshold=args.threshold,
-                   level_threshold=args.threshold, profile=1, autotune=args.autotune)
+                   level_threshold=args.threshold, profile=2, autotune=args.autotune)
     for i in range(args.warmup):
         c.reset()
         r = c.run(prm, best=False, n0=info["n"])

@@ -58,7 +58,10 @@ typedef struct lv_params {
     double level_threshold;   /* ... and for the level loop (modularityAlgorithms.h:52) */
     int64_t max_sweeps;       /* EARLY_STOP_LIMIT (constants.h:16) */
     int32_t max_levels;       /* <= 0: no limit */
-    int32_t profile;          /* 1: time every best-move launch with HIP events */
+    int32_t profile;          /* 1: time every best-move launch with HIP events;
+                                 2: time the sweeps, the best-move passes and the
+                                    degree 257-1024 bucket's launches only (the one
+                                    that runs with the device to itself) */
     int32_t autotune;         /* 1: on a community's first fast and first hash sweep, time
                                  each degree bucket's tile candidates and keep the fastest
                                  (decisions, hence results, do not depend on the tile) */
@@ -98,17 +101,19 @@ typedef struct lv_result {
 } lv_result;
 
 typedef struct lv_kernel_stats {
-    /* best-move kernels (all degree buckets), summed over the launches timed
-     * when lv_params.profile = 1 */
+    /* best-move passes (all degree buckets), summed over the sweeps timed
+     * when lv_params.profile is 1 or 2 */
     uint64_t move_launches;
-    double move_ms;             /* sum of HIP-event durations on the library stream */
+    double move_ms;             /* HIP-event time of the passes on the library stream */
     uint64_t move_bytes;        /* algorithmic bytes of those launches */
     uint64_t move_entries;      /* entries traversed by those launches */
     double sweep_ms;            /* whole sweeps (move + commit + modularity), event timed */
     uint64_t sweeps;
     /* per best-move kernel variant v = ((bucket*2 + fast)*2 + weighted)*3 + tile,
      * bucket 0-7, tile = index of the bucket's tile candidate (one kernel launch
-     * each; lv_kernel_variant_name gives the kernel name) */
+     * each; lv_kernel_variant_name gives the kernel name).  Buckets 5 and 7
+     * share the device with others while they run, so their times include
+     * that; with profile = 2 only bucket 6's variants are recorded. */
     uint64_t variant_launches[96];
     double variant_ms[96];
     uint64_t variant_bytes[96];

@@ -445,6 +445,10 @@ namespace lv {
 // (phaseOptimizationSort.cu:46-60, :81-86, :186-204, :235-240).
 uint32_t chunk_vertex_end(Community &c, const lv_params &p, int kind, const uint8_t *active_host);
 
+// profile = 2 times only the degree 257-1024 bucket (bucket 6), the one the
+// best-move pass runs with the device to itself.
+constexpr uint32_t kProfileLightMask = 1u << 6;
+
 static SweepArgs sweep_args(Community &c, const lv_params &p, int kind, uint32_t v_end) {
     SweepState &st = *c.state;
     SweepArgs a;
@@ -457,6 +461,7 @@ static SweepArgs sweep_args(Community &c, const lv_params &p, int kind, uint32_t
     a.stats = st.stats.data() + kGlobalSection;
     a.bstats = st.stats.data();
     a.bev = (p.profile && kind != LV_SWEEP_SORT) ? st.bev : nullptr;
+    a.bev_mask = p.profile == 2 ? kProfileLightMask : ~0u;
     a.m2 = (int64_t)(2.0 * c.graph.total_weight);
     a.tw = c.graph.total_weight;
     a.v_end = v_end;
@@ -588,6 +593,7 @@ int OptimizationPhase::sweep_loop(Community &c, const lv_params &p, int64_t exec
                 const uint64_t bb = bytes_of(be, bc, verts, bt);
                 bytes += bb;
                 if (b == kHubBucket || kind == LV_SWEEP_SORT) continue;
+                if (p.profile == 2 && !(kProfileLightMask >> b & 1u)) continue;
                 float ms_b = 0;
                 LV_HIP(hipEventElapsedTime(&ms_b, st.bev[2 * b], st.bev[2 * b + 1]));
                 const int vi = move_variant(b, fast, wtd, st.tiles.sel[b][fast ? 1 : 0]);

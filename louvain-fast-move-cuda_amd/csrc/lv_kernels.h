@@ -98,6 +98,7 @@ struct SweepArgs {
     unsigned long long *stats;   // [kNumStats * kStatSlots] counters of commit/finalize
     unsigned long long *bstats;  // [kNumBuckets * kNumStats * kStatSlots] best-move counters per bucket
     hipEvent_t *bev;             // nullptr, or 2 events per bucket around its best-move launch
+    uint32_t bev_mask = ~0u;     // buckets whose launches bev times
     int64_t m2;                  // 2m as an integer
     double tw;                   // the reference's total_weight (m)
     uint32_t v_end;              // vertices >= v_end are not examined (chunk emulation)

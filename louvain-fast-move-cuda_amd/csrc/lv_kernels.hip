@@ -1589,7 +1589,8 @@ void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
                               : b == kHubBucket ? side->s[1]
                               : s0;
         m.stats = a.bstats + (size_t)b * kNumStats * kStatSlots;
-        if (a.bev) (void)hipEventRecord(a.bev[2 * b], s);
+        const bool timed = a.bev && (a.bev_mask >> b & 1u);
+        if (timed) (void)hipEventRecord(a.bev[2 * b], s);
         if (fast) {
             switch (b) {
                 case 0: case 1: case 2: case 3: case 4: case 5: case 6: case 7:
@@ -1639,7 +1640,7 @@ void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
                 }
             }
         }
-        if (a.bev) (void)hipEventRecord(a.bev[2 * b + 1], s);
+        if (timed) (void)hipEventRecord(a.bev[2 * b + 1], s);
     }
 }
 
