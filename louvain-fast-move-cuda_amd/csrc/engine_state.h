@@ -119,7 +119,7 @@ struct SweepState {
     // tiles of the best-move kernels (tuned once per community when asked)
     TileChoice tiles;
     bool tuned[2] = {false, false};   // [fast sweep]
-    // side streams of the best-move pass (created on first use)
+    // side streams of the sweep passes and the contraction
     SideStreams side;
     const SideStreams *side_streams();
     // profiling
@@ -130,6 +130,7 @@ struct SweepState {
     SweepState() {
         for (auto &e : ev) LV_HIP(hipEventCreate(&e));
         for (auto &e : bev) LV_HIP(hipEventCreate(&e));
+        (void)side_streams();   // created with the community, not inside a timed run
     }
     ~SweepState();
     SweepState(const SweepState &) = delete;
