@@ -487,6 +487,7 @@ void autotune_tiles(Community &c, const lv_params &p, int kind, uint32_t v_end) 
     hipStream_t s = c.stream();
     const SweepArgs a = sweep_args(c, p, kind, v_end);   // the sweep's own vertex bound
     uint32_t *const *L = st.lists(st.cur);
+    if (kind == LV_SWEEP_HASH) pack_comm_tot(a, s);
     // best-move launches only write the decisions, the same ones each time
     for (int b = 0; b < kNumBuckets - 1; ++b) {
         const uint32_t cnt = st.count[b];

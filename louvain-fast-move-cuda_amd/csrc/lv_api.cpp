@@ -498,14 +498,15 @@ lv_status lv_bandwidth_probe(uint64_t bytes, int reps, double *gbs) {
 
 lv_status lv_kernel_variant_name(int variant, int gain, char *buf, size_t len) {
     if (variant < 0 || variant >= lv::kMoveVariants || !buf || len == 0) return LV_ERR_ARG;
-    static const int T[8] = {8, 16, 32, 64, 128, 512, 2048, 8192};
+    const int *T = lv::kTableT;
     const int sel = variant % lv::kTileChoices, rest = variant / lv::kTileChoices;
     const int b = rest / 4, fast = (rest / 2) % 2, wtd = rest % 2;
     const int G = lv::kTileG[b][sel];
     if (G == 0) return LV_ERR_ARG;
     const char *w = wtd ? "true" : "false";
     if (fast) std::snprintf(buf, len, "k_move_fast<%d, %d, %s>", G, gain, w);
-    else std::snprintf(buf, len, "k_move_table<%d, %d, %d, %s>", G, T[b], gain, w);
+    else std::snprintf(buf, len, "k_move_table<%d, %d, %d, %s, %s>", G, T[b], gain, w,
+                       T[b] <= lv::kPackedMaxT ? "true" : "false");
     return LV_OK;
 }
 

@@ -133,6 +133,12 @@ __device__ inline uint32_t slot_hash_dyn(uint32_t c, uint32_t log2t) {
     return (c * 2654435761u) >> (32 - log2t);
 }
 
+// Slot in [0, t) of key c for tables of any size t (multiplicative hash,
+// scaled by a high multiply).
+__device__ inline uint32_t slot_hash_mod(uint32_t c, uint32_t t) {
+    return (uint32_t)(((uint64_t)(c * 2654435761u) * t) >> 32);
+}
+
 constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
 
 // ---- LDS sums of (key << 32 | weight) pairs, range by range of a key mix ----

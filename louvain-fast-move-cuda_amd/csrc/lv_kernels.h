@@ -27,10 +27,27 @@ constexpr uint32_t kMaxChunks = 4096; // reference chunks per level (STEP_ROUND 
 // Maximum degree handled by each LDS bucket; larger goes to the hub bucket.
 constexpr uint32_t kBucketMaxDeg[kNumBuckets - 1] = {4, 8, 16, 32, 64, 256, 1024, 4096};
 
+// Slots of a best-move table for a row of d entries: d + d/2 + 1 rounded up
+// to a multiple of 4, so the table is at most two thirds full and always has
+// an empty slot (a lookup of a missing key ends).  A bucket's LDS table has
+// the slots of its maximum degree; the LDS per vertex bounds how many
+// vertices a CU holds at once, which bounds these latency-bound kernels.
+constexpr uint32_t table_slots(uint64_t d) { return (uint32_t)((d + d / 2 + 1 + 3) & ~3ull); }
+constexpr int kTableT[kNumBuckets - 1] = {
+    (int)table_slots(kBucketMaxDeg[0]), (int)table_slots(kBucketMaxDeg[1]),
+    (int)table_slots(kBucketMaxDeg[2]), (int)table_slots(kBucketMaxDeg[3]),
+    (int)table_slots(kBucketMaxDeg[4]), (int)table_slots(kBucketMaxDeg[5]),
+    (int)table_slots(kBucketMaxDeg[6]), (int)table_slots(kBucketMaxDeg[7])};
+static_assert(kTableT[0] == 8 && kTableT[5] == 388 && kTableT[6] == 1540 && kTableT[7] == 6148,
+              "table sizes");
+
 // Tile (lanes per vertex) candidates of each LDS bucket's best-move kernels;
 // the first is the default, 0 ends a list.  The LDS table of a bucket stays
-// 2 x its maximum degree whatever the tile.
+// kTableT whatever the tile.
 constexpr int kTileChoices = 3;
+// best-move tables of at most this many slots carry each community's total
+// (PK); bucket 7's would need 72 KB of LDS per vertex packed, 48 KB unpacked
+constexpr int kPackedMaxT = kTableT[6];
 constexpr int kTileG[kNumBuckets - 1][kTileChoices] = {
     {4, 2, 8}, {8, 4, 16}, {16, 8, 32}, {32, 16, 64}, {64, 32, 0}, {64, 256, 0}, {256, 0, 0},
     {256, 0, 0}};
@@ -108,9 +125,15 @@ struct SweepArgs {
     TileChoice tiles;            // best-move kernel tile per bucket
     uint32_t *movers;            // [n] commit scratch: movers of the large-row buckets
     uint32_t *n_movers;          // [kBigBuckets] their counts
-    unsigned long long *ct;      // [n] commit scratch: comm | target << 32
+    unsigned long long *ct;      // [n] scratch: comm | tot[comm] << 32 during a hash sweep's
+                                 // best-move pass, then comm | target << 32 for its commit
     bool counters_zeroed = false; // n_movers and the next-list counts were zeroed by sweep_reset
 };
+
+// ct <- comm | tot[comm] << 32 of every vertex: the packed operand of the hash
+// sweep's table kernels.  move_pass builds it itself; the tile tuner calls it
+// before timing move_bucket on a hash sweep.
+void pack_comm_tot(const SweepArgs &a, hipStream_t s);
 
 // One LDS bucket's best-move launch with tile candidate `sel` (the sweep kind
 // and the bucket's list as in move_pass); used by the tile tuner.

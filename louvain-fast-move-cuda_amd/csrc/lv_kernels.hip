@@ -427,6 +427,7 @@ struct MoveCtx {
     const uint32_t *kdeg;
     const uint32_t *comm;
     const uint32_t *tot;
+    const unsigned long long *ctot;   // comm | tot[comm] << 32 of every vertex (hash sweeps)
     uint32_t *target;
     unsigned long long *stats;
     int64_t m2;
@@ -435,8 +436,15 @@ struct MoveCtx {
     bool highest;
 };
 
-// Group-per-vertex with an LDS table of T slots (T >= 2 * max degree).
-template <int G, int T, int GAIN, bool WEIGHTED>
+// Group-per-vertex with an LDS table of T slots (T >= table_slots of the
+// bucket's maximum degree); a vertex uses the first table_slots(degree).
+// PK: a neighbour's community and that community's total come in one 8-byte
+// gather of ctot (comm | tot << 32), and the first lane to insert a community
+// keeps its total in a third LDS array; the candidates are then scored from
+// LDS alone.  Without PK the totals are gathered once per candidate (one more
+// dependent random access per distinct community, for 4 bytes less LDS per
+// slot).
+template <int G, int T, int GAIN, bool WEIGHTED, bool PK>
 __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t *__restrict__ list,
                                                        uint32_t count) {
     using SC = Score<GAIN>;
@@ -444,6 +452,7 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
     constexpr int GPB = kBlock / G;
     __shared__ uint32_t s_key[GPB * T];
     __shared__ uint32_t s_val[GPB * T];
+    __shared__ uint32_t s_tot[PK ? GPB * T : 1];
     const int lane = threadIdx.x & (G - 1);
     const int gi = threadIdx.x / G;
     const uint32_t idx = blockIdx.x * GPB + gi;
@@ -454,23 +463,33 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
     const bool live = valid && v < a.v_end && vertex_graph(a.bt, v, a.m2, a.tw, m2, tw);
     uint32_t *K = s_key + gi * T;
     uint32_t *V = s_val + gi * T;
-    // the vertex's table: the smallest power of two >= 2*degree (<= T)
-    uint32_t own = 0, lg = 1;
+    uint32_t *TT = s_tot + (PK ? gi * T : 0);
+    // the vertex's table: table_slots(degree) <= T slots.  The vertex's own
+    // words are loaded here, beside its row bounds, so that their latency
+    // overlaps the row's instead of following the table's barrier.
+    uint32_t own = 0, tv = 4, kv = 0, tot_own = 0;
     uint64_t beg = 0, end = 0;
     if (live) {
-        own = a.comm[v];
+        if constexpr (PK) {
+            const unsigned long long p = a.ctot[v];
+            own = (uint32_t)p;
+            tot_own = (uint32_t)(p >> 32);
+        } else {
+            own = a.comm[v];
+        }
+        kv = a.kdeg[v];
         beg = a.row[v];
         end = a.row[v + 1];
-        lg = table_log2(end - beg);
+        tv = table_slots(end - beg);
+        if constexpr (!PK) tot_own = a.tot[own];
     }
-    const uint32_t tv = 1u << lg, mask = tv - 1u;
     for (uint32_t s = lane; s < tv; s += G) { K[s] = LV_EMPTY; V[s] = 0u; }
     __syncthreads();
     uint64_t nself = 0;
     if (live) {
         // kIlp entries per lane in flight: loads first, then gathers, then inserts
         for (uint64_t jb = beg + lane; jb < end; jb += (uint64_t)G * kIlp) {
-            uint32_t xs[kIlp], ws[kIlp], cs[kIlp];
+            uint32_t xs[kIlp], ws[kIlp], cs[kIlp], ts[kIlp];
 #pragma unroll
             for (int u = 0; u < kIlp; ++u) {
                 const uint64_t j = jb + (uint64_t)u * G;
@@ -478,18 +497,38 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
                 ws[u] = (WEIGHTED && j < end) ? stream_load(&a.wt[j]) : 1u;
             }
 #pragma unroll
-            for (int u = 0; u < kIlp; ++u) cs[u] = xs[u] != v ? a.comm[xs[u]] : 0u;
+            for (int u = 0; u < kIlp; ++u) {
+                if constexpr (PK) {
+                    const unsigned long long p = xs[u] != v ? a.ctot[xs[u]] : 0ull;
+                    cs[u] = (uint32_t)p;
+                    ts[u] = (uint32_t)(p >> 32);
+                } else {
+                    cs[u] = xs[u] != v ? a.comm[xs[u]] : 0u;
+                    ts[u] = 0u;
+                }
+            }
+            // the first probe of all kIlp entries is issued back to back (one
+            // LDS round trip instead of kIlp); a lane's own entries are still
+            // inserted in order, so an entry sees the keys its predecessors
+            // placed, and the rare occupied first slots probe on afterwards
+            uint32_t hs[kIlp], pv[kIlp];
+#pragma unroll
+            for (int u = 0; u < kIlp; ++u) {
+                hs[u] = slot_hash_mod(cs[u], tv);
+                pv[u] = xs[u] != v ? atomicCAS(&K[hs[u]], LV_EMPTY, cs[u]) : LV_EMPTY;
+            }
 #pragma unroll
             for (int u = 0; u < kIlp; ++u) {
                 if (xs[u] == v) continue;
                 ++nself;
                 const uint32_t c = cs[u];
-                uint32_t h = slot_hash_dyn(c, lg);
-                for (;;) {
-                    const uint32_t prev = atomicCAS(&K[h], LV_EMPTY, c);
-                    if (prev == LV_EMPTY || prev == c) { atomicAdd(&V[h], ws[u]); break; }
-                    h = (h + 1) & mask;
+                uint32_t h = hs[u], prev = pv[u];
+                while (prev != LV_EMPTY && prev != c) {
+                    h = h + 1 == tv ? 0u : h + 1;
+                    prev = atomicCAS(&K[h], LV_EMPTY, c);
                 }
+                if (PK && prev == LV_EMPTY) TT[h] = ts[u];
+                atomicAdd(&V[h], ws[u]);
             }
         }
     }
@@ -497,17 +536,15 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
     S best = SC::none();
     uint32_t best_c = a.highest ? 0u : LV_EMPTY;
     uint64_t ncand = 0;
-    uint32_t self = 0, kv = 0, tot_own = 0;
+    uint32_t self = 0;
     if (live) {
-        uint32_t h = slot_hash_dyn(own, lg);
+        uint32_t h = slot_hash_mod(own, tv);
         for (;;) {
             const uint32_t c = K[h];
             if (c == own) { self = V[h]; break; }
             if (c == LV_EMPTY) break;
-            h = (h + 1) & mask;
+            h = h + 1 == tv ? 0u : h + 1;
         }
-        kv = a.kdeg[v];
-        tot_own = a.tot[own];
         for (uint32_t sb = lane; sb < tv; sb += (uint32_t)G * kIlp) {
             uint32_t cs[kIlp], Ws[kIlp], ts[kIlp];
 #pragma unroll
@@ -518,7 +555,11 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
                 Ws[u] = cs[u] != LV_EMPTY ? V[s2] : 0u;
             }
 #pragma unroll
-            for (int u = 0; u < kIlp; ++u) ts[u] = cs[u] != LV_EMPTY ? a.tot[cs[u]] : 0u;
+            for (int u = 0; u < kIlp; ++u) {
+                const uint32_t s2 = sb + (uint32_t)u * G;
+                if constexpr (PK) ts[u] = cs[u] != LV_EMPTY ? TT[s2] : 0u;
+                else ts[u] = cs[u] != LV_EMPTY ? a.tot[cs[u]] : 0u;
+            }
 #pragma unroll
             for (int u = 0; u < kIlp; ++u) {
                 if (cs[u] == LV_EMPTY) continue;
@@ -1048,6 +1089,17 @@ __global__ void k_pack_ct(const uint32_t *comm, const uint32_t *target, uint32_t
         ct[v] = (unsigned long long)comm[v] | ((unsigned long long)target[v] << 32);
 }
 
+// ctot[v] = comm[v] | tot[comm[v]] << 32: the operand of the packed best-move
+// tables, built at the start of every hash sweep's best-move pass.
+__global__ void k_pack_ctot(const uint32_t *comm, const uint32_t *tot, uint32_t n,
+                            unsigned long long *ctot) {
+    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n;
+         v += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t c = comm[v];
+        ctot[v] = (unsigned long long)c | ((unsigned long long)tot[c] << 32);
+    }
+}
+
 // Commit of mover v (target != comm) over the entries part, part + stride, ...
 // of its row in steps of `stride` lanes: tot moves with v's degree (lane 0 of
 // part 0), neighbours left in another community are marked, and the change of
@@ -1479,17 +1531,20 @@ void modularity_parts(const LevelView &g, const uint32_t *comm, const uint32_t *
 
 namespace {
 
+// Tables of up to kPackedMaxT slots are packed; bucket 7's keeps the
+// per-candidate total gathers rather than 50% more LDS per vertex.
 template <int G, int T>
 void launch_move_table(const MoveCtx &m, bool weighted, int gain, const uint32_t *list,
                        uint32_t count, hipStream_t s) {
     constexpr int GPB = kBlock / G;
+    constexpr bool PK = T <= kPackedMaxT;
     const dim3 grid((count + GPB - 1) / GPB), blk(kBlock);
     if (gain == 0) {
-        if (weighted) hipLaunchKernelGGL((k_move_table<G, T, 0, true>), grid, blk, 0, s, m, list, count);
-        else hipLaunchKernelGGL((k_move_table<G, T, 0, false>), grid, blk, 0, s, m, list, count);
+        if (weighted) hipLaunchKernelGGL((k_move_table<G, T, 0, true, PK>), grid, blk, 0, s, m, list, count);
+        else hipLaunchKernelGGL((k_move_table<G, T, 0, false, PK>), grid, blk, 0, s, m, list, count);
     } else {
-        if (weighted) hipLaunchKernelGGL((k_move_table<G, T, 1, true>), grid, blk, 0, s, m, list, count);
-        else hipLaunchKernelGGL((k_move_table<G, T, 1, false>), grid, blk, 0, s, m, list, count);
+        if (weighted) hipLaunchKernelGGL((k_move_table<G, T, 1, true, PK>), grid, blk, 0, s, m, list, count);
+        else hipLaunchKernelGGL((k_move_table<G, T, 1, false, PK>), grid, blk, 0, s, m, list, count);
     }
 }
 
@@ -1512,14 +1567,14 @@ void launch_table_tile(int b, int G, const MoveCtx &m, bool wtd, int gain, const
                        uint32_t cnt, hipStream_t s) {
 #define LV_T(GG, TT) launch_move_table<GG, TT>(m, wtd, gain, L, cnt, s)
     switch (b) {
-        case 0: if (G == 2) LV_T(2, 8); else if (G == 8) LV_T(8, 8); else LV_T(4, 8); break;
-        case 1: if (G == 4) LV_T(4, 16); else if (G == 16) LV_T(16, 16); else LV_T(8, 16); break;
-        case 2: if (G == 8) LV_T(8, 32); else if (G == 32) LV_T(32, 32); else LV_T(16, 32); break;
-        case 3: if (G == 16) LV_T(16, 64); else if (G == 64) LV_T(64, 64); else LV_T(32, 64); break;
-        case 4: if (G == 32) LV_T(32, 128); else LV_T(64, 128); break;
-        case 5: if (G == 256) LV_T(256, 512); else LV_T(64, 512); break;
-        case 6: LV_T(256, 2048); break;
-        default: LV_T(256, 8192); break;
+        case 0: if (G == 2) LV_T(2, kTableT[0]); else if (G == 8) LV_T(8, kTableT[0]); else LV_T(4, kTableT[0]); break;
+        case 1: if (G == 4) LV_T(4, kTableT[1]); else if (G == 16) LV_T(16, kTableT[1]); else LV_T(8, kTableT[1]); break;
+        case 2: if (G == 8) LV_T(8, kTableT[2]); else if (G == 32) LV_T(32, kTableT[2]); else LV_T(16, kTableT[2]); break;
+        case 3: if (G == 16) LV_T(16, kTableT[3]); else if (G == 64) LV_T(64, kTableT[3]); else LV_T(32, kTableT[3]); break;
+        case 4: if (G == 32) LV_T(32, kTableT[4]); else LV_T(64, kTableT[4]); break;
+        case 5: if (G == 256) LV_T(256, kTableT[5]); else LV_T(64, kTableT[5]); break;
+        case 6: LV_T(256, kTableT[6]); break;
+        default: LV_T(256, kTableT[7]); break;
     }
 #undef LV_T
 }
@@ -1541,12 +1596,19 @@ MoveCtx move_ctx(const SweepArgs &a) {
     MoveCtx m;
     m.row = a.g.row; m.col = a.g.col; m.wt = a.g.wt; m.kdeg = a.g.kdeg;
     m.comm = a.comm; m.tot = a.tot; m.target = a.target;
+    m.ctot = a.ct;
     m.m2 = a.m2; m.tw = a.tw; m.v_end = a.v_end; m.highest = a.highest;
     m.bt = a.bt;
     return m;
 }
 
 }  // namespace
+
+void pack_comm_tot(const SweepArgs &a, hipStream_t s) {
+    if (!a.g.n) return;
+    hipLaunchKernelGGL(k_pack_ctot, dim3(grid_for(a.g.n, kBlock * 4)), dim3(kBlock), 0, s, a.comm,
+                       a.tot, a.g.n, a.ct);
+}
 
 void move_bucket(const SweepArgs &a, int b, int sel, const uint32_t *list, uint32_t count,
                  hipStream_t s) {
@@ -1567,6 +1629,7 @@ void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
     // tails and launch gaps overlap); bucket 6, the largest launch, runs after
     // the join with the device to itself.  A bucket's profiling events go on
     // the stream it runs on.
+    if (!fast) pack_comm_tot(a, s0);   // before the fork: every stream reads it
     const bool fork = side && side->ready();
     if (fork) {
         LV_HIP(hipEventRecord(side->fork, s0));

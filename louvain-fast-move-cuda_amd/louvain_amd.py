@@ -10,6 +10,7 @@ no CPU fallback: a missing library raises at import.
 """
 import ctypes as C
 import os
+import threading
 
 import numpy as np
 
@@ -513,10 +514,24 @@ def footprint(graph):
     return b.value
 
 
+_rec_bufs = threading.local()
+
+
+def _record_buffers(sweeps_cap, levels_cap):
+    """Per-thread record arrays, grown on demand and reused: a zeroed array
+    of 100,000 sweep records costs about a millisecond to allocate, as much as
+    a whole small run.  The records are copied out before the next call."""
+    sw, lv = getattr(_rec_bufs, "sw", None), getattr(_rec_bufs, "lv", None)
+    if sw is None or len(sw) < sweeps_cap:
+        sw = _rec_bufs.sw = (SweepRec * sweeps_cap)()
+    if lv is None or len(lv) < levels_cap:
+        lv = _rec_bufs.lv = (LevelRec * levels_cap)()
+    return sw, lv
+
+
 def _run_result(call, n, sweeps_cap, levels_cap, hash_trace, best):
     out = np.empty(n, np.uint32) if best else None
-    sw = (SweepRec * sweeps_cap)()
-    lv = (LevelRec * levels_cap)()
+    sw, lv = _record_buffers(sweeps_cap, levels_cap)
     r = Result()
     r.hash_trace = int(hash_trace)
     if best:
