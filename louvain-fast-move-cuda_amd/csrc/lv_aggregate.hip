@@ -129,17 +129,25 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
             }
 #pragma unroll
             for (int u = 0; u < kAggIlp; ++u) ds[u] = xs[u] != LV_EMPTY ? a.newc[xs[u]] : LV_EMPTY;
+            // first probes issued back to back (one LDS round trip), then the
+            // rare occupied first slots probe on; a lane's entries stay in order
+            uint32_t hs[kAggIlp], pv[kAggIlp];
+#pragma unroll
+            for (int u = 0; u < kAggIlp; ++u) {
+                hs[u] = slot_hash_dyn(ds[u], lg);
+                pv[u] = ds[u] != LV_EMPTY ? atomicCAS(&K[hs[u]], LV_EMPTY, ds[u]) : LV_EMPTY;
+            }
 #pragma unroll
             for (int u = 0; u < kAggIlp; ++u) {
                 const uint32_t d = ds[u];
                 if (d == LV_EMPTY) continue;
-                uint32_t h = slot_hash_dyn(d, lg);
-                for (;;) {
-                    const uint32_t prev = atomicCAS(&K[h], LV_EMPTY, d);
-                    if (prev == LV_EMPTY) { atomicAdd(&s_cnt[gi], 1u); atomicAdd(&V[h], ws[u]); break; }
-                    if (prev == d) { atomicAdd(&V[h], ws[u]); break; }
+                uint32_t h = hs[u], prev = pv[u];
+                while (prev != LV_EMPTY && prev != d) {
                     h = (h + 1) & mask;
+                    prev = atomicCAS(&K[h], LV_EMPTY, d);
                 }
+                if (prev == LV_EMPTY) atomicAdd(&s_cnt[gi], 1u);
+                atomicAdd(&V[h], ws[u]);
             }
         }
         group_barrier<G>();
