@@ -491,7 +491,7 @@ def main():
     sw0 = last["sweeps"][0]
     # hardware traffic of the dominant kernel from the committed counter runs
     traffic = None
-    tf = os.path.join(REPO, "profiles", "k_move_table_256_2048_rmat22_traffic.json")
+    tf = os.path.join(REPO, "profiles", "k_move_table_256_1540_rmat22_traffic.json")
     if os.path.exists(tf) and args.scale == 22:
         t = json.load(open(tf))
         if t.get("kernel") == L.kernel_variant_name(vi):
