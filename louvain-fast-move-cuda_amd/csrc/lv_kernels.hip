@@ -1111,16 +1111,17 @@ __global__ void k_pack_ctot(const uint32_t *comm, const uint32_t *tot, uint32_t 
 constexpr int kCommitIdent = 1;
 constexpr int kCommitPacked = 2;
 
+// The caller loads v's words (communities, row bounds, degree) together, so
+// their latencies overlap.
 template <bool WEIGHTED, int CM>
 __device__ inline int64_t commit_row(const CommitCtx &a, uint32_t v, uint32_t own, uint32_t c,
-                                     uint32_t first, uint32_t stride) {
+                                     uint64_t beg, uint64_t end, uint32_t kv, uint32_t first,
+                                     uint32_t stride) {
     if (first == 0) {
-        const uint32_t kv = a.kdeg[v];
         atomicAdd(&a.tot[c], kv);
         atomicSub(&a.tot[own], kv);
     }
     int64_t dsig = 0;
-    const uint64_t beg = a.row[v], end = a.row[v + 1];
     for (uint64_t jb = beg + first; jb < end; jb += (uint64_t)stride * kIlp) {
         uint32_t xs[kIlp], ws[kIlp], ox[kIlp], nx[kIlp];
 #pragma unroll
@@ -1176,7 +1177,10 @@ __device__ inline int64_t commit_group(const CommitCtx &a, const uint32_t *__res
         v = list[idx];
         const uint32_t own = a.comm[v];
         const uint32_t c = a.target[v];
-        if (c != own) dsig = commit_row<WEIGHTED, CM>(a, v, own, c, (uint32_t)lane, (uint32_t)G);
+        const uint64_t beg = a.row[v], end = a.row[v + 1];
+        const uint32_t kv = a.kdeg[v];
+        if (c != own)
+            dsig = commit_row<WEIGHTED, CM>(a, v, own, c, beg, end, kv, (uint32_t)lane, (uint32_t)G);
     }
     return dsig;
 }
@@ -1253,7 +1257,8 @@ __global__ __launch_bounds__(kBlock) void k_commit_big(CommitCtx a, BigLists bl,
         while (j >= nm[q] * kBigParts[q]) { j -= nm[q] * kBigParts[q]; ++q; }
         const uint32_t v = movers[bl.off[q] + j / kBigParts[q]];
         const uint32_t part = j % kBigParts[q];
-        const int64_t dsig = commit_row<WEIGHTED, CM>(a, v, a.comm[v], a.target[v],
+        const int64_t dsig = commit_row<WEIGHTED, CM>(a, v, a.comm[v], a.target[v], a.row[v],
+                                                  a.row[v + 1], a.kdeg[v],
                                                   part * kBlock + threadIdx.x,
                                                   kBlock * kBigParts[q]);
         if (a.gid) wave_keyed_add(a.g_dsig, a.gid[v], (uint64_t)dsig, dsig != 0);
