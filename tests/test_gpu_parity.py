@@ -198,6 +198,67 @@ def test_fast_sweep_identity():
         assert np.array_equal(tg, to)
 
 
+def _table_slots(d):
+    """Slots of the best-move LDS table of a row of d entries (d + d/2 + 1,
+    rounded up to 4)."""
+    return (d + d // 2 + 1 + 3) & ~3
+
+
+def _first_slot(c, t):
+    """First probe of key c in a table of t slots (multiplicative hash scaled
+    by a high multiply)."""
+    return ((c.astype(np.uint64) * 2654435761) & 0xFFFFFFFF) * t >> 32
+
+
+def test_tables_full_at_bucket_limits():
+    """A vertex at the largest degree of every LDS bucket (4 ... 4096) and one
+    past it, each neighbour in its own community: the tables hold as many
+    distinct keys as they ever do (two thirds of the slots), and every key
+    hashes into the last quarter of the table, so the probe runs wrap past its
+    end.  Best moves of the hash sweep equal the oracle's in every gain and
+    tie mode, weighted and not."""
+    rng = np.random.default_rng(5)
+    degs = [4, 5, 8, 9, 16, 17, 32, 33, 64, 65, 256, 257, 1024, 1025, 4096]
+    n = 400_000
+    centers = np.arange(len(degs)) + 10
+    ids = np.arange(1000, n, dtype=np.uint64)
+    ea, eb, leaf_comm = [], [], {}
+    next_leaf = n - 1
+    for v, d in zip(centers, degs):
+        # d - 2 leaves; the centre's two chain neighbours complete its degree
+        t = _table_slots(d)
+        pool = ids[_first_slot(ids, t) >= (3 * t) // 4]
+        comms = rng.choice(pool, size=d - 2, replace=False)
+        leaves = np.arange(next_leaf - (d - 2) + 1, next_leaf + 1)
+        next_leaf -= d - 2
+        ea.append(np.full(d - 2, v)); eb.append(leaves)
+        leaf_comm.update(zip(leaves.tolist(), comms.tolist()))
+    chain = np.arange(n - 1)   # every id in use
+    ea = np.concatenate(ea + [chain]); eb = np.concatenate(eb + [chain + 1])
+    comm = np.arange(n, dtype=np.uint32)
+    lv = np.fromiter(leaf_comm.keys(), np.int64)
+    comm[lv] = np.fromiter(leaf_comm.values(), np.int64)
+    active = np.zeros(n, np.uint8)
+    active[centers] = 1
+    for weighted in (False, True):
+        w = rng.integers(1, 9, len(ea)).astype(np.uint32) if weighted else None
+        g = L.Graph.from_edges(ea, eb, w)
+        og = O.Graph.from_edges(ea, eb, w)
+        assert g.n == n
+        row, _, _ = g.csr()
+        assert np.array_equal(np.diff(row.astype(np.int64))[centers], degs)
+        c = L.Community(g)
+        c.set_partition(comm)
+        for (gn, gg, og_), (tn, gt, ot) in itertools.product(GAINS, TIES):
+            say(f"full tables weighted={weighted} {gn} {tn}")
+            tg, ea_g = c.best_moves(L.SWEEP_HASH, L.params(gain=gg, tie=gt), active)
+            to, _, ea_o, _ = og.best_moves(comm, active, O.SWEEP_HASH,
+                                           O.params(gain=og_, tie=ot))
+            assert np.array_equal(tg[centers], to[centers]), (gn, tn, tg[centers], to[centers])
+            assert np.array_equal(tg, to)
+            assert ea_g == ea_o
+
+
 def test_aggregation_matches_oracle():
     rng = np.random.default_rng(5)
     ea, eb = _random_graph(rng, n=8000, m=40000, hubs=1, hub_deg=5000, selfloops=20)
