@@ -27,6 +27,7 @@
 #include "../../include/operators.h"
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 
@@ -436,6 +437,40 @@ struct MoveCtx {
     bool highest;
 };
 
+// The words of one listed vertex a best-move table kernel needs before its
+// row: loaded together (their latencies overlap), for the vertex at hand or,
+// in the persistent kernels, for the next one while this one is scored.
+struct MoveWords {
+    uint32_t v = 0, own = 0, kv = 0, tot_own = 0;
+    uint64_t beg = 0, end = 0;
+    int64_t m2 = 0;
+    double tw = 0;
+    bool live = false;
+};
+
+template <bool PK>
+__device__ inline MoveWords move_words(const MoveCtx &a, uint32_t v, bool valid) {
+    MoveWords w;
+    w.v = v;
+    w.m2 = a.m2;
+    w.tw = a.tw;
+    w.live = valid && v < a.v_end && vertex_graph(a.bt, v, a.m2, a.tw, w.m2, w.tw);
+    if (w.live) {
+        if constexpr (PK) {
+            const unsigned long long p = a.ctot[v];
+            w.own = (uint32_t)p;
+            w.tot_own = (uint32_t)(p >> 32);
+        } else {
+            w.own = a.comm[v];
+        }
+        w.kv = a.kdeg[v];
+        w.beg = a.row[v];
+        w.end = a.row[v + 1];
+        if constexpr (!PK) w.tot_own = a.tot[w.own];
+    }
+    return w;
+}
+
 // Group-per-vertex with an LDS table of T slots (T >= table_slots of the
 // bucket's maximum degree); a vertex uses the first table_slots(degree).
 // PK: a neighbour's community and that community's total come in one 8-byte
@@ -444,169 +479,168 @@ struct MoveCtx {
 // LDS alone.  Without PK the totals are gathered once per candidate (one more
 // dependent random access per distinct community, for 4 bytes less LDS per
 // slot).
+// G == kBlock (one workgroup per vertex) runs persistent: a workgroup walks
+// the list with a stride of the grid, the next vertex's list entry is loaded
+// when a vertex starts and its words when its inserts are done, so both
+// latencies pass under this vertex's barrier and scoring, and the statistics
+// are reduced once per workgroup.  The grid is then at most what the device
+// holds at once (persistent_grid).
 template <int G, int T, int GAIN, bool WEIGHTED, bool PK>
 __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t *__restrict__ list,
                                                        uint32_t count) {
     using SC = Score<GAIN>;
     using S = typename SC::T;
     constexpr int GPB = kBlock / G;
+    constexpr bool PERSIST = G == kBlock;
     __shared__ uint32_t s_key[GPB * T];
     __shared__ uint32_t s_val[GPB * T];
     __shared__ uint32_t s_tot[PK ? GPB * T : 1];
+    __shared__ S r_s[kBlock / 64];
+    __shared__ uint32_t r_c[kBlock / 64];
+    __shared__ uint64_t r_n[kBlock / 64];
     const int lane = threadIdx.x & (G - 1);
     const int gi = threadIdx.x / G;
-    const uint32_t idx = blockIdx.x * GPB + gi;
-    const bool valid = idx < count;
-    const uint32_t v = valid ? list[idx] : 0u;
-    int64_t m2 = a.m2;
-    double tw = a.tw;
-    const bool live = valid && v < a.v_end && vertex_graph(a.bt, v, a.m2, a.tw, m2, tw);
     uint32_t *K = s_key + gi * T;
     uint32_t *V = s_val + gi * T;
     uint32_t *TT = s_tot + (PK ? gi * T : 0);
-    // the vertex's table: table_slots(degree) <= T slots.  The vertex's own
-    // words are loaded here, beside its row bounds, so that their latency
-    // overlaps the row's instead of following the table's barrier.
-    uint32_t own = 0, tv = 4, kv = 0, tot_own = 0;
-    uint64_t beg = 0, end = 0;
-    if (live) {
-        if constexpr (PK) {
-            const unsigned long long p = a.ctot[v];
-            own = (uint32_t)p;
-            tot_own = (uint32_t)(p >> 32);
-        } else {
-            own = a.comm[v];
-        }
-        kv = a.kdeg[v];
-        beg = a.row[v];
-        end = a.row[v + 1];
-        tv = table_slots(end - beg);
-        if constexpr (!PK) tot_own = a.tot[own];
-    }
-    for (uint32_t s = lane; s < tv; s += G) { K[s] = LV_EMPTY; V[s] = 0u; }
-    __syncthreads();
-    uint64_t nself = 0;
-    if (live) {
-        // kIlp entries per lane in flight: loads first, then gathers, then inserts
-        for (uint64_t jb = beg + lane; jb < end; jb += (uint64_t)G * kIlp) {
-            uint32_t xs[kIlp], ws[kIlp], cs[kIlp], ts[kIlp];
-#pragma unroll
-            for (int u = 0; u < kIlp; ++u) {
-                const uint64_t j = jb + (uint64_t)u * G;
-                xs[u] = j < end ? stream_load(&a.col[j]) : v;
-                ws[u] = (WEIGHTED && j < end) ? stream_load(&a.wt[j]) : 1u;
-            }
-#pragma unroll
-            for (int u = 0; u < kIlp; ++u) {
-                if constexpr (PK) {
-                    const unsigned long long p = xs[u] != v ? a.ctot[xs[u]] : 0ull;
-                    cs[u] = (uint32_t)p;
-                    ts[u] = (uint32_t)(p >> 32);
-                } else {
-                    cs[u] = xs[u] != v ? a.comm[xs[u]] : 0u;
-                    ts[u] = 0u;
-                }
-            }
-            // the first probe of all kIlp entries is issued back to back (one
-            // LDS round trip instead of kIlp); a lane's own entries are still
-            // inserted in order, so an entry sees the keys its predecessors
-            // placed, and the rare occupied first slots probe on afterwards
-            uint32_t hs[kIlp], pv[kIlp];
-#pragma unroll
-            for (int u = 0; u < kIlp; ++u) {
-                hs[u] = slot_hash_mod(cs[u], tv);
-                pv[u] = xs[u] != v ? atomicCAS(&K[hs[u]], LV_EMPTY, cs[u]) : LV_EMPTY;
-            }
-#pragma unroll
-            for (int u = 0; u < kIlp; ++u) {
-                if (xs[u] == v) continue;
-                ++nself;
-                const uint32_t c = cs[u];
-                uint32_t h = hs[u], prev = pv[u];
-                while (prev != LV_EMPTY && prev != c) {
-                    h = h + 1 == tv ? 0u : h + 1;
-                    prev = atomicCAS(&K[h], LV_EMPTY, c);
-                }
-                if (PK && prev == LV_EMPTY) TT[h] = ts[u];
-                atomicAdd(&V[h], ws[u]);
-            }
-        }
-    }
-    __syncthreads();
-    S best = SC::none();
-    uint32_t best_c = a.highest ? 0u : LV_EMPTY;
-    uint64_t ncand = 0;
-    uint32_t self = 0;
-    if (live) {
-        uint32_t h = slot_hash_mod(own, tv);
-        for (;;) {
-            const uint32_t c = K[h];
-            if (c == own) { self = V[h]; break; }
-            if (c == LV_EMPTY) break;
-            h = h + 1 == tv ? 0u : h + 1;
-        }
-        for (uint32_t sb = lane; sb < tv; sb += (uint32_t)G * kIlp) {
-            uint32_t cs[kIlp], Ws[kIlp], ts[kIlp];
-#pragma unroll
-            for (int u = 0; u < kIlp; ++u) {
-                const uint32_t s2 = sb + (uint32_t)u * G;
-                const uint32_t c = s2 < tv ? K[s2] : LV_EMPTY;
-                cs[u] = c == own ? LV_EMPTY : c;
-                Ws[u] = cs[u] != LV_EMPTY ? V[s2] : 0u;
-            }
-#pragma unroll
-            for (int u = 0; u < kIlp; ++u) {
-                const uint32_t s2 = sb + (uint32_t)u * G;
-                if constexpr (PK) ts[u] = cs[u] != LV_EMPTY ? TT[s2] : 0u;
-                else ts[u] = cs[u] != LV_EMPTY ? a.tot[cs[u]] : 0u;
-            }
-#pragma unroll
-            for (int u = 0; u < kIlp; ++u) {
-                if (cs[u] == LV_EMPTY) continue;
-                ++ncand;
-                const S sc = SC::of(Ws[u], self, kv, tot_own, ts[u], m2, tw);
-                if (better_candidate(sc, cs[u], best, best_c, a.highest)) { best = sc; best_c = cs[u]; }
-            }
-        }
-    }
-    uint64_t considered = 0, twrite = 0;
-    if constexpr (G <= 64) {
-        group_best<G, S>(best, best_c, a.highest);
-        const uint64_t gself = group_sum_u64<G>(nself);
-        if (live && lane == 0) {
-            considered = gself > 0;
-            if (best != SC::none() && SC::positive(best, self, kv, tot_own, m2)) {
-                a.target[v] = best_c;
-                twrite = 1;
-            }
-        }
-    } else {
-        __shared__ S r_s[kBlock / 64];
-        __shared__ uint32_t r_c[kBlock / 64];
-        __shared__ uint64_t r_n[kBlock / 64];
-        group_best<64, S>(best, best_c, a.highest);
-        const uint64_t wself = group_sum_u64<64>(nself);
-        const int w = threadIdx.x >> 6;
-        if ((threadIdx.x & 63) == 0) { r_s[w] = best; r_c[w] = best_c; r_n[w] = wself; }
+    uint32_t idx = PERSIST ? blockIdx.x : blockIdx.x * GPB + gi;
+    MoveWords nw = move_words<PK>(a, idx < count ? list[idx] : 0u, idx < count);
+    uint64_t acc_self = 0, acc_cand = 0, acc_cons = 0, acc_tw = 0;
+    for (;;) {
+        const MoveWords cw = nw;
+        const uint32_t v = cw.v, own = cw.own, kv = cw.kv, tot_own = cw.tot_own;
+        const uint64_t beg = cw.beg, end = cw.end;
+        const bool live = cw.live;
+        const uint32_t nidx = idx + gridDim.x;   // PERSIST only
+        const bool nvalid = PERSIST && nidx > idx && nidx < count;
+        const uint32_t nv = nvalid ? list[nidx] : 0u;
+        // the vertex's table: table_slots(degree) <= T slots
+        const uint32_t tv = live ? table_slots(end - beg) : 4u;
+        for (uint32_t s = lane; s < tv; s += G) { K[s] = LV_EMPTY; V[s] = 0u; }
         __syncthreads();
-        if (threadIdx.x == 0 && live) {
-            uint64_t tn = 0;
-            for (int i = 0; i < kBlock / 64; ++i) {
-                tn += r_n[i];
-                if (better_candidate(r_s[i], r_c[i], best, best_c, a.highest)) {
-                    best = r_s[i];
-                    best_c = r_c[i];
+        uint64_t nself = 0;
+        if (live) {
+            // kIlp entries per lane in flight: loads first, then gathers, then inserts
+            for (uint64_t jb = beg + lane; jb < end; jb += (uint64_t)G * kIlp) {
+                uint32_t xs[kIlp], ws[kIlp], cs[kIlp], ts[kIlp];
+#pragma unroll
+                for (int u = 0; u < kIlp; ++u) {
+                    const uint64_t j = jb + (uint64_t)u * G;
+                    xs[u] = j < end ? stream_load(&a.col[j]) : v;
+                    ws[u] = (WEIGHTED && j < end) ? stream_load(&a.wt[j]) : 1u;
+                }
+#pragma unroll
+                for (int u = 0; u < kIlp; ++u) {
+                    if constexpr (PK) {
+                        const unsigned long long p = xs[u] != v ? a.ctot[xs[u]] : 0ull;
+                        cs[u] = (uint32_t)p;
+                        ts[u] = (uint32_t)(p >> 32);
+                    } else {
+                        cs[u] = xs[u] != v ? a.comm[xs[u]] : 0u;
+                        ts[u] = 0u;
+                    }
+                }
+                // the first probe of all kIlp entries is issued back to back (one
+                // LDS round trip instead of kIlp); a lane's own entries are still
+                // inserted in order, so an entry sees the keys its predecessors
+                // placed, and the rare occupied first slots probe on afterwards
+                uint32_t hs[kIlp], pv[kIlp];
+#pragma unroll
+                for (int u = 0; u < kIlp; ++u) {
+                    hs[u] = slot_hash_mod(cs[u], tv);
+                    pv[u] = xs[u] != v ? atomicCAS(&K[hs[u]], LV_EMPTY, cs[u]) : LV_EMPTY;
+                }
+#pragma unroll
+                for (int u = 0; u < kIlp; ++u) {
+                    if (xs[u] == v) continue;
+                    ++nself;
+                    const uint32_t c = cs[u];
+                    uint32_t h = hs[u], prev = pv[u];
+                    while (prev != LV_EMPTY && prev != c) {
+                        h = h + 1 == tv ? 0u : h + 1;
+                        prev = atomicCAS(&K[h], LV_EMPTY, c);
+                    }
+                    if (PK && prev == LV_EMPTY) TT[h] = ts[u];
+                    atomicAdd(&V[h], ws[u]);
                 }
             }
-            considered = tn > 0;
-            if (best != SC::none() && SC::positive(best, self, kv, tot_own, m2)) {
-                a.target[v] = best_c;
-                twrite = 1;
+        }
+        if constexpr (PERSIST) nw = move_words<PK>(a, nv, nvalid);
+        __syncthreads();
+        S best = SC::none();
+        uint32_t best_c = a.highest ? 0u : LV_EMPTY;
+        uint32_t self = 0;
+        if (live) {
+            uint32_t h = slot_hash_mod(own, tv);
+            for (;;) {
+                const uint32_t c = K[h];
+                if (c == own) { self = V[h]; break; }
+                if (c == LV_EMPTY) break;
+                h = h + 1 == tv ? 0u : h + 1;
+            }
+            for (uint32_t sb = lane; sb < tv; sb += (uint32_t)G * kIlp) {
+                uint32_t cs[kIlp], Ws[kIlp], ts[kIlp];
+#pragma unroll
+                for (int u = 0; u < kIlp; ++u) {
+                    const uint32_t s2 = sb + (uint32_t)u * G;
+                    const uint32_t c = s2 < tv ? K[s2] : LV_EMPTY;
+                    cs[u] = c == own ? LV_EMPTY : c;
+                    Ws[u] = cs[u] != LV_EMPTY ? V[s2] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < kIlp; ++u) {
+                    const uint32_t s2 = sb + (uint32_t)u * G;
+                    if constexpr (PK) ts[u] = cs[u] != LV_EMPTY ? TT[s2] : 0u;
+                    else ts[u] = cs[u] != LV_EMPTY ? a.tot[cs[u]] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < kIlp; ++u) {
+                    if (cs[u] == LV_EMPTY) continue;
+                    ++acc_cand;
+                    const S sc = SC::of(Ws[u], self, kv, tot_own, ts[u], cw.m2, cw.tw);
+                    if (better_candidate(sc, cs[u], best, best_c, a.highest)) { best = sc; best_c = cs[u]; }
+                }
             }
         }
+        acc_self += nself;
+        if constexpr (G <= 64) {
+            group_best<G, S>(best, best_c, a.highest);
+            const uint64_t gself = group_sum_u64<G>(nself);
+            if (live && lane == 0) {
+                acc_cons += gself > 0;
+                if (best != SC::none() && SC::positive(best, self, kv, tot_own, cw.m2)) {
+                    a.target[v] = best_c;
+                    ++acc_tw;
+                }
+            }
+        } else {
+            group_best<64, S>(best, best_c, a.highest);
+            const uint64_t wself = group_sum_u64<64>(nself);
+            const int w = threadIdx.x >> 6;
+            if ((threadIdx.x & 63) == 0) { r_s[w] = best; r_c[w] = best_c; r_n[w] = wself; }
+            __syncthreads();
+            if (threadIdx.x == 0 && live) {
+                uint64_t tn = 0;
+                for (int i = 0; i < kBlock / 64; ++i) {
+                    tn += r_n[i];
+                    if (better_candidate(r_s[i], r_c[i], best, best_c, a.highest)) {
+                        best = r_s[i];
+                        best_c = r_c[i];
+                    }
+                }
+                acc_cons += tn > 0;
+                if (best != SC::none() && SC::positive(best, self, kv, tot_own, cw.m2)) {
+                    a.target[v] = best_c;
+                    ++acc_tw;
+                }
+            }
+        }
+        // uniform over the workgroup: idx and the grid are the same for all lanes
+        if (!nvalid) break;
+        idx = nidx;
     }
-    block_add_stats(a.stats, ST_E_ACTIVE, nself, ST_CANDIDATES, ncand, ST_CONSIDERED, considered,
-                    ST_TWRITES, twrite);
+    block_add_stats(a.stats, ST_E_ACTIVE, acc_self, ST_CANDIDATES, acc_cand, ST_CONSIDERED,
+                    acc_cons, ST_TWRITES, acc_tw);
 }
 
 // First sweep of a level (identity partition, phaseOptimizationSort.cu:28-158):
@@ -1538,18 +1572,43 @@ namespace {
 
 // Tables of up to kPackedMaxT slots are packed; bucket 7's keeps the
 // per-candidate total gathers rather than 50% more LDS per vertex.
+// Workgroups of `kernel` the device holds at once (CUs x occupancy), cached
+// in `slot` at the first launch (every device in use is the same model).
+template <typename K>
+uint32_t resident_blocks(K kernel, std::atomic<uint32_t> &slot) {
+    uint32_t r = slot.load(std::memory_order_relaxed);
+    if (r) return r;
+    int dev = 0, cus = 0, per = 0;
+    LV_HIP(hipGetDevice(&dev));
+    LV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    LV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, 0));
+    r = (uint32_t)std::max(1, cus * std::max(per, 1));
+    slot.store(r, std::memory_order_relaxed);
+    return r;
+}
+
+template <int G, int T, int GAIN, bool WEIGHTED>
+void launch_move_table_k(const MoveCtx &m, const uint32_t *list, uint32_t count, hipStream_t s) {
+    constexpr int GPB = kBlock / G;
+    constexpr bool PK = T <= kPackedMaxT;
+    auto kernel = k_move_table<G, T, GAIN, WEIGHTED, PK>;
+    uint32_t blocks = (count + GPB - 1) / GPB;
+    if (G == kBlock) {   // persistent: at most one wave of resident workgroups
+        static std::atomic<uint32_t> resident{0};
+        blocks = std::min(blocks, resident_blocks(kernel, resident));
+    }
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), 0, s, m, list, count);
+}
+
 template <int G, int T>
 void launch_move_table(const MoveCtx &m, bool weighted, int gain, const uint32_t *list,
                        uint32_t count, hipStream_t s) {
-    constexpr int GPB = kBlock / G;
-    constexpr bool PK = T <= kPackedMaxT;
-    const dim3 grid((count + GPB - 1) / GPB), blk(kBlock);
     if (gain == 0) {
-        if (weighted) hipLaunchKernelGGL((k_move_table<G, T, 0, true, PK>), grid, blk, 0, s, m, list, count);
-        else hipLaunchKernelGGL((k_move_table<G, T, 0, false, PK>), grid, blk, 0, s, m, list, count);
+        if (weighted) launch_move_table_k<G, T, 0, true>(m, list, count, s);
+        else launch_move_table_k<G, T, 0, false>(m, list, count, s);
     } else {
-        if (weighted) hipLaunchKernelGGL((k_move_table<G, T, 1, true, PK>), grid, blk, 0, s, m, list, count);
-        else hipLaunchKernelGGL((k_move_table<G, T, 1, false, PK>), grid, blk, 0, s, m, list, count);
+        if (weighted) launch_move_table_k<G, T, 1, true>(m, list, count, s);
+        else launch_move_table_k<G, T, 1, false>(m, list, count, s);
     }
 }
 
