@@ -26,6 +26,7 @@ namespace lv {
 namespace {
 
 constexpr int kAggIlp = 4;   // entries per lane with loads in flight together
+constexpr int kCopyIlp = 8;  // row copies: loads per lane issued before their stores
 
 struct AggCtx {
     const uint64_t *row;
@@ -376,10 +377,23 @@ __global__ __launch_bounds__(kBlock) void k_giant_copy(GiantCtx h, const uint64_
     const unsigned long long *in = h.p0 + h.poff[I.pbase + p];
     const uint64_t dst = out_row[I.c] + (h.doff[I.pbase + p] - h.doff[I.pbase]);
     const uint32_t n = h.dcnt[I.pbase + p];
-    for (uint32_t t = threadIdx.x; t < n; t += kBlock) {
-        const unsigned long long pr = in[t];
-        col[dst + t] = (uint32_t)(pr >> 32);
-        w[dst + t] = (uint32_t)pr;
+    // kCopyIlp loads per lane before their stores: the output may alias the
+    // input as far as the compiler knows, so a plain loop waits on every load
+    for (uint32_t t0 = threadIdx.x; t0 < n; t0 += kBlock * kCopyIlp) {
+        unsigned long long pr[kCopyIlp];
+#pragma unroll
+        for (int u = 0; u < kCopyIlp; ++u) {
+            const uint32_t t = t0 + (uint32_t)u * kBlock;
+            pr[u] = t < n ? in[t] : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < kCopyIlp; ++u) {
+            const uint32_t t = t0 + (uint32_t)u * kBlock;
+            if (t < n) {
+                col[dst + t] = (uint32_t)(pr[u] >> 32);
+                w[dst + t] = (uint32_t)pr[u];
+            }
+        }
     }
 }
 
@@ -427,9 +441,25 @@ __global__ __launch_bounds__(kBlock) void k_agg_copy(AggCtx a, const uint32_t *_
     const uint64_t dst = out_row[c];
     const uint32_t stride = G * (G == kBlock ? parts : 1u);
     const uint64_t src = a.crow[c];
-    for (uint32_t t = part * G + lane; t < r; t += stride) {
-        col[dst + t] = a.tcol[src + t];
-        w[dst + t] = a.tw[src + t];
+    // kCopyIlp loads per lane before their stores: the output may alias the
+    // staging rows as far as the compiler knows, so a plain loop waits on
+    // every load
+    for (uint32_t t0 = part * G + lane; t0 < r; t0 += stride * kCopyIlp) {
+        uint32_t cs[kCopyIlp], ws[kCopyIlp];
+#pragma unroll
+        for (int u = 0; u < kCopyIlp; ++u) {
+            const uint32_t t = t0 + (uint32_t)u * stride;
+            cs[u] = t < r ? a.tcol[src + t] : 0u;
+            ws[u] = t < r ? a.tw[src + t] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < kCopyIlp; ++u) {
+            const uint32_t t = t0 + (uint32_t)u * stride;
+            if (t < r) {
+                col[dst + t] = cs[u];
+                w[dst + t] = ws[u];
+            }
+        }
     }
 }
 
