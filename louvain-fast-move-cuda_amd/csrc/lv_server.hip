@@ -22,6 +22,8 @@ __global__ void k_fill_gid_ranges(const uint32_t *voff, uint32_t jobs, uint32_t 
     }
 }
 
+constexpr int kSegIlp = 4;   // entries per lane with loads in flight together
+
 // One segment per blockIdx.y: rows re-based to the destination entry offset,
 // columns shifted to the destination vertex range, weights (1 when the source
 // has none) and degrees copied.
@@ -37,9 +39,24 @@ __global__ __launch_bounds__(kBlock) void k_segment_copy(const Segment *segs, ui
         if (sg.kdeg) kdeg[sg.dst_voff + i] = sg.kdeg[sg.src_voff + i];
     }
     const uint32_t shift = sg.dst_voff - sg.src_voff;   // modulo 2^32
-    for (size_t j = t; j < sg.E; j += stride) {
-        col[sg.dst_eoff + j] = sg.col[e0 + j] + shift;
-        w[sg.dst_eoff + j] = sg.w ? sg.w[e0 + j] : 1u;
+    // kSegIlp loads per lane before their stores (the compiler cannot move a
+    // load above a store that may alias it)
+    for (size_t j0 = t; j0 < sg.E; j0 += stride * kSegIlp) {
+        uint32_t cs[kSegIlp], ws[kSegIlp];
+#pragma unroll
+        for (int u = 0; u < kSegIlp; ++u) {
+            const size_t j = j0 + (size_t)u * stride;
+            cs[u] = j < sg.E ? sg.col[e0 + j] : 0u;
+            ws[u] = (j < sg.E && sg.w) ? sg.w[e0 + j] : 1u;
+        }
+#pragma unroll
+        for (int u = 0; u < kSegIlp; ++u) {
+            const size_t j = j0 + (size_t)u * stride;
+            if (j < sg.E) {
+                col[sg.dst_eoff + j] = cs[u] + shift;
+                w[sg.dst_eoff + j] = ws[u];
+            }
+        }
     }
 }
 
@@ -64,9 +81,21 @@ __global__ __launch_bounds__(kBlock) void k_fresh_shift(const FreshSeg *segs, ui
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     for (size_t i = t; i < sg.n; i += stride) row[sg.dst_voff + i] += sg.dst_eoff;
-    for (size_t j = t; j < sg.E; j += stride) {
-        col[sg.dst_eoff + j] += sg.dst_voff;
-        if (sg.unit) w[sg.dst_eoff + j] = 1u;
+    for (size_t j0 = t; j0 < sg.E; j0 += stride * kSegIlp) {
+        uint32_t cs[kSegIlp];
+#pragma unroll
+        for (int u = 0; u < kSegIlp; ++u) {
+            const size_t j = j0 + (size_t)u * stride;
+            cs[u] = j < sg.E ? col[sg.dst_eoff + j] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < kSegIlp; ++u) {
+            const size_t j = j0 + (size_t)u * stride;
+            if (j < sg.E) {
+                col[sg.dst_eoff + j] = cs[u] + sg.dst_voff;
+                if (sg.unit) w[sg.dst_eoff + j] = 1u;
+            }
+        }
     }
 }
 
