@@ -45,7 +45,7 @@ void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
     list_buf[0].ensure(std::max<size_t>(n, 1));
     list_buf[1].ensure(std::max<size_t>(n, 1));
     counts_dev.ensure(2 * kNumBuckets);
-    stats.ensure((size_t)(kNumBuckets + 1) * kSection);
+    stats.ensure(kHostWords);
     if (!host_stats) {
         LV_HIP(hipHostMalloc((void **)&host_stats, kHostWords * sizeof(unsigned long long),
                              hipHostMallocDefault));
@@ -516,7 +516,7 @@ void launch_sweep(Community &c, const lv_params &p, int kind, bool apply, uint32
     hipStream_t s = c.stream();
     if (p.autotune) autotune_tiles(c, p, kind, v_end);
     SweepArgs a = sweep_args(c, p, kind, v_end);
-    sweep_reset(st.stats.data(), st.stats.size(), a.n_movers, st.counts_dev.data() + kNumBuckets, s);
+    sweep_reset(st.stats.data(), st.stats.size(), a.n_movers, st.next_counts(), s);
     a.counters_zeroed = true;
     uint32_t *const *cur = st.lists(st.cur);
     if (p.profile) LV_HIP(hipEventRecord(st.ev[0], s));
@@ -525,14 +525,14 @@ void launch_sweep(Community &c, const lv_params &p, int kind, bool apply, uint32
     if (p.profile) LV_HIP(hipEventRecord(st.ev[1], s));
     if (apply) {
         commit_pass(a, cur, st.count, s, st.side_streams());
-        finalize_pass(a, st.lists(st.cur ^ 1), st.counts_dev.data() + kNumBuckets, s);
-        LV_HIP(hipMemcpyAsync(st.host_stats + kStatWordsOffset,
-                              st.counts_dev.data() + kNumBuckets, kNumBuckets * sizeof(uint32_t),
-                              hipMemcpyDeviceToHost, s));
+        finalize_pass(a, st.lists(st.cur ^ 1), st.next_counts(), s);
     }
     if (p.profile) LV_HIP(hipEventRecord(st.ev[2], s));
+    // statistics and (after a commit) the next lists' sizes, adjacent on both sides
     LV_HIP(hipMemcpyAsync(st.host_stats, st.stats.data(),
-                          kStatWordsOffset * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+                          kStatWordsOffset * sizeof(unsigned long long) +
+                              (apply ? kNumBuckets * sizeof(uint32_t) : 0),
+                          hipMemcpyDeviceToHost, s));
 }
 
 }  // namespace lv

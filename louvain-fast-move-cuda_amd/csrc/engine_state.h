@@ -84,7 +84,10 @@ struct SweepState {
     DeviceArray<unsigned long long> ct;         // commit scratch: comm | target << 32
     DeviceArray<uint32_t> list_buf[2];
     DeviceArray<uint32_t> counts_dev;
+    // statistics (kStatWordsOffset words) followed by the next lists' sizes
+    // (kNumBuckets u32), the same layout as host_stats: one copy per sweep
     DeviceArray<unsigned long long> stats;
+    uint32_t *next_counts() { return reinterpret_cast<uint32_t *>(stats.data() + kStatWordsOffset); }
     unsigned long long *host_stats = nullptr;   // pinned
     uint64_t bucket_off[kNumBuckets] = {};
     uint32_t bucket_size[kNumBuckets] = {};

@@ -421,9 +421,9 @@ lv_status lv_best_moves(lv_community *c, int kind, const lv_params *p, const uin
             a.stats = st.stats.data() + lv::kGlobalSection;
             a.bstats = st.stats.data();
             lv::fill_u64(st.stats.data(), 0ull, st.stats.size(), s);
-            lv::finalize_pass(a, st.lists(st.cur ^ 1), st.counts_dev.data() + lv::kNumBuckets, s);
+            lv::finalize_pass(a, st.lists(st.cur ^ 1), st.next_counts(), s);
             uint32_t cnt[lv::kNumBuckets];
-            LV_HIP(hipMemcpyAsync(cnt, st.counts_dev.data() + lv::kNumBuckets, sizeof(cnt),
+            LV_HIP(hipMemcpyAsync(cnt, st.next_counts(), sizeof(cnt),
                                   hipMemcpyDeviceToHost, s));
             LV_HIP(hipStreamSynchronize(s));
             st.cur ^= 1;
