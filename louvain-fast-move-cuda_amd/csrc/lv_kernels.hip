@@ -5,7 +5,8 @@
 // degree; each degree bucket runs a kernel in which a group of G lanes (a
 // quarter wave up to a whole workgroup) owns one vertex, accumulates the
 // weights towards each neighbour community in an LDS open-addressing table
-// (2x the degree, so it never overflows), looks up its own community's weight,
+// (1.5x the degree plus one slot: never more than two thirds full, so every
+// probe ends), looks up its own community's weight,
 // scores every candidate and reduces to the best one.  Vertices above 4096
 // entries are cut into 32 chunks and 32 community hash partitions, each
 // partition summed in an LDS table by a workgroup of its own.  The first
