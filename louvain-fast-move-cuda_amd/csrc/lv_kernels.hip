@@ -1114,7 +1114,8 @@ struct CommitCtx {
     unsigned long long *stats;
     const uint32_t *gid;          // batches: graph of each vertex (nullptr: one graph)
     unsigned long long *g_dsig;   // batches: sigma_in change per graph
-    const unsigned long long *ct; // comm | target << 32 of every vertex (after k_pack_ct)
+    unsigned long long *ct;       // comm | target << 32 of every vertex (after k_pack_ct);
+                                  // kCommitPackedBit marks a vertex in bit 63
 };
 
 __global__ void k_pack_ct(const uint32_t *comm, const uint32_t *target, uint32_t n,
@@ -1143,8 +1144,14 @@ __global__ void k_pack_ctot(const uint32_t *comm, const uint32_t *tot, uint32_t 
 // identity partition (every first sweep of a level), so a neighbour's old
 // community is the neighbour itself; CM == kCommitPacked reads both
 // communities of a neighbour from one 8-byte word (ct = comm | target << 32).
+// CM == kCommitPackedBit also keeps the neighbour's mark in bit 63 of its
+// word (bit 31 of the target, free while ids are below 2^31): the mark costs
+// no access of its own, and is set by storing the word's high half with the
+// bit, the same value from every writer.  k_finalize reads it back.
 constexpr int kCommitIdent = 1;
 constexpr int kCommitPacked = 2;
+constexpr int kCommitPackedBit = 3;
+constexpr uint32_t kMarkBitMaxN = 0x80000000u;   // the bit path needs every id < 2^31
 
 // The caller loads v's words (communities, row bounds, degree) together, so
 // their latencies overlap.
@@ -1173,18 +1180,25 @@ __device__ inline int64_t commit_row(const CommitCtx &a, uint32_t v, uint32_t ow
             } else {
                 const unsigned long long p = xs[u] != v ? a.ct[xs[u]] : 0ull;
                 ox[u] = (uint32_t)p;
-                nx[u] = (uint32_t)(p >> 32);
+                nx[u] = (uint32_t)(p >> 32);   // CM == kCommitPackedBit: with the mark
             }
         }
 #pragma unroll
         for (int u = 0; u < kIlp; ++u) {
             if (xs[u] == v) continue;
+            const uint32_t hi = nx[u];
+            if (CM == kCommitPackedBit) nx[u] &= 0x7FFFFFFFu;
             const int d = (int)(nx[u] == c) - (int)(ox[u] == own);
             if (d) dsig += (int64_t)ws[u] * d * (ox[u] != nx[u] ? 1 : 2);
             // most neighbours are marked by several movers: store only the
-            // first mark, so lines of `mark` are not dirtied and written back
-            // again and again
-            if (nx[u] != c && !a.mark[xs[u]]) a.mark[xs[u]] = 1;
+            // first mark, so lines are not dirtied and written back again and
+            // again
+            if (nx[u] == c) continue;
+            if (CM == kCommitPackedBit) {
+                if (!(hi >> 31)) reinterpret_cast<uint32_t *>(&a.ct[xs[u]])[1] = hi | 0x80000000u;
+            } else if (!a.mark[xs[u]]) {
+                a.mark[xs[u]] = 1;
+            }
         }
     }
     return dsig;
@@ -1314,7 +1328,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(uint32_t *comm, const uint3
                                                      ListPtrs next, uint32_t *next_counts,
                                                      unsigned long long *stats,
                                                      const uint32_t *gid,
-                                                     unsigned long long *g_sumsq) {
+                                                     unsigned long long *g_sumsq,
+                                                     const unsigned long long *ctm) {
     __shared__ uint32_t s_cnt[kNumBuckets];
     __shared__ uint32_t s_base[kNumBuckets];
     if (threadIdx.x < kNumBuckets) s_cnt[threadIdx.x] = 0u;
@@ -1336,8 +1351,11 @@ __global__ __launch_bounds__(kBlock) void k_finalize(uint32_t *comm, const uint3
             q = square_u64(tot[v]);
             sq += q;
             if (gid) g = gid[v];
-            if (mark[v]) {
-                mark[v] = 0;
+            // marks of a kCommitPackedBit commit are in bit 63 of ctm
+            const bool in_ct = ctm && (ctm[v] >> 63);
+            const uint8_t mb = mark[v];
+            if (mb || in_ct) {
+                if (mb) mark[v] = 0;
                 bk[i] = bucket[v];
                 rank[i] = atomicAdd(&s_cnt[bk[i]], 1u);
             }
@@ -1773,7 +1791,12 @@ void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
 }
 
 static void commit_big_rows(const SweepArgs &a, const CommitCtx &c, uint32_t *const *lists,
-                            const uint32_t *counts, bool wtd, bool ident, hipStream_t s);
+                            const uint32_t *counts, bool wtd, bool ident, bool bit, hipStream_t s);
+
+// A packed commit keeps its marks in bit 63 of ct while every id is below 2^31.
+static bool marks_in_ct(const SweepArgs &a) {
+    return a.kind != 0 && a.ct != nullptr && a.g.n <= kMarkBitMaxN;
+}
 
 void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *counts,
                  hipStream_t s, const SideStreams *side) {
@@ -1783,6 +1806,7 @@ void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *cou
     c.gid = a.bt.gid; c.g_dsig = a.bt.dsig;
     const bool wtd = a.g.wt != nullptr;
     const bool ident = a.kind == 0;   // fast sweeps run at the identity partition
+    const bool bit = marks_in_ct(a);
     const dim3 blk(kBlock);
     c.ct = a.ct;
     if (!ident && a.g.n)
@@ -1810,12 +1834,15 @@ void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *cou
         if (ident) {
             if (wtd) hipLaunchKernelGGL((k_commit<true, kCommitIdent>), dim3(wgs), blk, 0, ss, c, sl);
             else hipLaunchKernelGGL((k_commit<false, kCommitIdent>), dim3(wgs), blk, 0, ss, c, sl);
+        } else if (bit) {
+            if (wtd) hipLaunchKernelGGL((k_commit<true, kCommitPackedBit>), dim3(wgs), blk, 0, ss, c, sl);
+            else hipLaunchKernelGGL((k_commit<false, kCommitPackedBit>), dim3(wgs), blk, 0, ss, c, sl);
         } else {
             if (wtd) hipLaunchKernelGGL((k_commit<true, kCommitPacked>), dim3(wgs), blk, 0, ss, c, sl);
             else hipLaunchKernelGGL((k_commit<false, kCommitPacked>), dim3(wgs), blk, 0, ss, c, sl);
         }
     }
-    commit_big_rows(a, c, lists, counts, wtd, ident, s);
+    commit_big_rows(a, c, lists, counts, wtd, ident, bit, s);
     if (fork) {
         LV_HIP(hipEventRecord(side->join[0], ss));
         LV_HIP(hipStreamWaitEvent(s, side->join[0], 0));
@@ -1824,7 +1851,7 @@ void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *cou
 
 // Large rows (buckets 6, 7, hubs): movers compacted, then the persistent commit.
 static void commit_big_rows(const SweepArgs &a, const CommitCtx &c, uint32_t *const *lists,
-                            const uint32_t *counts, bool wtd, bool ident, hipStream_t s) {
+                            const uint32_t *counts, bool wtd, bool ident, bool bit, hipStream_t s) {
     const dim3 blk(kBlock);
     BigLists bl;
     uint32_t listed = 0, items = 0;
@@ -1845,8 +1872,13 @@ static void commit_big_rows(const SweepArgs &a, const CommitCtx &c, uint32_t *co
         if (wtd) hipLaunchKernelGGL((k_commit_big<true, kCommitIdent>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
         else hipLaunchKernelGGL((k_commit_big<false, kCommitIdent>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
     } else {
-        if (wtd) hipLaunchKernelGGL((k_commit_big<true, kCommitPacked>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
-        else hipLaunchKernelGGL((k_commit_big<false, kCommitPacked>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
+        if (bit) {
+            if (wtd) hipLaunchKernelGGL((k_commit_big<true, kCommitPackedBit>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
+            else hipLaunchKernelGGL((k_commit_big<false, kCommitPackedBit>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
+        } else {
+            if (wtd) hipLaunchKernelGGL((k_commit_big<true, kCommitPacked>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
+            else hipLaunchKernelGGL((k_commit_big<false, kCommitPacked>), grid, blk, 0, s, c, bl, a.movers, a.n_movers);
+        }
     }
 }
 
@@ -1861,7 +1893,7 @@ void finalize_pass(const SweepArgs &a, uint32_t *const *next_lists, uint32_t *ne
     hipLaunchKernelGGL(k_finalize, dim3((n + per - 1) / per), dim3(kBlock), 0, s,
                        const_cast<uint32_t *>(a.comm), (const uint32_t *)a.target,
                        (const uint32_t *)a.tot, a.mark, a.bucket, n, lp, next_counts, a.stats,
-                       a.bt.gid, a.bt.sumsq);
+                       a.bt.gid, a.bt.sumsq, marks_in_ct(a) ? a.ct : nullptr);
 }
 
 void renumber_flags(const uint32_t *tot, uint32_t n, uint32_t *flags, hipStream_t s) {
