@@ -1793,9 +1793,11 @@ void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
 static void commit_big_rows(const SweepArgs &a, const CommitCtx &c, uint32_t *const *lists,
                             const uint32_t *counts, bool wtd, bool ident, bool bit, hipStream_t s);
 
-// A packed commit keeps its marks in bit 63 of ct while every id is below 2^31.
+// While every id is below 2^31 every commit is packed and keeps its marks in
+// bit 63 of ct (at the identity partition too: a neighbour's word then holds
+// the neighbour itself and its target).
 static bool marks_in_ct(const SweepArgs &a) {
-    return a.kind != 0 && a.ct != nullptr && a.g.n <= kMarkBitMaxN;
+    return a.ct != nullptr && a.g.n <= kMarkBitMaxN;
 }
 
 void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *counts,
@@ -1805,8 +1807,10 @@ void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *cou
     c.comm = a.comm; c.target = a.target; c.tot = a.tot; c.mark = a.mark; c.stats = a.stats;
     c.gid = a.bt.gid; c.g_dsig = a.bt.dsig;
     const bool wtd = a.g.wt != nullptr;
-    const bool ident = a.kind == 0;   // fast sweeps run at the identity partition
     const bool bit = marks_in_ct(a);
+    // fast sweeps run at the identity partition: without the mark bit their
+    // commit reads a neighbour's target only
+    const bool ident = a.kind == 0 && !bit;
     const dim3 blk(kBlock);
     c.ct = a.ct;
     if (!ident && a.g.n)
