@@ -225,6 +225,12 @@ lv_status lv_last_run_kernel_stats(lv_kernel_stats *s);
 lv_status lv_bandwidth_probe(uint64_t bytes, int reps, double *gbs);
 /* Kernel name of best-move variant v for gain mode `gain` (as rocprofv3 lists it). */
 lv_status lv_kernel_variant_name(int variant, int gain, char *buf, size_t len);
+/* Test hook, no counterpart in the reference: on != 0 makes every commit keep
+ * its next-sweep marks in the byte array, the path of graphs with 2^31 or more
+ * vertices (below that, marks ride in bit 63 of each vertex's packed
+ * (community, target) word); 0 restores the default.  Process-wide; set it
+ * only while no run is in flight. */
+lv_status lv_debug_mark_bytes(int32_t on);
 
 /* ---- batches: many independent graphs in one launch sequence ----
  * No counterpart in the reference (it runs one graph per process, main.cu:16-22).

@@ -496,6 +496,11 @@ lv_status lv_bandwidth_probe(uint64_t bytes, int reps, double *gbs) {
     });
 }
 
+lv_status lv_debug_mark_bytes(int32_t on) {
+    lv::set_mark_bytes(on != 0);
+    return LV_OK;
+}
+
 lv_status lv_kernel_variant_name(int variant, int gain, char *buf, size_t len) {
     if (variant < 0 || variant >= lv::kMoveVariants || !buf || len == 0) return LV_ERR_ARG;
     const int *T = lv::kTableT;

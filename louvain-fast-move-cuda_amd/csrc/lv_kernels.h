@@ -135,6 +135,10 @@ struct SweepArgs {
 // before timing move_bucket on a hash sweep.
 void pack_comm_tot(const SweepArgs &a, hipStream_t s);
 
+// Commits keep their marks in the byte array even below 2^31 vertices (the
+// lv_debug_mark_bytes test hook).
+void set_mark_bytes(bool on);
+
 // One LDS bucket's best-move launch with tile candidate `sel` (the sweep kind
 // and the bucket's list as in move_pass); used by the tile tuner.
 void move_bucket(const SweepArgs &a, int bucket, int sel, const uint32_t *list, uint32_t count,

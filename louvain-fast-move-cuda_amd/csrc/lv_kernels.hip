@@ -1793,11 +1793,14 @@ void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
 static void commit_big_rows(const SweepArgs &a, const CommitCtx &c, uint32_t *const *lists,
                             const uint32_t *counts, bool wtd, bool ident, bool bit, hipStream_t s);
 
+static std::atomic<bool> g_mark_bytes{false};
+void set_mark_bytes(bool on) { g_mark_bytes.store(on); }
+
 // While every id is below 2^31 every commit is packed and keeps its marks in
 // bit 63 of ct (at the identity partition too: a neighbour's word then holds
 // the neighbour itself and its target).
 static bool marks_in_ct(const SweepArgs &a) {
-    return a.ct != nullptr && a.g.n <= kMarkBitMaxN;
+    return a.ct != nullptr && a.g.n <= kMarkBitMaxN && !g_mark_bytes.load();
 }
 
 void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *counts,

@@ -138,6 +138,7 @@ SIGNATURES = {
     "lv_kernel_stats_get": (C.c_int, [_vp, C.POINTER(KernelStats)]),
     "lv_last_run_kernel_stats": (C.c_int, [C.POINTER(KernelStats)]),
     "lv_kernel_variant_name": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.c_size_t]),
+    "lv_debug_mark_bytes": (C.c_int, [C.c_int32]),
     "lv_bandwidth_probe": (C.c_int, [C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
     "lv_batch_create": (C.c_int, [C.POINTER(_vp), C.c_uint32, _vp, C.POINTER(_vp)]),
     "lv_batch_destroy": (None, [_vp]),
@@ -561,6 +562,12 @@ def _rec(s):
         v = getattr(s, name)
         out[name] = list(v) if isinstance(v, C.Array) else v
     return out
+
+
+def debug_mark_bytes(on):
+    """Test hook: commits keep their marks in the byte array (the path of
+    graphs with 2^31 or more vertices) while `on`; process-wide."""
+    _check(lib.lv_debug_mark_bytes(1 if on else 0))
 
 
 def bandwidth_probe(nbytes=1 << 30, reps=5):

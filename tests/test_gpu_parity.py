@@ -198,6 +198,27 @@ def test_fast_sweep_identity():
         assert np.array_equal(tg, to)
 
 
+def test_commit_mark_bytes_path():
+    """Graphs below 2^31 vertices keep the commit's marks in bit 63 of each
+    vertex's (community, target) word; larger ones use the byte array.  The
+    test hook forces the byte path: full runs still equal the oracle."""
+    rng = np.random.default_rng(23)
+    ea, eb = _random_graph(rng, n=20000, m=60000, hubs=2, hub_deg=7000, selfloops=20)
+    w = rng.integers(1, 8, len(ea)).astype(np.uint32)
+    L.debug_mark_bytes(True)
+    try:
+        for (mn, gm, om), (gn, gg, og_) in itertools.product(MODES[:2], GAINS):
+            label = f"mark bytes {mn} {gn}"
+            say(label)
+            a, b = run_both((ea, eb), L.params(mode=gm, gain=gg, sweep_threshold=1e-6,
+                                               level_threshold=1e-6),
+                            O.params(mode=om, gain=og_, sweep_threshold=1e-6,
+                                     level_threshold=1e-6), w)
+            compare_runs(a, b, label)
+    finally:
+        L.debug_mark_bytes(False)
+
+
 def _table_slots(d):
     """Slots of the best-move LDS table of a row of d entries (d + d/2 + 1,
     rounded up to 4)."""
