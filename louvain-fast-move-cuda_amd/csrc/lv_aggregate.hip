@@ -42,19 +42,6 @@ struct AggCtx {
     uint32_t *rcount;          // [n_new] distinct destinations per community
 };
 
-// Barrier over the G lanes that share one community: a wavefront barrier with
-// LDS ordering for groups inside one wave, the workgroup barrier otherwise.
-template <int G>
-__device__ inline void group_barrier() {
-    if constexpr (G == kBlock) {
-        __syncthreads();
-    } else {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-}
-
 // Largest i in [0, n) with p[i] <= k, for a staged LDS prefix (p[0] <= k).
 __device__ inline uint32_t lds_member(const uint64_t *p, uint32_t n, uint64_t k) {
     uint32_t lo = 0, hi = n;

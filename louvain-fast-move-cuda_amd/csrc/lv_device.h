@@ -133,6 +133,21 @@ __device__ inline uint32_t slot_hash_dyn(uint32_t c, uint32_t log2t) {
     return (c * 2654435761u) >> (32 - log2t);
 }
 
+// Barrier over the G lanes that share one vertex or community: a wavefront
+// barrier with LDS ordering for groups inside one wave (G <= 64), the
+// workgroup barrier otherwise.
+template <int G>
+__device__ inline void group_barrier() {
+    static_assert(G <= 64 || G == kBlock, "a group is inside one wave or the whole workgroup");
+    if constexpr (G == kBlock) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
 // Slot in [0, t) of key c for tables of any size t (multiplicative hash,
 // scaled by a high multiply).
 __device__ inline uint32_t slot_hash_mod(uint32_t c, uint32_t t) {

@@ -518,7 +518,9 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
         // the vertex's table: table_slots(degree) <= T slots
         const uint32_t tv = live ? table_slots(end - beg) : 4u;
         for (uint32_t s = lane; s < tv; s += G) { K[s] = LV_EMPTY; V[s] = 0u; }
-        __syncthreads();
+        // a vertex's table is its group's alone: groups inside one wave wait
+        // for no other wave of the workgroup
+        group_barrier<G>();
         uint64_t nself = 0;
         if (live) {
             // kIlp entries per lane in flight: loads first, then gathers, then inserts
@@ -567,7 +569,7 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
             }
         }
         if constexpr (PERSIST) nw = move_words<PK>(a, nv, nvalid);
-        __syncthreads();
+        group_barrier<G>();
         S best = SC::none();
         uint32_t best_c = a.highest ? 0u : LV_EMPTY;
         uint32_t self = 0;
