@@ -78,6 +78,9 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
     uint32_t *V = s_val + gi * T;
     uint64_t k0 = 0;
     uint32_t lg = 1;
+    // the community's words together: row bounds and member range
+    const uint32_t m0 = valid ? a.mstart[c] : 0u;
+    const uint32_t m1 = valid ? a.mstart[c + 1] : 0u;
     if (valid) {
         k0 = a.crow[c];
         lg = table_log2(a.crow[c + 1] - k0);   // at most 2x the virtual entries, <= T
@@ -85,12 +88,11 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
     const uint32_t tv = 1u << lg, mask = tv - 1u;
     for (uint32_t s = lane; s < tv; s += G) { K[s] = LV_EMPTY; V[s] = 0u; }
     if (lane == 0) { s_cnt[gi] = 0u; s_pos[gi] = 0u; }
-    __syncthreads();
+    // a community's table is its group's alone
+    group_barrier<G>();
     // members staged G at a time: row start and first virtual entry in LDS
     __shared__ uint64_t s_mrow[GPB][G];
     __shared__ uint64_t s_mp[GPB][G + 1];
-    const uint32_t m0 = valid ? a.mstart[c] : 0u;
-    const uint32_t m1 = valid ? a.mstart[c + 1] : 0u;
     for (uint32_t mb = m0; mb < m1; mb += G) {
         const uint32_t nm = m1 - mb < (uint32_t)G ? m1 - mb : (uint32_t)G;
         if ((uint32_t)lane < nm) {
@@ -140,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
         }
         group_barrier<G>();
     }
-    __syncthreads();
+    group_barrier<G>();
     // distinct (destination, weight) pairs in table order; the row order is not
     // part of the result (every consumer is order independent; exported graphs
     // are put in destination order on the way out)
