@@ -117,11 +117,6 @@ __device__ inline bool vertex_graph(const BatchView &b, uint32_t v, int64_t m2, 
 // evicting the gathered comm/tot lines from L2.
 __device__ inline uint32_t stream_load(const uint32_t *p) { return __builtin_nontemporal_load(p); }
 
-template <int LOG2T>
-__device__ inline uint32_t slot_hash(uint32_t c) {
-    return (c * 2654435761u) >> (32 - LOG2T);
-}
-
 // log2 of the smallest power of two >= 2*d (at least 2 slots): an
 // open-addressing table for d keys then stays at most half full.
 __device__ inline uint32_t table_log2(uint64_t d) {
@@ -187,8 +182,9 @@ struct SumTable {
     uint32_t n_used;
 };
 
-// Workgroup-wide: empty table (call once, then the table is left empty by
-// every sum_piece / sum_clear_used).
+// Workgroup-wide: empty table (call once; a caller leaves it empty again by
+// clearing the slots it read after each sum_piece, and an overflowing
+// sum_piece clears it itself).
 __device__ inline void sum_init(SumTable &t) {
     for (int i = threadIdx.x; i < kSumLds; i += blockDim.x) { t.keys[i] = LV_EMPTY; t.vals[i] = 0u; }
     if (threadIdx.x == 0) t.n_used = 0;
