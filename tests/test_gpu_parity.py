@@ -370,6 +370,19 @@ def test_rmat_scale16_parity():
         compare_runs(a, b, f"rmat16 {mn}")
 
 
+def test_rmat_scale18_parity():
+    """R-MAT scale 18 (262 K vertices, 7 M entries: hub rows, giant
+    communities, every LDS bucket) at the benchmark's thresholds, trace for
+    trace against the oracle in the hash and sort modes and both gains."""
+    ea, eb, n = L.gen_rmat(18, 16, 0.57, 0.19, 0.19, seed=18)
+    for (mn, gm, om), (gn, gg, og_) in [(MODES[0], GAINS[0]), (MODES[1], GAINS[0]),
+                                         (MODES[0], GAINS[1])]:
+        label = f"rmat18 {mn} {gn}"
+        say(label)
+        a, b = run_both((ea, eb), L.params(mode=gm, gain=gg), O.params(mode=om, gain=og_))
+        compare_runs(a, b, label)
+
+
 def test_chunk_emulation_parity(fixture_path):
     for mode in (MODES[0], MODES[1]):
         say(f"chunk emulation {mode[0]}")
