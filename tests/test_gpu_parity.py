@@ -383,6 +383,32 @@ def test_rmat_scale18_parity():
         compare_runs(a, b, label)
 
 
+def test_rmat22_full_size_golden_trace():
+    """The benchmark workload at full size (R-MAT scale 22, 2.4 M vertices,
+    128 M entries) against the oracle's trace of the same run, computed once
+    by tests/golden/make_rmat22_trace.py: every sweep record (active, moved,
+    entries, exact sigma_in and sum of tot^2, Q, FNV hash of the partition),
+    the final modularity, communities and the final partition's SHA-256."""
+    import hashlib
+    import json
+    import os
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                       "rmat22_hash_trace.json")))
+    ea, eb, n = L.gen_rmat(22, 16, 0.57, 0.19, 0.19, seed=2)
+    g = L.Graph.from_edges(ea, eb)
+    assert g.n == gold["n"]
+    say("rmat22 full size")
+    r = g.louvain(L.params(mode=L.HASH), hash_trace=True)
+    got = [{k: s[k] for k in REC_KEYS} for s in r["sweeps"]]
+    assert len(got) == gold["total_sweeps"]
+    for i, (x, y) in enumerate(zip(got, gold["sweeps"])):
+        assert x == y, f"sweep record {i} differs\nGPU    {x}\noracle {y}"
+    assert r["modularity"] == gold["modularity"]
+    assert r["n_comm"] == gold["n_comm"] and r["levels"] == gold["levels"]
+    assert hashlib.sha256(np.ascontiguousarray(r["best"], np.uint32).tobytes()).hexdigest() \
+        == gold["best_sha256"]
+
+
 def test_chunk_emulation_parity(fixture_path):
     for mode in (MODES[0], MODES[1]):
         say(f"chunk emulation {mode[0]}")
