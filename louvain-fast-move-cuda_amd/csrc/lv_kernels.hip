@@ -484,8 +484,8 @@ __device__ inline MoveWords move_words(const MoveCtx &a, uint32_t v, bool valid)
 // the list with a stride of the grid, the next vertex's list entry is loaded
 // when a vertex starts and its words when its inserts are done, so both
 // latencies pass under this vertex's barrier and scoring, and the statistics
-// are reduced once per workgroup.  The grid is then at most what the device
-// holds at once (persistent_grid).
+// are reduced once per workgroup.  The grid is a few rounds of the
+// workgroups the device holds at once (kPersistRounds).
 template <int G, int T, int GAIN, bool WEIGHTED, bool PK>
 __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t *__restrict__ list,
                                                        uint32_t count) {
@@ -753,7 +753,11 @@ struct HubCtx {
 constexpr int kHubPartBits = ilog2(kHubParts);
 constexpr int kHubSubBits = 32 - kHubPartBits;            // mix bits below the partition
 constexpr uint64_t kHubGroupEntries = 4096;                // entries per reduce group
-constexpr uint32_t kHubReduceBlocks = 2048;                // persistent grid of k_hub_reduce
+// Grid of the persistent k_hub_reduce: about eight times the workgroups
+// resident at once.  The items are uneven, and a grid that the device runs in
+// several rounds balances them better than one that keeps every workgroup
+// resident (measured: 2,048 and the resident count were slower).
+constexpr uint32_t kHubReduceBlocks = 8192;
 constexpr int kHubIlp = 16;                                // pair loads / tot gathers in flight per lane
 static_assert(kHubParts == 32, "items pack the group in 5 bits");
 
@@ -1258,7 +1262,9 @@ __global__ __launch_bounds__(kBlock) void k_commit(CommitCtx a, SmallLists sl) {
 // first sweep, so the movers are compacted first (per bucket, one atomic per
 // wave) and the commit runs over (mover, part) items on a persistent grid.
 constexpr uint32_t kBigParts[kBigBuckets] = {1u, 4u, (uint32_t)kHubParts};
-constexpr uint32_t kCommitBlocks = 2048;
+// Grid of k_commit_big: many rounds of resident workgroups, for the same
+// reason (measured faster than 2,048 to 16,384).
+constexpr uint32_t kCommitBlocks = 32768;
 
 struct BigLists {
     const uint32_t *list[kBigBuckets];
@@ -1595,6 +1601,10 @@ namespace {
 // per-candidate total gathers rather than 50% more LDS per vertex.
 // Workgroups of `kernel` the device holds at once (CUs x occupancy), cached
 // in `slot` at the first launch (every device in use is the same model).
+// The persistent kernels launch kPersistRounds times that: each workgroup
+// still walks several vertices with the next one in flight, and the uneven
+// rows are balanced in rounds (measured faster than one resident wave).
+constexpr uint32_t kPersistRounds = 4;
 template <typename K>
 uint32_t resident_blocks(K kernel, std::atomic<uint32_t> &slot) {
     uint32_t r = slot.load(std::memory_order_relaxed);
@@ -1614,9 +1624,9 @@ void launch_move_table_k(const MoveCtx &m, const uint32_t *list, uint32_t count,
     constexpr bool PK = T <= kPackedMaxT;
     auto kernel = k_move_table<G, T, GAIN, WEIGHTED, PK>;
     uint32_t blocks = (count + GPB - 1) / GPB;
-    if (G == kBlock) {   // persistent: at most one wave of resident workgroups
+    if (G == kBlock) {   // persistent: a few rounds of resident workgroups
         static std::atomic<uint32_t> resident{0};
-        blocks = std::min(blocks, resident_blocks(kernel, resident));
+        blocks = std::min(blocks, kPersistRounds * resident_blocks(kernel, resident));
     }
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), 0, s, m, list, count);
 }
