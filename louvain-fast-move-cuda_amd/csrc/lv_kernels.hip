@@ -1213,7 +1213,11 @@ __device__ inline int64_t commit_row(const CommitCtx &a, uint32_t v, uint32_t ow
 // A group of G (<= 64) lanes per listed vertex of buckets 0-5, all six
 // buckets in one launch: workgroup b takes bucket q with wg[q] <= b < wg[q+1].
 constexpr int kSmallBuckets = kNumBuckets - kBigBuckets;
-constexpr int kSmallG[kSmallBuckets] = {4, 8, 16, 32, 64, 64};
+// Lanes per vertex: a few entries per lane (up to 4-16, one to four kIlp
+// steps), so a wave commits many vertices and keeps many independent chains
+// of row, neighbour word and mark in flight (measured: {4, 8, 16, 32, 64, 64},
+// one lane per entry, was 15% slower on the 4,096-graph batch).
+constexpr int kSmallG[kSmallBuckets] = {2, 2, 4, 8, 8, 16};
 
 struct SmallLists {
     const uint32_t *list[kSmallBuckets];
@@ -1248,6 +1252,7 @@ __global__ __launch_bounds__(kBlock) void k_commit(CommitCtx a, SmallLists sl) {
     uint32_t v = LV_EMPTY;
     int64_t dsig = 0;
     switch (kSmallG[q]) {   // uniform over the workgroup
+        case 2: dsig = commit_group<2, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v); break;
         case 4: dsig = commit_group<4, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v); break;
         case 8: dsig = commit_group<8, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v); break;
         case 16: dsig = commit_group<16, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v); break;
