@@ -534,6 +534,8 @@ static AggCtx ctx_of(const AggArgs &x) {
 void agg_tables(const AggArgs &x, uint32_t *const *lists, const uint32_t *counts, hipStream_t s) {
     const AggCtx a = ctx_of(x);
     const bool wtd = x.g.wt != nullptr;
+    // lanes per community in the small buckets: a quarter of the table's
+    // slots, so a wave holds many communities (measured faster than half)
     for (int b = 0; b < kHubBucket; ++b) {
         const uint32_t cnt = counts[b];
         if (!cnt) continue;
@@ -545,11 +547,11 @@ void agg_tables(const AggArgs &x, uint32_t *const *lists, const uint32_t *counts
         else hipLaunchKernelGGL((k_agg_table<GG, TT, false>), grid, dim3(kBlock), 0, s, a, L, cnt); \
     } while (0)
         switch (b) {
-            case 0: LV_AGG(4, 8); break;
-            case 1: LV_AGG(8, 16); break;
-            case 2: LV_AGG(16, 32); break;
-            case 3: LV_AGG(32, 64); break;
-            case 4: LV_AGG(64, 128); break;
+            case 0: LV_AGG(2, 8); break;
+            case 1: LV_AGG(4, 16); break;
+            case 2: LV_AGG(8, 32); break;
+            case 3: LV_AGG(16, 64); break;
+            case 4: LV_AGG(32, 128); break;
             case 5: LV_AGG(64, 512); break;
             case 6: LV_AGG(256, 2048); break;
             default: LV_AGG(256, 8192); break;
@@ -613,12 +615,12 @@ void agg_copy(const AggArgs &x, uint32_t *const *lists, const uint32_t *counts,
                            (uint32_t)(PARTS), out_row, col, w);                                 \
     } while (0)
         switch (b) {
-            case 0: LV_CP(4, 1); break;
-            case 1: LV_CP(8, 1); break;
-            case 2: LV_CP(16, 1); break;
-            case 3: LV_CP(32, 1); break;
-            case 4: LV_CP(64, 1); break;
-            case 5: LV_CP(64, 1); break;
+            case 0: LV_CP(2, 1); break;
+            case 1: LV_CP(4, 1); break;
+            case 2: LV_CP(8, 1); break;
+            case 3: LV_CP(16, 1); break;
+            case 4: LV_CP(32, 1); break;
+            case 5: LV_CP(32, 1); break;
             case 6: LV_CP(256, 1); break;
             default: LV_CP(256, 4); break;
         }
