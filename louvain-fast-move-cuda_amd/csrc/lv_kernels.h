@@ -43,13 +43,14 @@ static_assert(kTableT[0] == 8 && kTableT[5] == 388 && kTableT[6] == 1540 && kTab
 
 // Tile (lanes per vertex) candidates of each LDS bucket's best-move kernels;
 // the first is the default, 0 ends a list.  The LDS table of a bucket stays
-// kTableT whatever the tile.
+// kTableT whatever the tile.  Few lanes per vertex (a few entries per lane)
+// win in the small buckets: a wave then holds many vertices' chains.
 constexpr int kTileChoices = 3;
 // best-move tables of at most this many slots carry each community's total
 // (PK); bucket 7's would need 72 KB of LDS per vertex packed, 48 KB unpacked
 constexpr int kPackedMaxT = kTableT[6];
 constexpr int kTileG[kNumBuckets - 1][kTileChoices] = {
-    {4, 2, 8}, {8, 4, 16}, {16, 8, 32}, {32, 16, 64}, {64, 32, 0}, {64, 256, 0}, {256, 0, 0},
+    {2, 1, 4}, {4, 2, 8}, {8, 4, 16}, {16, 8, 32}, {32, 16, 64}, {64, 32, 256}, {256, 0, 0},
     {256, 0, 0}};
 // Chosen candidate per (bucket, fast sweep or not).
 struct TileChoice {

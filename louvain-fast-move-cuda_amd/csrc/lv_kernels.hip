@@ -1667,12 +1667,12 @@ void launch_table_tile(int b, int G, const MoveCtx &m, bool wtd, int gain, const
                        uint32_t cnt, hipStream_t s) {
 #define LV_T(GG, TT) launch_move_table<GG, TT>(m, wtd, gain, L, cnt, s)
     switch (b) {
-        case 0: if (G == 2) LV_T(2, kTableT[0]); else if (G == 8) LV_T(8, kTableT[0]); else LV_T(4, kTableT[0]); break;
-        case 1: if (G == 4) LV_T(4, kTableT[1]); else if (G == 16) LV_T(16, kTableT[1]); else LV_T(8, kTableT[1]); break;
-        case 2: if (G == 8) LV_T(8, kTableT[2]); else if (G == 32) LV_T(32, kTableT[2]); else LV_T(16, kTableT[2]); break;
-        case 3: if (G == 16) LV_T(16, kTableT[3]); else if (G == 64) LV_T(64, kTableT[3]); else LV_T(32, kTableT[3]); break;
-        case 4: if (G == 32) LV_T(32, kTableT[4]); else LV_T(64, kTableT[4]); break;
-        case 5: if (G == 256) LV_T(256, kTableT[5]); else LV_T(64, kTableT[5]); break;
+        case 0: if (G == 1) LV_T(1, kTableT[0]); else if (G == 4) LV_T(4, kTableT[0]); else LV_T(2, kTableT[0]); break;
+        case 1: if (G == 2) LV_T(2, kTableT[1]); else if (G == 8) LV_T(8, kTableT[1]); else LV_T(4, kTableT[1]); break;
+        case 2: if (G == 4) LV_T(4, kTableT[2]); else if (G == 16) LV_T(16, kTableT[2]); else LV_T(8, kTableT[2]); break;
+        case 3: if (G == 8) LV_T(8, kTableT[3]); else if (G == 32) LV_T(32, kTableT[3]); else LV_T(16, kTableT[3]); break;
+        case 4: if (G == 16) LV_T(16, kTableT[4]); else if (G == 64) LV_T(64, kTableT[4]); else LV_T(32, kTableT[4]); break;
+        case 5: if (G == 32) LV_T(32, kTableT[5]); else if (G == 256) LV_T(256, kTableT[5]); else LV_T(64, kTableT[5]); break;
         case 6: LV_T(256, kTableT[6]); break;
         default: LV_T(256, kTableT[7]); break;
     }
@@ -1682,6 +1682,7 @@ void launch_table_tile(int b, int G, const MoveCtx &m, bool wtd, int gain, const
 void launch_fast_tile(int G, const MoveCtx &m, bool wtd, int gain, const uint32_t *L, uint32_t cnt,
                       hipStream_t s) {
     switch (G) {
+        case 1: launch_move_fast<1>(m, wtd, gain, L, cnt, s); break;
         case 2: launch_move_fast<2>(m, wtd, gain, L, cnt, s); break;
         case 4: launch_move_fast<4>(m, wtd, gain, L, cnt, s); break;
         case 8: launch_move_fast<8>(m, wtd, gain, L, cnt, s); break;
