@@ -21,7 +21,7 @@ constexpr int kNumBuckets = 9;        // 8 LDS-table buckets + global-table hubs
 constexpr int kHubBucket = 8;
 constexpr int kStatSlots = 64;        // spread of the per-block statistic atomics
 constexpr int kHubParts = 32;         // row chunks and community partitions per hub vertex
-constexpr int kBigBuckets = 3;        // large-row buckets (the last three) committed over movers
+constexpr int kBigBuckets = 2;        // large-row buckets (bucket 7, hubs) committed over movers
 constexpr uint32_t kMaxChunks = 4096; // reference chunks per level (STEP_ROUND emulation)
 
 // Maximum degree handled by each LDS bucket; larger goes to the hub bucket.

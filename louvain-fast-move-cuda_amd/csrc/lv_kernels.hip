@@ -1210,14 +1210,16 @@ __device__ inline int64_t commit_row(const CommitCtx &a, uint32_t v, uint32_t ow
     return dsig;
 }
 
-// A group of G (<= 64) lanes per listed vertex of buckets 0-5, all six
+// A group of G (<= 64) lanes per listed vertex of buckets 0-6, all seven
 // buckets in one launch: workgroup b takes bucket q with wg[q] <= b < wg[q+1].
 constexpr int kSmallBuckets = kNumBuckets - kBigBuckets;
 // Lanes per vertex: a few entries per lane (up to 4-16, one to four kIlp
 // steps), so a wave commits many vertices and keeps many independent chains
 // of row, neighbour word and mark in flight (measured: {4, 8, 16, 32, 64, 64},
-// one lane per entry, was 15% slower on the 4,096-graph batch).
-constexpr int kSmallG[kSmallBuckets] = {2, 2, 4, 8, 8, 16};
+// one lane per entry, was 15% slower on the 4,096-graph batch).  Bucket 6
+// (257-1024 entries, 64 lanes) is committed here rather than as compacted
+// movers of a whole workgroup each (measured slightly faster).
+constexpr int kSmallG[kSmallBuckets] = {2, 2, 4, 8, 8, 16, 64};
 
 struct SmallLists {
     const uint32_t *list[kSmallBuckets];
@@ -1263,10 +1265,10 @@ __global__ __launch_bounds__(kBlock) void k_commit(CommitCtx a, SmallLists sl) {
     block_add_stats(a.stats, ST_DSIGMA, (uint64_t)dsig, -1, 0, -1, 0);
 }
 
-// Large rows (buckets 6 and 7, hubs): most listed vertices stay after the
+// Large rows (bucket 7, hubs): most listed vertices stay after the
 // first sweep, so the movers are compacted first (per bucket, one atomic per
 // wave) and the commit runs over (mover, part) items on a persistent grid.
-constexpr uint32_t kBigParts[kBigBuckets] = {1u, 4u, (uint32_t)kHubParts};
+constexpr uint32_t kBigParts[kBigBuckets] = {4u, (uint32_t)kHubParts};
 // Grid of k_commit_big: many rounds of resident workgroups, for the same
 // reason (measured faster than 2,048 to 16,384).
 constexpr uint32_t kCommitBlocks = 32768;
@@ -1847,7 +1849,7 @@ void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *cou
         wgs += (counts[q] + gpb - 1) / gpb;
     }
     sl.wg[kSmallBuckets] = wgs;
-    // buckets 0-5 on a side stream beside the large rows (both after k_pack_ct;
+    // buckets 0-6 on a side stream beside the large rows (both after k_pack_ct;
     // their writes are atomics or the same mark bytes)
     const bool fork = wgs && side && side->ready();
     const hipStream_t ss = fork ? side->s[0] : s;
@@ -1874,7 +1876,7 @@ void commit_pass(const SweepArgs &a, uint32_t *const *lists, const uint32_t *cou
     }
 }
 
-// Large rows (buckets 6, 7, hubs): movers compacted, then the persistent commit.
+// Large rows (bucket 7, hubs): movers compacted, then the persistent commit.
 static void commit_big_rows(const SweepArgs &a, const CommitCtx &c, uint32_t *const *lists,
                             const uint32_t *counts, bool wtd, bool ident, bool bit, hipStream_t s) {
     const dim3 blk(kBlock);
