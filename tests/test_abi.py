@@ -216,3 +216,20 @@ def test_write_edge_round_trip(tmp_path):
         r2, c2, w2 = ref.csr()[:3]
         assert np.array_equal(r1, r2) and np.array_equal(c1, c2) and np.array_equal(w1, w2)
         assert g.info()["min_id"] == -50
+
+
+def test_traffic_profile_names_the_roofline_kernel():
+    """bench.py reads the hardware traffic of its roofline kernel (the degree
+    257-1024 bucket's default hash-sweep variant on unit weights) from the
+    committed counter profile; the profile must name that kernel."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "profiles",
+                        "k_move_table_256_1540_rmat22_traffic.json")
+    t = json.load(open(path))
+    assert t["kernel"] == L.kernel_variant_name(((6 * 2 + 0) * 2 + 0) * 3 + 0)
+    # the KiB figures are stored rounded to 1/100 KiB
+    assert abs(t["traffic_bytes_per_launch"] -
+               (t["fetch_kib_per_dispatch"] * t["fetch_scale"] +
+                t["write_kib_per_dispatch"] * t["write_scale"]) * 1024) < 64
+
