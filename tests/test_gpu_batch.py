@@ -100,6 +100,27 @@ def test_batch_equals_each_graph_alone(batch_graphs, label, gp, op):
     assert all(np.array_equal(x, y) for x, y in zip(r2["best"], r["best"]))
 
 
+def test_batch_of_larger_graphs_equals_oracle():
+    """Twelve Barabasi-Albert graphs of 20 K-200 K vertices (the batched
+    configuration's upper range: rows in every LDS bucket, hubs, giant
+    communities at the contraction) as one batch, graph by graph against the
+    oracle."""
+    rng = np.random.default_rng(404)
+    graphs = []
+    for i in range(12):
+        n = int(np.exp(rng.uniform(np.log(20_000), np.log(200_000))))
+        graphs.append(L.gen_ba(n, int(rng.integers(2, 9)), seed=int(rng.integers(1 << 30))))
+    gs = [L.Graph.from_edges(a, b) for a, b in graphs]
+    say(f"larger batch: n={sum(g.n for g in gs)}, E={sum(g.E for g in gs)}")
+    r = L.Batch(gs).louvain(L.params(mode=L.HASH, autotune=1))
+    for i, (a, b) in enumerate(graphs):
+        orc = O.Graph.from_edges(a, b).louvain(O.params(mode=O.HASH))
+        got = (r["modularity"][i], int(r["n_comm"][i]), int(r["levels"][i]), int(r["sweeps"][i]))
+        exp = (orc["modularity"], orc["n_comm"], orc["levels"], orc["total_sweeps"])
+        assert got == exp, f"graph {i}: {got} != {exp}"
+        assert np.array_equal(r["best"][i], orc["best"]), f"graph {i}"
+
+
 def test_batch_of_one_and_identical_copies():
     ea, eb = L.gen_ba(3000, 3, seed=5)
     g = L.Graph.from_edges(ea, eb)
