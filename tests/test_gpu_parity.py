@@ -108,6 +108,29 @@ def test_random_graph_full_run(name, weighted):
         compare_runs(a, b, label)
 
 
+def test_heavy_weights_full_run():
+    """Weights up to 200,000 with the total weight just under the 2^31 bound
+    of 2m (exact scores 2m*w - k*tot in int64, totals in u32): full runs in
+    the hash and sort modes and both gains equal the oracle's traces."""
+    rng = np.random.default_rng(77)
+    ea, eb = _random_graph(rng, n=3000, m=7000, hubs=1, hub_deg=2500, selfloops=10)
+    w = rng.integers(1, 200_001, len(ea)).astype(np.uint64)
+    # scale so that 2m stays below 2^31 (self-loops count twice)
+    two_m = int(2 * w.sum())
+    if two_m >= (1 << 31) - 1:
+        w = np.maximum(1, (w * ((1 << 31) - 2) // (two_m + 1))).astype(np.uint64)
+    w = w.astype(np.uint32)
+    assert 2 * int(w.sum()) < (1 << 31)
+    for (mn, gm, om), (gn, gg, og_) in itertools.product(MODES[:2], GAINS):
+        label = f"heavy weights {mn} {gn}"
+        say(label)
+        a, b = run_both((ea, eb), L.params(mode=gm, gain=gg, sweep_threshold=1e-6,
+                                           level_threshold=1e-6),
+                        O.params(mode=om, gain=og_, sweep_threshold=1e-6,
+                                 level_threshold=1e-6), w)
+        compare_runs(a, b, label)
+
+
 @pytest.mark.parametrize("kind", [L.SWEEP_SORT, L.SWEEP_HASH])
 def test_best_moves_snapshots(kind):
     rng = np.random.default_rng(7 + kind)
