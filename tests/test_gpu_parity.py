@@ -385,12 +385,16 @@ def test_aggregation_giant_partitions():
 
 
 def test_rmat_scale16_parity():
+    """R-MAT scale 16 in all four modes, and the hash mode with the
+    reference's tie rule, trace for trace against the oracle."""
     ea, eb, n = L.gen_rmat(16, seed=2)
-    for (mn, gm, om) in MODES[:2]:
-        say(f"rmat16 {mn}")
-        a, b = run_both((ea, eb), L.params(mode=gm, sweep_threshold=1e-6, level_threshold=1e-6),
-                        O.params(mode=om, sweep_threshold=1e-6, level_threshold=1e-6))
-        compare_runs(a, b, f"rmat16 {mn}")
+    runs = [(m, TIES[0]) for m in MODES] + [(MODES[0], TIES[1])]
+    for (mn, gm, om), (tn, gt, ot) in runs:
+        say(f"rmat16 {mn} {tn}")
+        a, b = run_both((ea, eb), L.params(mode=gm, tie=gt, sweep_threshold=1e-6,
+                                           level_threshold=1e-6),
+                        O.params(mode=om, tie=ot, sweep_threshold=1e-6, level_threshold=1e-6))
+        compare_runs(a, b, f"rmat16 {mn} {tn}")
 
 
 def test_rmat_scale18_parity():
