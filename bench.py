@@ -9,14 +9,21 @@ One step = one full Louvain run to convergence (all levels, the reference's
 value = traversed directed entries per second over the whole run (sum over
 sweeps of the entries of the vertices examined, divided by the step time).
 The roofline block covers the dominant best-move kernel, timed with HIP
-events on the library stream; the CPU baseline is the single-threaded oracle
-on the first sweeps of level 0 of the same graph.  The `batched` sub-object
+events on the stream it runs on, and the whole best-move pass; the CPU
+baseline is the single-threaded oracle run to convergence on the same CSR.
+`per_sweep` gives traversed entries per second of the best-move pass for
+sweep 0 of level 0 and the entry-weighted mean over all sweeps; `buckets`
+gives each degree bucket's achieved bandwidth from one extra run with the
+pass on a single stream; `reference_method` is the same graph run with the
+reference's f32-rounded gains and hash-sweep highest-id ties; `c1` is the
+CPU-path configuration (the bundled 10,000-vertex fixture and a
+Barabasi-Albert graph of the same size) on the oracle.  The `batched` sub-object
 is the batched configuration (4,096 Barabasi-Albert graphs in one
 disjoint-union launch sequence, graphs/s, against a loop of per-graph runs);
 `--workload batched` prints it as its own line with value = graphs/s.  The
 `served` sub-object is the served configuration (Poisson arrivals at 200
-req/s, continuous batching, p50/p99 latency, graphs/s) over a 20 s arrival
-window; `--workload served` prints it as its own line over 60 s.
+req/s, continuous batching, p50/p99 latency, graphs/s) over a 60 s arrival
+window; `--workload served` prints it as its own line.
 """
 import argparse
 import json
@@ -31,6 +38,9 @@ sys.path.insert(0, REPO)
 
 METRIC = "Louvain s-to-converge + edges/s vs HBM roofline; batched graphs/s, p99 lat"
 HBM_PEAK_GBS = 8000.0
+FIXTURE = os.path.join(REPO, "tests", "golden", "graph-power-law-10000-2-99-1.edge")
+BUCKET_NAMES = ["deg<=4", "deg<=8", "deg<=16", "deg<=32", "deg<=64", "deg<=256", "deg<=1024",
+                "deg<=4096", "hub"]
 
 
 def log(*a):
@@ -48,18 +58,55 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(row, col, sweeps):
-    """Single-threaded oracle on the first `sweeps` sweeps of level 0."""
+def cpu_baseline(row, col, mode, threshold):
+    """The single-threaded oracle run to convergence on the same CSR (all
+    levels, the same mode and thresholds): the CPU path's s-to-converge."""
     from oracle import oracle as O
     og = O.Graph.from_csr(row, col)
+    mode_id = {"hash": O.HASH, "sort": O.SORT, "adaptive_speed": O.ADAPTIVE_SPEED,
+               "adaptive_memory": O.ADAPTIVE_MEMORY}[mode]
     t0 = time.perf_counter()
-    r = og.louvain(O.params(max_total_sweeps=sweeps))
+    r = og.louvain(O.params(mode=mode_id, sweep_threshold=threshold, level_threshold=threshold))
     dt = time.perf_counter() - t0
     ent = sum(s["e_active"] for s in r["sweeps"])
     return {"value": ent / dt, "unit": "edges/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (C, -O3, 1 thread) on level-0 sweeps 0..{sweeps - 1} of the same "
-                      f"R-MAT-22 graph: {ent} entries in {dt:.2f} s; CPU {cpu_model()}, "
-                      f"nproc {os.cpu_count()}"}
+            "s_to_converge": dt, "modularity": r["modularity"], "n_communities": r["n_comm"],
+            "sample": f"oracle (C, -O3, 1 thread) run to convergence on the same R-MAT CSR "
+                      f"({r['levels']} levels, {len(r['sweeps'])} sweeps, {mode}, exact gains, "
+                      f"thresholds {threshold:g}): {ent} traversed entries in {dt:.2f} s; "
+                      f"CPU {cpu_model()}, nproc {os.cpu_count()}"}
+
+
+def c1_result(L):
+    """The CPU-path configuration: the bundled fixture (networkx
+    powerlaw_cluster_graph(10000, 2, 0.99, seed=1)) and a literal
+    Barabasi-Albert graph (n = 10,000, m = 2, seed 1), each run to convergence
+    on the oracle at thresholds 1e-6 and 0.001, with the GPU's Q beside it."""
+    from oracle import oracle as O
+    ea, eb = L.gen_ba(10000, 2, seed=1)
+    cases = [("fixture graph-power-law-10000-2-99-1.edge", O.Graph.load_edge(FIXTURE),
+              L.Graph.load_edge(FIXTURE)),
+             ("Barabasi-Albert n=10000 m=2 seed 1", O.Graph.from_edges(ea, eb),
+              L.Graph.from_edges(ea, eb))]
+    out = []
+    for name, og, g in cases:
+        for thr in (1e-6, 1e-3):
+            t0 = time.perf_counter()
+            r = og.louvain(O.params(sweep_threshold=thr, level_threshold=thr))
+            dt = time.perf_counter() - t0
+            c = L.Community(g)
+            prm = L.params(sweep_threshold=thr, level_threshold=thr)
+            c.reset()
+            c.run(prm, best=False, n0=g.n)               # warm
+            c.reset()
+            rg = c.run(prm, best=False, n0=g.n)
+            c.close()
+            out.append({"graph": name, "threshold": thr, "cpu_s_to_converge": dt,
+                        "modularity": r["modularity"], "n_communities": r["n_comm"],
+                        "levels": r["levels"], "sweeps": len(r["sweeps"]),
+                        "gpu_s_to_converge": rg["seconds"],
+                        "gpu_same_modularity": rg["modularity"] == r["modularity"]})
+    return out
 
 
 def modularity_fp64(row, col, w, part):
@@ -397,7 +444,6 @@ def main():
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--mode", default="hash")
     ap.add_argument("--threshold", type=float, default=0.001)
-    ap.add_argument("--cpu-sweeps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe-bytes", type=int, default=1 << 30,
                     help="size of the plain copy kernel run as the bandwidth reference")
@@ -407,11 +453,11 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0, help="served: arrivals per second")
     ap.add_argument("--window", type=float, default=60.0, help="served: arrival window (s)")
     ap.add_argument("--served-mix", default="ba", choices=["ba", "planted"],
-                    help="served: the 95% requests (ba: the batched configuration's "
+                    help="served: the 95%% requests (ba: the batched configuration's "
                          "distribution; planted: planted-partition graphs of --served-n vertices)")
     ap.add_argument("--served-n", type=int, default=1_000_000,
                     help="served: vertices of the planted-partition requests")
-    ap.add_argument("--pool", type=int, default=64, help="served: distinct graphs of the 95% mix")
+    ap.add_argument("--pool", type=int, default=64, help="served: distinct graphs of the 95%% mix")
     ap.add_argument("--burst", type=int, default=400, help="served: saturation burst size")
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-batched", action="store_true",
@@ -421,7 +467,7 @@ def main():
     ap.add_argument("--tmpdir", default=None, help="where the ingest .edge file is written")
     ap.add_argument("--no-served", action="store_true",
                     help="rmat: leave out the served configuration's sub-object")
-    ap.add_argument("--served-window", type=float, default=20.0,
+    ap.add_argument("--served-window", type=float, default=60.0,
                     help="rmat: arrival window of the served sub-object (s)")
     ap.add_argument("--graphs", type=int, default=4096, help="batched: number of graphs")
     ap.add_argument("--loop-sample", type=int, default=1,
@@ -475,6 +521,62 @@ def main():
     ks = {k: (sum(s[k] for s in kstats) if not isinstance(kstats[0][k], list)
               else [sum(s[k][i] for s in kstats) for i in range(len(kstats[0][k]))])
           for k in kstats[0]}
+    # traversed entries per second of the best-move pass (HIP events), per sweep
+    all_sw = [s for r in results for s in r["sweeps"]]
+    sw0 = [s for r in results for s in r["sweeps"][:1]]
+    per_sweep = {
+        "sweep0_level0_teps": sum(s["e_active"] for s in sw0) / (sum(s["move_ms"] for s in sw0) / 1e3),
+        "weighted_mean_teps": sum(s["e_active"] for s in all_sw) / (sum(s["move_ms"] for s in all_sw) / 1e3),
+        "sweep0_level0_move_ms": sum(s["move_ms"] for s in sw0) / len(sw0),
+        "sweeps": [{"level": s["level"], "sweep": s["sweep"], "kind": s["kind"],
+                    "e_active": s["e_active"], "n_active": s["n_active"],
+                    "candidates": s["candidates"], "move_ms": s["move_ms"],
+                    "sweep_ms": s["sweep_ms"],
+                    "move_gbs": s["move_bytes"] / s["move_ms"] / 1e6 if s["move_ms"] > 0 else 0.0}
+                   for s in last["sweeps"]],
+        "note": "e_active / HIP-event time of the sweep's best-move pass; the weighted mean is "
+                "total entries over total pass time of the timed runs"}
+    # the reference's own method on the same resident graph
+    prm_ref = L.params(mode=args.mode, sweep_threshold=args.threshold,
+                       level_threshold=args.threshold, gain=L.GAIN_REF_F32, tie=L.TIE_REFERENCE,
+                       autotune=0)
+    c.reset()
+    c.run(prm_ref, best=False, n0=info["n"])
+    c.reset()
+    t0 = time.perf_counter()
+    rr = c.run(prm_ref, best=False, n0=info["n"])
+    ref_s = time.perf_counter() - t0
+    q_ex, q_rf = last["modularity"], rr["modularity"]
+    reference_method = {
+        "modularity": q_rf, "n_communities": rr["n_comm"], "levels": rr["levels"],
+        "sweeps": rr["total_sweeps"], "s_to_converge": ref_s,
+        "relative_q_difference": (q_ex - q_rf) / abs(q_rf),
+        "within_half_percent_below": q_ex >= (1 - 0.005) * q_rf,
+        "within_half_percent_two_sided": abs(q_ex - q_rf) <= 0.005 * abs(q_rf),
+        "note": "gain=ref_f32 (operatorsDelta.h:19-20) with highest-id ties in hash sweeps "
+                "(phaseOptimizationHash.cu:23-35); relative_q_difference = (Q_default - Q_ref) "
+                "/ Q_ref"}
+    log(f"reference method: Q={q_rf:.9f} (default {q_ex:.9f}), {ref_s * 1e3:.1f} ms")
+    # each bucket alone: one run with the pass on a single stream, every bucket timed
+    prm3 = L.params(mode=args.mode, sweep_threshold=args.threshold,
+                    level_threshold=args.threshold, profile=3, autotune=0)
+    c.reset()
+    c.run(prm3, best=False, n0=info["n"])
+    kb = c.kernel_stats()
+    buckets = []
+    for b in range(9):
+        for f in (1, 0):
+            i = 2 * b + f
+            if not kb["bucket_launches"][i]:
+                continue
+            ms = kb["bucket_ms"][i]
+            gbs = kb["bucket_bytes"][i] / ms / 1e6 if ms > 0 else 0.0
+            buckets.append({"bucket": BUCKET_NAMES[b], "sweep": "fast" if f else args.mode,
+                            "launches": kb["bucket_launches"][i], "ms": ms,
+                            "entries": kb["bucket_entries"][i], "bytes": kb["bucket_bytes"][i],
+                            "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS})
+    log("buckets: " + ", ".join(f"{x['bucket']}/{x['sweep']} {x['ms']:.2f} ms {x['achieved_gbs']:.0f} GB/s"
+                                 for x in buckets))
     move_s = ks["move_ms"] / 1e3
     pass_gbs = ks["move_bytes"] / move_s / 1e9 if move_s > 0 else 0.0
     # dominant best-move kernel: the degree 257-1024 bucket's variant (one
@@ -488,7 +590,6 @@ def main():
     v_ms = ks["variant_ms"][vi] / max(1, v_launch)
     v_bytes = ks["variant_bytes"][vi] / max(1, v_launch)
     achieved = v_bytes / (v_ms / 1e3) / 1e9 if v_ms > 0 else 0.0
-    sw0 = last["sweeps"][0]
     # hardware traffic of the dominant kernel from the committed counter runs
     traffic = None
     tf = os.path.join(REPO, "profiles", "k_move_table_256_1540_rmat22_traffic.json")
@@ -518,7 +619,7 @@ def main():
         "n_communities": last["n_comm"],
         "levels": last["levels"],
         "sweeps": last["total_sweeps"],
-        "sweep0_entries": sw0["e_active"],
+        "sweep0_entries": last["sweeps"][0]["e_active"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": L.kernel_variant_name(vi),
@@ -528,12 +629,21 @@ def main():
                                         "frac": pass_gbs / HBM_PEAK_GBS,
                                         "ms_per_run": ks["move_ms"] / args.steps,
                                         "sweeps_per_run": ks["move_launches"] / args.steps}},
+        "per_sweep": per_sweep,
+        "buckets": buckets,
+        "reference_method": reference_method,
     }
     if not args.no_cpu:
         row, col, _ = g.csr()
-        log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(row, col, args.cpu_sweeps)
+        log("cpu baseline (oracle to convergence) ...")
+        cb = cpu_baseline(row, col, args.mode, args.threshold)
+        out["cpu_baseline"] = cb
+        out["speedup_s_to_converge_vs_cpu"] = cb["s_to_converge"] / out["s_to_converge"]
+        out["cpu_same_modularity"] = cb["modularity"] == out["modularity"]
+        log(f"cpu: {cb['s_to_converge']:.1f} s, Q={cb['modularity']:.9f}")
         del row, col
+        log("c1 (CPU-path configuration) ...")
+        out["c1"] = c1_result(L)
     del c, g
     if not args.no_ingest:
         log("ingest ...")

@@ -61,7 +61,9 @@ typedef struct lv_params {
     int32_t profile;          /* 1: time every best-move launch with HIP events;
                                  2: time the sweeps, the best-move passes and the
                                     degree 257-1024 bucket's launches only (the one
-                                    that runs with the device to itself) */
+                                    that runs with the device to itself);
+                                 3: as 1, with the best-move pass on one stream
+                                    (each bucket timed alone; slower runs) */
     int32_t autotune;         /* 1: on a community's first fast and first hash sweep, time
                                  each degree bucket's tile candidates and keep the fastest
                                  (decisions, hence results, do not depend on the tile) */
@@ -79,6 +81,13 @@ typedef struct lv_sweep_rec {
     uint64_t sum_tot_sq;       /* sum over communities of tot_c^2 */
     double q;
     uint64_t comm_hash;        /* FNV-1a 64 over comm[] (0 unless requested) */
+    /* with lv_params.profile set (else 0): HIP-event time of the sweep's
+     * best-move pass and of the whole sweep, the distinct (vertex, neighbour
+     * community) candidates it scored, and its algorithmic bytes
+     * beta*e_active + 8*candidates + 28*n_active + n (beta = 8 at unit
+     * weights, 12 weighted) */
+    double move_ms, sweep_ms;
+    uint64_t candidates, move_bytes;
 } lv_sweep_rec;
 
 typedef struct lv_level_rec {
@@ -118,6 +127,14 @@ typedef struct lv_kernel_stats {
     double variant_ms[96];
     uint64_t variant_bytes[96];
     uint64_t variant_entries[96];
+    /* per degree bucket b (0-7, 8 = hub rows) and sweep kind f (0 hash/sort,
+     * 1 fast), index 2*b + f: launches (one per sweep), event time, bytes and
+     * entries.  With profile = 3 the pass runs without side streams, so each
+     * bucket is timed with the device to itself. */
+    uint64_t bucket_launches[18];
+    double bucket_ms[18];
+    uint64_t bucket_bytes[18];
+    uint64_t bucket_entries[18];
 } lv_kernel_stats;
 
 typedef struct lv_graph lv_graph;          /* GraphHost (graphHost.cuh:7-18): host CSR */
@@ -134,7 +151,8 @@ lv_status lv_graph_load_edge(const char *path, int weighted, int skip_line, lv_g
 /* Same semantics from in-memory pairs (w may be NULL for unit weights). */
 lv_status lv_graph_from_edges(uint64_t n_edges, const int64_t *a, const int64_t *b,
                               const uint32_t *w, lv_graph **out);
-/* Direct CSR (row[n+1], col[E], w[E] or NULL); total_weight <= 0 means sum(w)/2. */
+/* Direct CSR (row[n+1], col[E], w[E] or NULL); total_weight <= 0 means sum(w)/2,
+ * any other value must equal sum(w)/2 (else LV_ERR_ARG). */
 lv_status lv_graph_from_csr(uint32_t n, uint64_t E, const uint64_t *row, const uint32_t *col,
                             const uint32_t *w, double total_weight, lv_graph **out);
 void lv_graph_free(lv_graph *g);

@@ -521,7 +521,8 @@ void launch_sweep(Community &c, const lv_params &p, int kind, bool apply, uint32
     uint32_t *const *cur = st.lists(st.cur);
     if (p.profile) LV_HIP(hipEventRecord(st.ev[0], s));
     if (kind == LV_SWEEP_SORT) sort_sweep(a, cur, st.count, st.bucket_off, st.ws, s);
-    else move_pass(a, st.hub_tables(), cur, st.count, s, st.side_streams());
+    else move_pass(a, st.hub_tables(), cur, st.count, s,
+                   p.profile == 3 ? nullptr : st.side_streams());
     if (p.profile) LV_HIP(hipEventRecord(st.ev[1], s));
     if (apply) {
         commit_pass(a, cur, st.count, s, st.side_streams());
@@ -571,18 +572,20 @@ int OptimizationPhase::sweep_loop(Community &c, const lv_params &p, int64_t exec
         const uint64_t sq = sum_slots(h, ST_SUMSQ);
         uint32_t next_counts[kNumBuckets];
         std::memcpy(next_counts, h + kStatWordsOffset, sizeof(next_counts));
+        float ms_move = 0, ms_sweep = 0;
+        uint64_t sweep_cand = 0, sweep_bytes = 0;
         if (p.profile) {
-            float ms_move = 0, ms_sweep = 0;
             LV_HIP(hipEventElapsedTime(&ms_move, st.ev[0], st.ev[1]));
             LV_HIP(hipEventElapsedTime(&ms_sweep, st.ev[0], st.ev[2]));
             const bool fast = kind == LV_SWEEP_FAST;
             const bool wtd = !c.graph.unit_weights;
-            // algorithmic bytes of a best-move launch: per entry col (+comm[x]
-            // unless fast, +w if weighted), per candidate tot[c], per listed
-            // vertex list+row+k+tot_own (+comm[v] unless fast), per mover target
-            auto bytes_of = [&](uint64_t ent, uint64_t cnd, uint64_t verts, uint64_t tw) {
-                const uint64_t per_entry = (fast ? 4u : 8u) + (wtd ? 4u : 0u);
-                return per_entry * ent + 4u * cnd + (fast ? 28u : 32u) * verts + 4u * tw;
+            // algorithmic bytes of a best-move launch, by the per-unit figures
+            // of the roofline basis: per entry col + comm[x] (+ w if weighted),
+            // per candidate an 8-byte total, per listed vertex row bounds,
+            // comm, degree, own total and the decision (28) plus its active
+            // flag (1).  The sweep adds the flags of the vertices no list holds.
+            auto bytes_of = [&](uint64_t ent, uint64_t cnd, uint64_t verts, uint64_t) {
+                return (wtd ? 12u : 8u) * ent + 8u * cnd + 29u * verts;
             };
             uint64_t bytes = 0;
             for (int b = 0; b < kNumBuckets; ++b) {
@@ -593,19 +596,27 @@ int OptimizationPhase::sweep_loop(Community &c, const lv_params &p, int64_t exec
                 const uint64_t bt = bucket_slots(h, b, ST_TWRITES);
                 const uint64_t bb = bytes_of(be, bc, verts, bt);
                 bytes += bb;
-                if (b == kHubBucket || kind == LV_SWEEP_SORT) continue;
+                sweep_cand += bc;
+                if (kind == LV_SWEEP_SORT) continue;
                 if (p.profile == 2 && !(kProfileLightMask >> b & 1u)) continue;
                 float ms_b = 0;
                 LV_HIP(hipEventElapsedTime(&ms_b, st.bev[2 * b], st.bev[2 * b + 1]));
+                const int fk = fast ? 1 : 0;
+                st.acc.bk_launches[b][fk] += 1;
+                st.acc.bk_ms[b][fk] += ms_b;
+                st.acc.bk_bytes[b][fk] += bb;
+                st.acc.bk_entries[b][fk] += be;
+                if (b == kHubBucket) continue;
                 const int vi = move_variant(b, fast, wtd, st.tiles.sel[b][fast ? 1 : 0]);
                 st.acc.var_launches[vi] += 1;
                 st.acc.var_ms[vi] += ms_b;
                 st.acc.var_bytes[vi] += bb;
                 st.acc.var_entries[vi] += be;
             }
+            sweep_bytes = bytes + (n - n_active);
             st.acc.move_launches += 1;
             st.acc.move_ms += ms_move;
-            st.acc.move_bytes += bytes;
+            st.acc.move_bytes += sweep_bytes;
             st.acc.move_entries += e_active;
             st.acc.sweep_ms += ms_sweep;
             st.acc.sweeps += 1;
@@ -630,6 +641,10 @@ int OptimizationPhase::sweep_loop(Community &c, const lv_params &p, int64_t exec
             r.sigma_in = c.sigma_in;
             r.sum_tot_sq = c.sum_tot_sq;
             r.q = c.modularity;
+            r.move_ms = ms_move;
+            r.sweep_ms = ms_sweep;
+            r.candidates = sweep_cand;
+            r.move_bytes = sweep_bytes;
             if (hash_trace) {
                 comm_host.resize(n);
                 LV_HIP(hipMemcpy(comm_host.data(), c.communities.data(), n * sizeof(uint32_t),

@@ -32,6 +32,12 @@ struct KernelStatsAcc {
     double var_ms[kMoveVariants] = {};
     uint64_t var_bytes[kMoveVariants] = {};
     uint64_t var_entries[kMoveVariants] = {};
+    // per degree bucket (8 = hubs), [fast sweep?]: launches, event time,
+    // algorithmic bytes, entries
+    uint64_t bk_launches[kNumBuckets][2] = {};
+    double bk_ms[kNumBuckets][2] = {};
+    uint64_t bk_bytes[kNumBuckets][2] = {};
+    uint64_t bk_entries[kNumBuckets][2] = {};
 };
 
 extern KernelStatsAcc g_last_run_stats;

@@ -174,6 +174,12 @@ GraphHost GraphHost::from_csr(uint32_t n, uint64_t E, const uint64_t *row, const
             if (!g.unit_weights) g.weights[j] = tmp[j - b].second;
         }
     }
+    // The exact gains use 2m = 2 * total_weight and rely on 2m being the
+    // degree sum (whose bound 2m < 2^31 the community checks), so a caller's
+    // total_weight must be half the weight sum, as it is for every graph the
+    // reference builds (graph.cu:81, phaseAggregation.cuh:172).
+    if (total_weight > 0 && 2.0 * total_weight != wsum)
+        throw lv::Error(LV_ERR_ARG, "total_weight is not half the sum of the weights");
     g.total_weight = total_weight > 0 ? total_weight : wsum / 2;
     g.n_links = E / 2;
     g.min_id = 0;

@@ -467,6 +467,13 @@ static void fill_stats(lv_kernel_stats *s, const lv::KernelStatsAcc &a) {
         s->variant_bytes[i] = a.var_bytes[i];
         s->variant_entries[i] = a.var_entries[i];
     }
+    for (int b = 0; b < lv::kNumBuckets; ++b)
+        for (int f = 0; f < 2; ++f) {
+            s->bucket_launches[2 * b + f] = a.bk_launches[b][f];
+            s->bucket_ms[2 * b + f] = a.bk_ms[b][f];
+            s->bucket_bytes[2 * b + f] = a.bk_bytes[b][f];
+            s->bucket_entries[2 * b + f] = a.bk_entries[b][f];
+        }
 }
 
 lv_status lv_bandwidth_probe(uint64_t bytes, int reps, double *gbs) {

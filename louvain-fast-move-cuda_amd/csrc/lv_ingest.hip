@@ -68,12 +68,13 @@ __device__ inline bool read_value(const uint8_t *b, uint64_t n, uint64_t i, int3
 }
 
 // Per thread range of bytes: read starts (count) and the first failing read.
+// Counts are u64: a file of more than 8 GiB can hold more than 2^32 reads.
 __global__ __launch_bounds__(kBlock) void k_scan_reads(const uint8_t *b, uint64_t n,
-                                                       uint32_t *counts,
+                                                       uint64_t *counts,
                                                        unsigned long long *fail) {
     const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     const uint64_t i0 = t * kBytesPerThread;
-    uint32_t c = 0;
+    uint64_t c = 0;
     if (i0 < n) {
         const uint64_t i1 = i0 + kBytesPerThread < n ? i0 + kBytesPerThread : n;
         for (uint64_t i = i0; i < i1; ++i) {
@@ -92,13 +93,13 @@ __global__ __launch_bounds__(kBlock) void k_scan_reads(const uint8_t *b, uint64_
 
 // Values of the reads before the first failure, in file order.
 __global__ __launch_bounds__(kBlock) void k_write_reads(const uint8_t *b, uint64_t n,
-                                                        const uint32_t *offs, uint64_t fail,
+                                                        const uint64_t *offs, uint64_t fail,
                                                         int32_t *vals) {
     const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     const uint64_t i0 = t * kBytesPerThread;
     if (i0 >= n || i0 >= fail) return;
     const uint64_t i1 = i0 + kBytesPerThread < n ? i0 + kBytesPerThread : n;
-    uint32_t k = offs[t];
+    uint64_t k = offs[t];
     for (uint64_t i = i0; i < i1 && i < fail; ++i) {
         if (!read_start(b, i)) continue;
         int32_t v = 0;
@@ -292,8 +293,8 @@ GraphDevice ingest_edge_file(const std::string &path, bool weighted, int skip_li
     const auto t1 = tnow();
     // reads
     const uint64_t nt = (n_bytes + kBytesPerThread - 1) / kBytesPerThread;
-    DeviceArray<uint32_t> counts(nt + 1);
-    DeviceArray<uint32_t> scan_tmp(scan_scratch_elems(nt + 1));
+    DeviceArray<uint64_t> counts(nt + 1);
+    DeviceArray<uint64_t> scan_tmp(scan_scratch_elems(nt + 1));
     DeviceArray<unsigned long long> fail(1);
     fill_u64(fail.data(), ~0ull, 1, s);
     if (nt) {
@@ -301,8 +302,8 @@ GraphDevice ingest_edge_file(const std::string &path, bool weighted, int skip_li
         hipLaunchKernelGGL(k_scan_reads, dim3(g), dim3(kBlock), 0, s, bytes.data(), n_bytes,
                            counts.data(), fail.data());
     }
-    fill_u32(counts.data() + nt, 0u, 1, s);
-    scan_u32(counts.data(), counts.data(), nt + 1, false, scan_tmp.data(), s);
+    set_u64(counts.data() + nt, 0ull, s);
+    scan_u64(counts.data(), counts.data(), nt + 1, false, scan_tmp.data(), s);
     const unsigned long long fail_pos = read_back(fail.data(), s);
     // reads before the first failure
     uint64_t n_reads = 0;

@@ -44,7 +44,9 @@ class SweepRec(C.Structure):
                 ("_pad", C.c_int32), ("n", C.c_uint32), ("n_active", C.c_uint32),
                 ("n_moved", C.c_uint32), ("n_considered", C.c_uint32),
                 ("E", C.c_uint64), ("e_active", C.c_uint64), ("sigma_in", C.c_int64),
-                ("sum_tot_sq", C.c_uint64), ("q", C.c_double), ("comm_hash", C.c_uint64)]
+                ("sum_tot_sq", C.c_uint64), ("q", C.c_double), ("comm_hash", C.c_uint64),
+                ("move_ms", C.c_double), ("sweep_ms", C.c_double), ("candidates", C.c_uint64),
+                ("move_bytes", C.c_uint64)]
 
 
 class LevelRec(C.Structure):
@@ -67,7 +69,9 @@ class KernelStats(C.Structure):
                 ("move_bytes", C.c_uint64), ("move_entries", C.c_uint64),
                 ("sweep_ms", C.c_double), ("sweeps", C.c_uint64),
                 ("variant_launches", C.c_uint64 * 96), ("variant_ms", C.c_double * 96),
-                ("variant_bytes", C.c_uint64 * 96), ("variant_entries", C.c_uint64 * 96)]
+                ("variant_bytes", C.c_uint64 * 96), ("variant_entries", C.c_uint64 * 96),
+                ("bucket_launches", C.c_uint64 * 18), ("bucket_ms", C.c_double * 18),
+                ("bucket_bytes", C.c_uint64 * 18), ("bucket_entries", C.c_uint64 * 18)]
 
 
 class BatchResult(C.Structure):

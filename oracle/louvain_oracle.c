@@ -339,6 +339,11 @@ static float dq_ref_f32(uint64_t w, uint64_t self, uint64_t k, int64_t tot_own,
     return (float)(t1 + t2);
 }
 
+float lvo_dq_ref_f32(uint64_t w, uint64_t self, uint64_t k, int64_t tot_own, int64_t tot_c,
+                     double tw) {
+    return dq_ref_f32(w, self, k, tot_own, tot_c, tw);
+}
+
 /* Vertex-aligned STEP_ROUND chunk ends (phaseOptimizationSort.cu:46-60,
  * :186-204; phaseOptimizationHash.cu:63-74):
  *   limit = round + STEP; limit = limit >= E ? E : row[src(limit) - 1].

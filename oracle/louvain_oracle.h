@@ -128,6 +128,13 @@ int64_t lvo_best_moves(const lvo_graph *g, const uint32_t *comm, const int64_t *
                        const uint8_t *active, int kind, const lvo_params *p,
                        uint32_t *target, uint64_t *e_active, uint32_t *n_considered);
 
+/* The reference-rounding delta-Q of one candidate (DeltaModularityHash /
+ * DeltaModularitySort, operatorsDelta.h:19-20, :38-39), as lvo_best_moves
+ * evaluates it in LVO_GAIN_REF_F32 mode: w = w(v,c), self = w(v,own),
+ * k = k_v, tw = total_weight.  Exported for the tests that pin it. */
+float lvo_dq_ref_f32(uint64_t w, uint64_t self, uint64_t k, int64_t tot_own, int64_t tot_c,
+                     double tw);
+
 /* AggregationPhase::run (phaseAggregation.cuh:19-175): renumber non-empty
  * communities in id order (map = inclusive scan of tot>0), contract.
  * new_id[c] = map[c]-1 for non-empty c, 0xFFFFFFFF otherwise. */

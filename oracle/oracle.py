@@ -81,6 +81,9 @@ def lib():
         L.lvo_aggregate.argtypes = [vp, vp, vp, vp, C.POINTER(vp)]
         L.lvo_louvain.argtypes = [vp, C.POINTER(Params), C.POINTER(Result)]
         L.lvo_fnv1a_u32.argtypes = [vp, C.c_uint64]; L.lvo_fnv1a_u32.restype = C.c_uint64
+        L.lvo_dq_ref_f32.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int64, C.c_int64,
+                                     C.c_double]
+        L.lvo_dq_ref_f32.restype = C.c_float
         _lib = L
     return _lib
 

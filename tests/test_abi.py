@@ -233,3 +233,41 @@ def test_traffic_profile_names_the_roofline_kernel():
                (t["fetch_kib_per_dispatch"] * t["fetch_scale"] +
                 t["write_kib_per_dispatch"] * t["write_scale"]) * 1024) < 64
 
+
+
+@pytest.mark.parametrize("name", ["basic", "crlf_tabs", "skip", "junk_stops", "odd_tokens",
+                                  "sign_after_digit", "signs", "bad_sign", "overflow_first",
+                                  "overflow_later", "int_limits", "dups_loops", "weighted",
+                                  "weighted_units", "weighted_partial", "long_line",
+                                  "no_newline_end", "not_dense", "empty", "only_junk"])
+def test_host_reader_edge_cases_match_oracle(tmp_path, name):
+    """The library's host .edge reader and the oracle's agree on every edge
+    case the GPU ingest test uses (same CSR, or the same error status)."""
+    from test_gpu_ingest import CASES, oracle_load
+    text, weighted, skip = CASES[name]
+    p = tmp_path / f"{name}.edge"
+    p.write_bytes(text.encode())
+    oi, oa = oracle_load(p, weighted, skip)
+    try:
+        g = L.Graph.load_edge(p, weighted, skip)
+        err = None
+    except L.LouvainError as e:
+        g, err = None, e.status
+    if oi is None:
+        assert err == oa, (err, oa)
+        return
+    assert err is None and g.info() == oi, (err, oi)
+    for x, y in zip(oa, g.csr()[:3]):
+        assert np.array_equal(x, y)
+
+
+def test_from_csr_total_weight_must_be_half_the_weight_sum():
+    row = np.array([0, 1, 2], np.uint64)
+    col = np.array([1, 0], np.uint32)
+    w = np.array([3, 3], np.uint32)
+    assert L.Graph.from_csr(row, col, w).info()["total_weight"] == 3.0
+    assert L.Graph.from_csr(row, col, w, total_weight=3.0).info()["total_weight"] == 3.0
+    for bad in (2.5, 4.0, 1e12):
+        with pytest.raises(L.LouvainError) as e:
+            L.Graph.from_csr(row, col, w, total_weight=bad)
+        assert e.value.status == 1                    # LV_ERR_ARG

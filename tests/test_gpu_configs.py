@@ -73,3 +73,26 @@ def test_planted_1m(planted, mode):
         f"communities={a['n_comm']}")
     assert abs(a["modularity"] - q64) < 1e-6
     assert a["modularity"] > 0.9 * q_planted
+
+
+def test_planted_1m_reference_method(planted):
+    """The reference's own method on the planted configuration (f32-rounded
+    delta-Q, operatorsDelta.h:19-20, and highest-id ties in hash sweeps,
+    phaseOptimizationHash.cu:23-35): the GPU against the oracle trace for
+    trace, and the default exact-gain run's final Q within 0.5% of it."""
+    ea, eb, ew, lab = planted
+    g = L.Graph.from_edges(ea, eb, ew)
+    row, col, w = g.csr()[:3]
+    say("planted 1e6 reference method: GPU run")
+    a = g.louvain(L.params(mode=L.HASH, gain=L.GAIN_REF_F32, tie=L.TIE_REFERENCE),
+                  hash_trace=True)
+    say("planted 1e6 reference method: oracle run")
+    b = O.Graph.from_edges(ea, eb, ew).louvain(
+        O.params(mode=O.HASH, gain=O.GAIN_REF_F32, tie=O.TIE_REFERENCE))
+    compare_runs(a, b, "planted-1e6 reference method")
+    q_ref = a["modularity"]
+    assert abs(q_ref - modularity_fp64(row, col, w, a["best"])) < 1e-6
+    q_exact = g.louvain(L.params(mode=L.HASH))["modularity"]
+    say(f"planted 1e6: Q exact {q_exact:.9f}, Q reference method {q_ref:.9f}, "
+        f"relative difference {(q_exact - q_ref) / abs(q_ref):+.3%}")
+    assert abs(q_exact - q_ref) <= 0.005 * abs(q_ref)

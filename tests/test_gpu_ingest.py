@@ -1,14 +1,18 @@
-"""GPU .edge ingest against the host reader (graph.cu:17-87 semantics).
+"""GPU .edge ingest against the oracle's reader (graph.cu:17-87 semantics).
 
 Bar: the same CSR (rows, destinations, weights), vertex count, entries,
-total weight and id shift, and the same error on inputs the reader rejects.
+total weight and id shift, and the same error on inputs the reader rejects,
+as the oracle's .edge reader (oracle/louvain_oracle.c, lvo_graph_load_edge)
+and the library's host reader.
 """
+import ctypes as C
 import sys
 
 import numpy as np
 import pytest
 
 import louvain_amd as L
+from oracle import oracle as O
 from test_gpu_parity import REC_KEYS, compare_runs
 
 pytestmark = pytest.mark.gpu
@@ -34,15 +38,32 @@ def _load(loader, path, weighted, skip):
         return None, e.status
 
 
+def oracle_load(path, weighted, skip):
+    """The oracle's reader: (graph info, (row, col, w)) or (None, status)."""
+    h = C.c_void_p()
+    rc = O.lib().lvo_graph_load_edge(str(path).encode(), int(weighted), int(skip), C.byref(h))
+    if rc != 0:
+        return None, rc
+    g = O.Graph(h)
+    info = {"n": g.n, "E": g.E, "total_weight": g.total_weight, "min_id": g.min_id}
+    return info, g.arrays()[:3]
+
+
 def _same(path, weighted=False, skip=0):
-    gh, eh = _load(L.Graph.load_edge, path, weighted, skip)
+    """GPU ingest == oracle reader (and == the host reader); returns the
+    error status they agree on, or None."""
+    oi, oa = oracle_load(path, weighted, skip)
     gd, ed = _load(L.Graph.load_edge_gpu, path, weighted, skip)
-    assert eh == ed, f"{path}: host error {eh}, GPU error {ed}"
-    if gh is None:
-        return eh
-    assert gh.info() == gd.info(), (gh.info(), gd.info())
-    for x, y in zip(gh.csr()[:3], gd.csr()[:3]):
-        assert np.array_equal(x, y)
+    gh, eh = _load(L.Graph.load_edge, path, weighted, skip)
+    eo = None if oi is not None else oa
+    assert ed == eo, f"{path}: oracle error {eo}, GPU error {ed}"
+    assert eh == eo, f"{path}: oracle error {eo}, host error {eh}"
+    if oi is None:
+        return eo
+    assert gd.info() == oi, (gd.info(), oi)
+    assert gh.info() == oi, (gh.info(), oi)
+    for x, y, z in zip(oa, gd.csr()[:3], gh.csr()[:3]):
+        assert np.array_equal(x, y) and np.array_equal(x, z)
     return None
 
 
@@ -71,7 +92,7 @@ CASES = {
 
 
 @pytest.mark.parametrize("name", list(CASES))
-def test_edge_cases_match_host_reader(tmp_path, name):
+def test_edge_cases_match_oracle_reader(tmp_path, name):
     text, weighted, skip = CASES[name]
     p = tmp_path / f"{name}.edge"
     p.write_bytes(text.encode())

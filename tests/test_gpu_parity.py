@@ -410,22 +410,29 @@ def test_rmat_scale18_parity():
         compare_runs(a, b, label)
 
 
-def test_rmat22_full_size_golden_trace():
+RMAT22_TRACES = [("rmat22_hash_trace.json", L.GAIN_EXACT, L.TIE_LOWEST),
+                 ("rmat22_hash_ref_trace.json", L.GAIN_REF_F32, L.TIE_REFERENCE)]
+
+
+@pytest.mark.parametrize("trace,gain,tie", RMAT22_TRACES, ids=["exact", "reference_method"])
+def test_rmat22_full_size_golden_trace(trace, gain, tie):
     """The benchmark workload at full size (R-MAT scale 22, 2.4 M vertices,
     128 M entries) against the oracle's trace of the same run, computed once
     by tests/golden/make_rmat22_trace.py: every sweep record (active, moved,
     entries, exact sigma_in and sum of tot^2, Q, FNV hash of the partition),
-    the final modularity, communities and the final partition's SHA-256."""
+    the final modularity, communities and the final partition's SHA-256.  Two
+    traces: exact gains with lowest-id ties, and the reference's own method
+    (f32-rounded delta-Q, operatorsDelta.h:19-20, with the hash sweeps'
+    highest-id ties, phaseOptimizationHash.cu:23-35)."""
     import hashlib
     import json
     import os
-    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
-                                       "rmat22_hash_trace.json")))
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", trace)))
     ea, eb, n = L.gen_rmat(22, 16, 0.57, 0.19, 0.19, seed=2)
     g = L.Graph.from_edges(ea, eb)
     assert g.n == gold["n"]
-    say("rmat22 full size")
-    r = g.louvain(L.params(mode=L.HASH), hash_trace=True)
+    say(f"rmat22 full size: {trace}")
+    r = g.louvain(L.params(mode=L.HASH, gain=gain, tie=tie), hash_trace=True)
     got = [{k: s[k] for k in REC_KEYS} for s in r["sweeps"]]
     assert len(got) == gold["total_sweeps"]
     for i, (x, y) in enumerate(zip(got, gold["sweeps"])):
