@@ -226,6 +226,22 @@ lv_status lv_best_moves(lv_community *c, int kind, const lv_params *p, const uin
 /* Exact modularity parts of the current partition, from scratch on the device. */
 lv_status lv_community_modularity_parts(lv_community *c, int64_t *sigma_in, uint64_t *sum_tot_sq,
                                         double *q);
+/* No counterpart in the reference: lv_community_modularity_parts for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The CSR (row[n+1],
+ * col[E], w[E] or NULL for unit weights), the degrees kdeg[n] and the
+ * partition comm[n] are device pointers of the caller; every community id
+ * must be below n (else LV_ERR_ARG); total_weight is the graph's m.  The
+ * call allocates nothing: its scratch (community totals and counters) is a
+ * fixed device array of the library, so n is at most 16,777,216
+ * (LV_ERR_RANGE above).  stream may be NULL; the call returns after the
+ * stream has drained.  Calls that overlap on one device share that array
+ * and race. */
+lv_status lv_modularity_parts_noalloc(uint32_t n, uint64_t E, const uint64_t *row,
+                                      const uint32_t *col, const uint32_t *w,
+                                      const uint32_t *kdeg, const uint32_t *comm,
+                                      double total_weight, void *stream, int64_t *sigma_in,
+                                      uint64_t *sum_tot_sq, double *q);
 /* Tile choice of the best-move kernels: sel[2*b + f] = candidate index of
  * degree bucket b (0-7) for hash/sort (f = 0) and fast (f = 1) sweeps.  get
  * returns the tuned (or default) choice; set pins one, e.g. a table tuned

@@ -137,6 +137,9 @@ SIGNATURES = {
                                 C.POINTER(C.c_uint64)]),
     "lv_community_modularity_parts": (C.c_int, [_vp, C.POINTER(C.c_int64),
                                                 C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
+    "lv_modularity_parts_noalloc": (C.c_int, [C.c_uint32, C.c_uint64, _vp, _vp, _vp, _vp, _vp,
+                                              C.c_double, _vp, C.POINTER(C.c_int64),
+                                              C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "lv_community_get_tiles": (C.c_int, [_vp, _vp]),
     "lv_community_set_tiles": (C.c_int, [_vp, _vp]),
     "lv_kernel_stats_get": (C.c_int, [_vp, C.POINTER(KernelStats)]),
