@@ -77,6 +77,54 @@ def cpu_baseline(row, col, mode, threshold):
                       f"CPU {cpu_model()}, nproc {os.cpu_count()}"}
 
 
+def extra_runs(args, L, c, info, q_ex):
+    """Runs outside the timed steps: the reference's own method on the same
+    resident graph, and one run with the best-move pass on a single stream so
+    that every degree bucket is timed alone."""
+    # the reference's own method on the same resident graph
+    prm_ref = L.params(mode=args.mode, sweep_threshold=args.threshold,
+                       level_threshold=args.threshold, gain=L.GAIN_REF_F32, tie=L.TIE_REFERENCE,
+                       autotune=0)
+    c.reset()
+    c.run(prm_ref, best=False, n0=info["n"])
+    c.reset()
+    t0 = time.perf_counter()
+    rr = c.run(prm_ref, best=False, n0=info["n"])
+    ref_s = time.perf_counter() - t0
+    q_rf = rr["modularity"]
+    reference_method = {
+        "modularity": q_rf, "n_communities": rr["n_comm"], "levels": rr["levels"],
+        "sweeps": rr["total_sweeps"], "s_to_converge": ref_s,
+        "relative_q_difference": (q_ex - q_rf) / abs(q_rf),
+        "within_half_percent_below": q_ex >= (1 - 0.005) * q_rf,
+        "within_half_percent_two_sided": abs(q_ex - q_rf) <= 0.005 * abs(q_rf),
+        "note": "gain=ref_f32 (operatorsDelta.h:19-20) with highest-id ties in hash sweeps "
+                "(phaseOptimizationHash.cu:23-35); relative_q_difference = (Q_default - Q_ref) "
+                "/ Q_ref"}
+    log(f"reference method: Q={q_rf:.9f} (default {q_ex:.9f}), {ref_s * 1e3:.1f} ms")
+    # each bucket alone: one run with the pass on a single stream, every bucket timed
+    prm3 = L.params(mode=args.mode, sweep_threshold=args.threshold,
+                    level_threshold=args.threshold, profile=3, autotune=0)
+    c.reset()
+    c.run(prm3, best=False, n0=info["n"])
+    kb = c.kernel_stats()
+    buckets = []
+    for b in range(9):
+        for f in (1, 0):
+            i = 2 * b + f
+            if not kb["bucket_launches"][i]:
+                continue
+            ms = kb["bucket_ms"][i]
+            gbs = kb["bucket_bytes"][i] / ms / 1e6 if ms > 0 else 0.0
+            buckets.append({"bucket": BUCKET_NAMES[b], "sweep": "fast" if f else args.mode,
+                            "launches": kb["bucket_launches"][i], "ms": ms,
+                            "entries": kb["bucket_entries"][i], "bytes": kb["bucket_bytes"][i],
+                            "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS})
+    log("buckets: " + ", ".join(f"{x['bucket']}/{x['sweep']} {x['ms']:.2f} ms {x['achieved_gbs']:.0f} GB/s"
+                                 for x in buckets))
+    return reference_method, buckets
+
+
 def c1_result(L):
     """The CPU-path configuration: the bundled fixture (networkx
     powerlaw_cluster_graph(10000, 2, 0.99, seed=1)) and a literal
@@ -470,6 +518,9 @@ def main():
     ap.add_argument("--served-window", type=float, default=60.0,
                     help="rmat: arrival window of the served sub-object (s)")
     ap.add_argument("--graphs", type=int, default=4096, help="batched: number of graphs")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="rmat: only the timed runs (no reference-method or per-bucket runs), "
+                         "for kernel traces")
     ap.add_argument("--loop-sample", type=int, default=1,
                     help="batched: time the per-graph loop on every k-th graph only")
     args = ap.parse_args()
@@ -536,47 +587,10 @@ def main():
                    for s in last["sweeps"]],
         "note": "e_active / HIP-event time of the sweep's best-move pass; the weighted mean is "
                 "total entries over total pass time of the timed runs"}
-    # the reference's own method on the same resident graph
-    prm_ref = L.params(mode=args.mode, sweep_threshold=args.threshold,
-                       level_threshold=args.threshold, gain=L.GAIN_REF_F32, tie=L.TIE_REFERENCE,
-                       autotune=0)
-    c.reset()
-    c.run(prm_ref, best=False, n0=info["n"])
-    c.reset()
-    t0 = time.perf_counter()
-    rr = c.run(prm_ref, best=False, n0=info["n"])
-    ref_s = time.perf_counter() - t0
-    q_ex, q_rf = last["modularity"], rr["modularity"]
-    reference_method = {
-        "modularity": q_rf, "n_communities": rr["n_comm"], "levels": rr["levels"],
-        "sweeps": rr["total_sweeps"], "s_to_converge": ref_s,
-        "relative_q_difference": (q_ex - q_rf) / abs(q_rf),
-        "within_half_percent_below": q_ex >= (1 - 0.005) * q_rf,
-        "within_half_percent_two_sided": abs(q_ex - q_rf) <= 0.005 * abs(q_rf),
-        "note": "gain=ref_f32 (operatorsDelta.h:19-20) with highest-id ties in hash sweeps "
-                "(phaseOptimizationHash.cu:23-35); relative_q_difference = (Q_default - Q_ref) "
-                "/ Q_ref"}
-    log(f"reference method: Q={q_rf:.9f} (default {q_ex:.9f}), {ref_s * 1e3:.1f} ms")
-    # each bucket alone: one run with the pass on a single stream, every bucket timed
-    prm3 = L.params(mode=args.mode, sweep_threshold=args.threshold,
-                    level_threshold=args.threshold, profile=3, autotune=0)
-    c.reset()
-    c.run(prm3, best=False, n0=info["n"])
-    kb = c.kernel_stats()
-    buckets = []
-    for b in range(9):
-        for f in (1, 0):
-            i = 2 * b + f
-            if not kb["bucket_launches"][i]:
-                continue
-            ms = kb["bucket_ms"][i]
-            gbs = kb["bucket_bytes"][i] / ms / 1e6 if ms > 0 else 0.0
-            buckets.append({"bucket": BUCKET_NAMES[b], "sweep": "fast" if f else args.mode,
-                            "launches": kb["bucket_launches"][i], "ms": ms,
-                            "entries": kb["bucket_entries"][i], "bytes": kb["bucket_bytes"][i],
-                            "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS})
-    log("buckets: " + ", ".join(f"{x['bucket']}/{x['sweep']} {x['ms']:.2f} ms {x['achieved_gbs']:.0f} GB/s"
-                                 for x in buckets))
+    if args.no_extras:
+        reference_method, buckets = None, None
+    else:
+        reference_method, buckets = extra_runs(args, L, c, info, last["modularity"])
     move_s = ks["move_ms"] / 1e3
     pass_gbs = ks["move_bytes"] / move_s / 1e9 if move_s > 0 else 0.0
     # dominant best-move kernel: the degree 257-1024 bucket's variant (one
