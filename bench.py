@@ -627,7 +627,7 @@ def main():
         "config": {"workload": f"plain Louvain, R-MAT scale {args.scale}, edge factor 16, "
                                f"(0.57,0.19,0.19,0.05), seed 2, symmetrised, unit weights",
                    "mode": args.mode, "threshold": args.threshold, "n": info["n"],
-                   "E": info["E"]},
+                   "E": info["E"], "bucket_max_deg": c.buckets()},
         "s_to_converge": total / args.steps,
         "modularity": last["modularity"],
         "n_communities": last["n_comm"],

@@ -24,6 +24,12 @@ struct GraphDevice {
     lv::DeviceArray<uint32_t> edge_destination;      // [E]
     lv::DeviceArray<uint32_t> weights;               // [E], empty when unit
     lv::DeviceArray<uint32_t> tot_weight_per_nodes;  // [n]
+    // The reference's COO members (graph.cuh:25-32): the source of every
+    // entry (entries in CSR order, so edge_destination is the same array) and
+    // every vertex's entry count.  The optimisation loop does not use them;
+    // they stay empty until build_coo() fills them.
+    lv::DeviceArray<uint32_t> edge_source;           // [E]
+    lv::DeviceArray<uint32_t> n_of_neighboor;        // [n]
 
     GraphDevice() = default;
     GraphDevice(const GraphHost &g, hipStream_t s);  // graph.cuh:46-99
@@ -31,6 +37,8 @@ struct GraphDevice {
     GraphDevice &operator=(GraphDevice &&) = default;
     // Non-owning view of another level graph (the level-0 CSR kept for re-runs).
     static GraphDevice view_of(const GraphDevice &g);
+    // Fills edge_source and n_of_neighboor from the CSR (kernels on stream s).
+    void build_coo(hipStream_t s);
 };
 
 #endif

@@ -32,6 +32,14 @@ struct GraphHost {
     // graph and is checked by the batch, not over the union
     bool disjoint_union = false;
 
+    // The reference's host COO (graphHost.cuh:12-14): every unique undirected
+    // pair (a, b), a <= b, in lexicographic order as (a, b) then (b, a), ids
+    // as read (min_id not subtracted), and the weights in the same order.
+    // Empty until build_coo() fills them; the library itself uses the CSR.
+    std::vector<int64_t> edge_source;
+    std::vector<int64_t> edge_destination;
+    std::vector<uint32_t> edge_weights;
+
     GraphHost() = default;
     // graph.cu:17 -- throws lv::Error on failure
     GraphHost(std::string name, bool weighted, int skip_line);
@@ -41,6 +49,7 @@ struct GraphHost {
     static GraphHost from_csr(uint32_t n, uint64_t E, const uint64_t *row, const uint32_t *col,
                               const uint32_t *w, double total_weight);
     uint64_t n_entries() const { return col.size(); }
+    void build_coo();
 };
 
 #endif

@@ -71,6 +71,14 @@ typedef struct lv_params {
                                  the level-0 graph (RefinePhase; the reference's is empty,
                                  phaseRefine.cuh:1-3, so 0 by default).  Not for batches
                                  or the server. */
+    uint32_t bucket_max_deg[8]; /* degree bounds of the best-move kernels' LDS buckets 0-7
+                                   (rows above bucket_max_deg[7] take the hub path):
+                                   non-decreasing, each at most the default
+                                   {4, 8, 16, 32, 64, 256, 1024, 4096}, the last at least 2
+                                   (else LV_ERR_ARG).  All zero: the default, or with
+                                   autotune = 1 the fastest of three sets, timed once per
+                                   community on its level-0 fast sweep.  Results do not
+                                   depend on the bounds.  Not for the server (default). */
 } lv_params;
 
 typedef struct lv_sweep_rec {
@@ -249,6 +257,8 @@ lv_status lv_modularity_parts_noalloc(uint32_t n, uint64_t E, const uint64_t *ro
  * lanes per vertex {4,2,8}, {8,4,16}, {16,8,32}, {32,16,64}, {64,32},
  * {64,256}, {256}, {256}. */
 lv_status lv_community_get_tiles(const lv_community *c, uint8_t sel[16]);
+/* The best-move buckets' degree bounds in use (as lv_params.bucket_max_deg). */
+lv_status lv_community_get_buckets(const lv_community *c, uint32_t max_deg[8]);
 lv_status lv_community_set_tiles(lv_community *c, const uint8_t sel[16]);
 lv_status lv_kernel_stats_get(const lv_community *c, lv_kernel_stats *s);
 lv_status lv_last_run_kernel_stats(lv_kernel_stats *s);

@@ -162,6 +162,10 @@ void GraphBatch::run(const lv_params &p, lv_batch_result *r) {
     C.reset();
     B.gid = B.gid0.data();
     B.next_gid = 0;
+    // bucket bounds (and their tuning, with every graph sweeping)
+    std::fill(B.host_live, B.host_live + G, (uint8_t)1);
+    LV_HIP(hipMemcpyAsync(B.live_dev.data(), B.host_live, G, hipMemcpyHostToDevice, s));
+    apply_bucket_bounds(C, p);
     std::vector<int64_t> sig(B.sigma0);
     std::vector<uint64_t> sq(B.sumsq0);
     std::vector<double> q(G), q_start(G);

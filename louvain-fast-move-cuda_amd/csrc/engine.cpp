@@ -52,7 +52,7 @@ void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
     }
     fill_u8(mark.data(), 0, n, s);
     // bucket ids and per-bucket sizes (static: degrees do not change within a level)
-    bucket_ids(g.neighboorhood_sum.data(), n, bucket.data(), s);
+    bucket_ids(g.neighboorhood_sum.data(), n, bucket.data(), bounds, s);
     bucket_counts(bucket.data(), n, counts_dev.data(), s);
     uint32_t cnt[kNumBuckets];
     LV_HIP(hipMemcpyAsync(host_stats, counts_dev.data(), kNumBuckets * sizeof(uint32_t),
@@ -302,6 +302,13 @@ GraphDevice GraphDevice::view_of(const GraphDevice &g) {
     return v;
 }
 
+void GraphDevice::build_coo(hipStream_t s) {
+    edge_source.resize(std::max<uint64_t>(n_entries, 1));
+    n_of_neighboor.resize(std::max<uint32_t>(n_nodes, 1));
+    coo_sources(neighboorhood_sum.data(), n_nodes, edge_source.data(), n_of_neighboor.data(), s);
+    LV_HIP(hipGetLastError());
+}
+
 void Community::reset() {
     graph = GraphDevice::view_of(graph0);
     state->next_level_buf = 0;
@@ -474,6 +481,65 @@ static SweepArgs sweep_args(Community &c, const lv_params &p, int kind, uint32_t
     a.ct = st.ct.data();
     if (c.batch) a.bt = c.batch->view();
     return a;
+}
+
+// The tuner of the bucket bounds: at the identity partition of level 0, each
+// set of kBoundSet buckets the level and times two best-move passes of the
+// fast sweep (after one warm-up); the fastest set is kept for the community.
+// Best-move passes only write decisions, which the sweep itself rewrites; the
+// targets are set back to the partition afterwards.
+static BucketBounds tune_bucket_bounds(Community &C, const lv_params &p) {
+    SweepState &st = *C.state;
+    hipStream_t s = C.stream();
+    const uint32_t n = C.graph.n_nodes;
+    float best_ms = 0;
+    int best = 0;
+    for (int k = 0; k < kBoundSets; ++k) {
+        std::copy(kBoundSet[k], kBoundSet[k] + kNumBuckets - 1, st.bounds.max_deg);
+        st.setup_level(C.graph, s);
+        st.build_static_lists(s);
+        SweepArgs a = sweep_args(C, p, LV_SWEEP_FAST, n);
+        a.bev = nullptr;
+        uint32_t *const *L = st.lists(st.cur);
+        move_pass(a, st.hub_tables(), L, st.count, s, st.side_streams());
+        LV_HIP(hipEventRecord(st.ev[0], s));
+        move_pass(a, st.hub_tables(), L, st.count, s, st.side_streams());
+        move_pass(a, st.hub_tables(), L, st.count, s, st.side_streams());
+        LV_HIP(hipEventRecord(st.ev[1], s));
+        LV_HIP(hipEventSynchronize(st.ev[1]));
+        float ms = 0;
+        LV_HIP(hipEventElapsedTime(&ms, st.ev[0], st.ev[1]));
+        if (k == 0 || ms < best_ms) { best_ms = ms; best = k; }
+    }
+    if (n)
+        LV_HIP(hipMemcpyAsync(st.target.data(), C.communities.data(), n * sizeof(uint32_t),
+                              hipMemcpyDeviceToDevice, s));
+    BucketBounds out;
+    std::copy(kBoundSet[best], kBoundSet[best] + kNumBuckets - 1, out.max_deg);
+    return out;
+}
+
+void apply_bucket_bounds(Community &C, const lv_params &p) {
+    SweepState &st = *C.state;
+    bool given = false;
+    for (int b = 0; b < kNumBuckets - 1; ++b) given = given || p.bucket_max_deg[b] != 0;
+    BucketBounds want;
+    if (given) {
+        if (!bounds_valid(p.bucket_max_deg)) throw Error(LV_ERR_ARG, "invalid bucket_max_deg");
+        std::copy(p.bucket_max_deg, p.bucket_max_deg + kNumBuckets - 1, want.max_deg);
+    } else {
+        if (p.autotune && !st.bounds_tuned && C.level == 0) {
+            st.tuned_bounds = tune_bucket_bounds(C, p);
+            st.bounds_tuned = true;
+        }
+        if (st.bounds_tuned) want = st.tuned_bounds;
+    }
+    // the level's buckets were built with st.bounds (the tuner leaves the last
+    // set it tried): rebuild them only for other bounds
+    if (!std::equal(want.max_deg, want.max_deg + kNumBuckets - 1, st.bounds.max_deg)) {
+        st.bounds = want;
+        st.setup_level(C.graph, C.stream());
+    }
 }
 
 constexpr uint32_t kTuneMinVertices = 4096;   // smaller buckets keep the default tile
@@ -810,7 +876,7 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
     // 2. bucket communities by their number of virtual entries
     uint8_t *cb = ws.get<uint8_t>(W_CBUCKET, n_new);
     uint32_t *cnt_dev = ws.get<uint32_t>(W_CCOUNT, kNumBuckets);
-    bucket_ids(crow, n_new, cb, s);
+    bucket_ids(crow, n_new, cb, BucketBounds(), s);   // the contraction's buckets stay the default
     bucket_counts(cb, n_new, cnt_dev, s);
     uint32_t cnt[kNumBuckets];
     LV_HIP(hipMemcpyAsync(cnt, cnt_dev, sizeof(cnt), hipMemcpyDeviceToHost, s));
@@ -980,6 +1046,7 @@ Community ModularityAlgorithms::Louvain(const GraphHost &graph, const lv_params 
 
 void ModularityAlgorithms::run(Community &C, const lv_params &p, lv_result *r) {
     hipStream_t s = C.stream();
+    apply_bucket_bounds(C, p);
     wait_stream(s);
     C.state->acc = KernelStatsAcc();
     const auto t0 = std::chrono::steady_clock::now();

@@ -128,6 +128,11 @@ struct SweepState {
     // tiles of the best-move kernels (tuned once per community when asked)
     TileChoice tiles;
     bool tuned[2] = {false, false};   // [fast sweep]
+    // degree bounds of the best-move buckets (chosen by the caller or tuned
+    // once per community)
+    BucketBounds bounds;
+    BucketBounds tuned_bounds;
+    bool bounds_tuned = false;
     // side streams of the sweep passes and the contraction
     SideStreams side;
     const SideStreams *side_streams();
@@ -197,6 +202,10 @@ LevelView level_view(const GraphDevice &g);
 void sort_sweep(const SweepArgs &x, uint32_t *const *lists, const uint32_t *counts,
                 const uint64_t *bucket_off, Workspace &ws, hipStream_t s);
 uint64_t sum_slots(const unsigned long long *h, int stat);            // global section
+// Sets the best-move buckets' degree bounds of a run from lv_params (or the
+// tuner, on the community's first run with autotune) and rebuilds the level's
+// buckets if they change.
+void apply_bucket_bounds(Community &C, const lv_params &p);
 uint64_t bucket_slots(const unsigned long long *h, int bucket, int stat);
 uint64_t all_bucket_slots(const unsigned long long *h, int stat);
 void launch_sweep(::Community &c, const lv_params &p, int kind, bool apply, uint32_t v_end);

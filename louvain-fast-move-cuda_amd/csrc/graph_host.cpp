@@ -137,6 +137,34 @@ GraphHost GraphHost::from_edges(uint64_t m, const int64_t *a, const int64_t *b, 
     return g;
 }
 
+// The reference's COO order (graph.cu:59-76): the set of (min, max) pairs in
+// lexicographic order, each as (a, b) then (b, a).  Rows are sorted by
+// destination, so the entries (v, x) with v <= x of the CSR walked in order
+// are that set; a self-loop has two (v, v) entries and is emitted once.
+void GraphHost::build_coo() {
+    edge_source.clear();
+    edge_destination.clear();
+    edge_weights.clear();
+    edge_source.reserve(col.size());
+    edge_destination.reserve(col.size());
+    edge_weights.reserve(col.size());
+    for (uint32_t v = 0; v < n_nodes; ++v) {
+        bool second_self = false;
+        for (uint64_t j = row[v]; j < row[v + 1]; ++j) {
+            const uint32_t x = col[j];
+            if (x < v) continue;
+            if (x == v) {
+                second_self = !second_self;
+                if (second_self) continue;
+            }
+            const uint32_t w = unit_weights ? 1u : weights[j];
+            const int64_t a = (int64_t)v + min_id, b = (int64_t)x + min_id;
+            edge_source.push_back(a); edge_destination.push_back(b); edge_weights.push_back(w);
+            edge_source.push_back(b); edge_destination.push_back(a); edge_weights.push_back(w);
+        }
+    }
+}
+
 GraphHost GraphHost::from_csr(uint32_t n, uint64_t E, const uint64_t *row, const uint32_t *col,
                               const uint32_t *w, double total_weight) {
     if (row[0] != 0 || row[n] != E) throw lv::Error(LV_ERR_ARG, "CSR offsets inconsistent");

@@ -82,6 +82,7 @@ void lv_default_params(lv_params *p) {
     p->profile = 0;
     p->autotune = 0;
     p->refine = 0;
+    for (int b = 0; b < 8; ++b) p->bucket_max_deg[b] = 0;
 }
 
 const char *lv_status_string(lv_status s) {
@@ -526,6 +527,12 @@ lv_status lv_community_get_tiles(const lv_community *c, uint8_t sel[16]) {
     if (!c || !sel) return LV_ERR_ARG;
     for (int b = 0; b < 8; ++b)
         for (int f = 0; f < 2; ++f) sel[2 * b + f] = c->c.state->tiles.sel[b][f];
+    return LV_OK;
+}
+
+lv_status lv_community_get_buckets(const lv_community *c, uint32_t max_deg[8]) {
+    if (!c || !max_deg) return LV_ERR_ARG;
+    for (int b = 0; b < 8; ++b) max_deg[b] = c->c.state->bounds.max_deg[b];
     return LV_OK;
 }
 

@@ -294,10 +294,10 @@ inline unsigned grid_for(size_t n, size_t per_block, unsigned cap = 65536u) {
     return (unsigned)g;
 }
 
-__device__ inline uint8_t bucket_of_degree(uint64_t d) {
+__device__ inline uint8_t bucket_of_degree(uint64_t d, const BucketBounds &bounds) {
 #pragma unroll
     for (int b = 0; b < kNumBuckets - 1; ++b)
-        if (d <= kBucketMaxDeg[b]) return (uint8_t)b;
+        if (d <= bounds.max_deg[b]) return (uint8_t)b;
     return (uint8_t)kHubBucket;
 }
 

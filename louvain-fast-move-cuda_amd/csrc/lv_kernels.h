@@ -27,6 +27,22 @@ constexpr uint32_t kMaxChunks = 4096; // reference chunks per level (STEP_ROUND 
 // Maximum degree handled by each LDS bucket; larger goes to the hub bucket.
 constexpr uint32_t kBucketMaxDeg[kNumBuckets - 1] = {4, 8, 16, 32, 64, 256, 1024, 4096};
 
+// Degree bounds of the LDS buckets at run time: a row of d entries goes to the
+// first bucket b with d <= max_deg[b], to the hubs above max_deg[7].  Each
+// bound is at most kBucketMaxDeg[b] (so a bucket's tables and tiles hold its
+// rows), the bounds do not decrease, and max_deg[7] >= 2 (a row in the hub
+// path then has a non-self entry).  Results do not depend on the bounds.
+struct BucketBounds {
+    uint32_t max_deg[kNumBuckets - 1] = {4, 8, 16, 32, 64, 256, 1024, 4096};
+};
+// The sets the tuner chooses from (the first is the default).
+constexpr int kBoundSets = 3;
+constexpr uint32_t kBoundSet[kBoundSets][kNumBuckets - 1] = {
+    {4, 8, 16, 32, 64, 256, 1024, 4096},
+    {3, 6, 12, 24, 48, 192, 768, 3072},
+    {2, 4, 8, 16, 32, 128, 512, 2048}};
+bool bounds_valid(const uint32_t *max_deg);
+
 // Slots of a best-move table for a row of d entries: d + d/2 + 1 rounded up
 // to a multiple of 4, so the table is at most two thirds full and always has
 // an empty slot (a lookup of a missing key ends).  A bucket's LDS table has
@@ -163,6 +179,8 @@ void nonself_counts(const uint64_t *row, const uint32_t *col, uint32_t n, uint32
 // csel[k] += non-self entries of listed vertices in chunk k (vertices [cv[k], cv[k+1]))
 void chunk_selected(const uint32_t *list, uint32_t count, const uint32_t *nsel, const uint32_t *cv,
                     uint32_t nc, unsigned long long *csel, hipStream_t s);
+// src[j] = the row of entry j, deg[v] = row[v+1] - row[v]
+void coo_sources(const uint64_t *row, uint32_t n, uint32_t *src, uint32_t *deg, hipStream_t s);
 // deg[i] = row[ids[i]+1] - row[ids[i]]
 void gather_degrees(const uint64_t *row, const uint32_t *ids, uint32_t n, uint64_t *deg,
                     hipStream_t s);
@@ -182,7 +200,8 @@ void radix_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, si
                       uint32_t *hist, uint32_t *scan_scratch, hipStream_t s);
 
 // ---- level setup ----
-void bucket_ids(const uint64_t *row, uint32_t n, uint8_t *bucket, hipStream_t s);
+void bucket_ids(const uint64_t *row, uint32_t n, uint8_t *bucket, const BucketBounds &bounds,
+                hipStream_t s);
 // lists[b] receives the vertices of bucket b (all vertices); counts[b] their number.
 void build_lists_all(const uint8_t *bucket, uint32_t n, uint32_t *const *lists, uint32_t *counts,
                      hipStream_t s);

@@ -129,6 +129,16 @@ __global__ void k_gather_degrees(const uint64_t *row, const uint32_t *ids, uint3
         deg[i] = row[ids[i] + 1] - row[ids[i]];
 }
 
+// COO form of a CSR: the source of every entry, and every row's length.
+__global__ void k_coo_sources(const uint64_t *row, uint32_t n, uint32_t *src, uint32_t *deg) {
+    for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n;
+         v += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t b = row[v], e = row[v + 1];
+        deg[v] = (uint32_t)(e - b);
+        for (uint64_t j = b; j < e; ++j) src[j] = (uint32_t)v;
+    }
+}
+
 __global__ void k_community_totals(const uint32_t *comm, const uint32_t *kdeg, uint32_t n,
                                    uint32_t *tot) {
     for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n;
@@ -312,10 +322,10 @@ __global__ __launch_bounds__(kBlock) void k_rs_downsweep(const uint64_t *kin, co
 
 // ------------------------------------------------------------ level setup
 
-__global__ void k_bucket_ids(const uint64_t *row, uint32_t n, uint8_t *bucket) {
+__global__ void k_bucket_ids(const uint64_t *row, uint32_t n, uint8_t *bucket, BucketBounds bounds) {
     for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n;
          v += (size_t)gridDim.x * blockDim.x)
-        bucket[v] = bucket_of_degree(row[v + 1] - row[v]);
+        bucket[v] = bucket_of_degree(row[v + 1] - row[v], bounds);
 }
 
 
@@ -1502,6 +1512,12 @@ void chunk_selected(const uint32_t *list, uint32_t count, const uint32_t *nsel, 
                        s, list, count, nsel, cv, nc, csel);
 }
 
+void coo_sources(const uint64_t *row, uint32_t n, uint32_t *src, uint32_t *deg, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_coo_sources, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, row, n, src,
+                       deg);
+}
+
 void gather_degrees(const uint64_t *row, const uint32_t *ids, uint32_t n, uint64_t *deg,
                     hipStream_t s) {
     if (!n) return;
@@ -1565,10 +1581,20 @@ void radix_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, si
     }
 }
 
-void bucket_ids(const uint64_t *row, uint32_t n, uint8_t *bucket, hipStream_t s) {
+bool bounds_valid(const uint32_t *max_deg) {
+    if (max_deg[kNumBuckets - 2] < 2) return false;
+    for (int b = 0; b < kNumBuckets - 1; ++b) {
+        if (max_deg[b] > kBucketMaxDeg[b]) return false;
+        if (b > 0 && max_deg[b] < max_deg[b - 1]) return false;
+    }
+    return true;
+}
+
+void bucket_ids(const uint64_t *row, uint32_t n, uint8_t *bucket, const BucketBounds &bounds,
+                hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(k_bucket_ids, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, row, n,
-                       bucket);
+                       bucket, bounds);
 }
 
 void bucket_counts(const uint8_t *bucket, uint32_t n, uint32_t *counts, hipStream_t s) {

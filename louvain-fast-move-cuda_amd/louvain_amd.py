@@ -36,7 +36,8 @@ class Params(C.Structure):
                 ("chunk_emulation", C.c_int32), ("chunk_entries", C.c_uint64),
                 ("sweep_threshold", C.c_double), ("level_threshold", C.c_double),
                 ("max_sweeps", C.c_int64), ("max_levels", C.c_int32), ("profile", C.c_int32),
-                ("autotune", C.c_int32), ("refine", C.c_int32)]
+                ("autotune", C.c_int32), ("refine", C.c_int32),
+                ("bucket_max_deg", C.c_uint32 * 8)]
 
 
 class SweepRec(C.Structure):
@@ -141,6 +142,7 @@ SIGNATURES = {
                                               C.c_double, _vp, C.POINTER(C.c_int64),
                                               C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "lv_community_get_tiles": (C.c_int, [_vp, _vp]),
+    "lv_community_get_buckets": (C.c_int, [_vp, _vp]),
     "lv_community_set_tiles": (C.c_int, [_vp, _vp]),
     "lv_kernel_stats_get": (C.c_int, [_vp, C.POINTER(KernelStats)]),
     "lv_last_run_kernel_stats": (C.c_int, [C.POINTER(KernelStats)]),
@@ -218,6 +220,8 @@ def params(**kw):
             raise TypeError(f"unknown lv_params field {k!r}")
         if k == "mode" and isinstance(v, str):
             v = MODES[v]
+        if k == "bucket_max_deg":
+            v = (C.c_uint32 * 8)(*v)
         setattr(p, k, v)
     return p
 
@@ -370,6 +374,12 @@ class Community:
         _check(lib.lv_best_moves(self.h, int(kind), C.byref(prm or params()), _p(act), _p(tgt),
                                  C.byref(ea)))
         return tgt, ea.value
+
+    def buckets(self):
+        """Degree bounds of the best-move buckets in use (lv_community_get_buckets)."""
+        out = np.zeros(8, np.uint32)
+        _check(lib.lv_community_get_buckets(self.h, _p(out)))
+        return out.tolist()
 
     def modularity_parts(self):
         si = C.c_int64(); s = C.c_uint64(); q = C.c_double()

@@ -185,10 +185,12 @@ def test_ba_10000_oracle_q_matches_fp64_recompute():
 
 def test_params_layout_and_tile_variant_names():
     import ctypes as C
-    assert C.sizeof(L.Params) == 64
+    assert C.sizeof(L.Params) == 96
     with pytest.raises(TypeError):
         L.params(no_such_field=1)
     assert L.params(autotune=1).autotune == 1
+    assert list(L.params().bucket_max_deg) == [0] * 8
+    assert list(L.params(bucket_max_deg=[2, 4, 8, 16, 32, 128, 512, 2048]).bucket_max_deg)[-1] == 2048
     # variant = ((bucket*2 + fast)*2 + weighted)*3 + tile
     assert L.kernel_variant_name(((0 * 2 + 0) * 2 + 0) * 3 + 1) == "k_move_table<1, 8, 0, false, true>"
     assert L.kernel_variant_name(((5 * 2 + 0) * 2 + 1) * 3 + 1) == "k_move_table<32, 388, 0, true, true>"

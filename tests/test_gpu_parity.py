@@ -561,3 +561,55 @@ def test_refine_matches_oracle(fixture_path, mode, gain):
         assert a["modularity"] >= plain["modularity"]
         r = c.refine(g.n, L.params(mode=mode[1], gain=gain[1], **kw))
         assert r["modularity"] == a["modularity"] and np.array_equal(r["best"], a["best"])
+
+
+BOUND_SETS = [[2, 6, 12, 24, 48, 128, 512, 2048], [4, 8, 16, 32, 64, 256, 1024, 1024],
+              [1, 3, 9, 27, 64, 200, 1000, 4000], [3, 6, 12, 24, 48, 192, 768, 3072]]
+
+
+def test_perturbed_bucket_bounds_equal_oracle(fixture_path):
+    """Runs with other degree bounds of the best-move buckets (vertices moved
+    between LDS tables of other sizes and tiles, and between the LDS buckets
+    and the hub path) are trace for trace the oracle's: the results do not
+    depend on the bucketing.  The extreme set sends every row of more than
+    two entries down the hub path."""
+    ea, eb, n = L.gen_rmat(16, 16, 0.57, 0.19, 0.19, seed=31)
+    g = L.Graph.from_edges(ea, eb)
+    og = O.Graph.from_edges(ea, eb)
+    for mode in MODES[:2]:
+        ref = og.louvain(O.params(mode=mode[2]))
+        for bs in BOUND_SETS:
+            label = f"rmat16 {mode[0]} bounds {bs}"
+            say(label)
+            compare_runs(g.louvain(L.params(mode=mode[1], bucket_max_deg=bs), hash_trace=True),
+                         ref, label)
+    fg = L.Graph.load_edge(fixture_path)
+    ofg = O.Graph.load_edge(fixture_path)
+    for gain in GAINS:
+        for tie in TIES:
+            label = f"fixture hubs everywhere {gain[0]} {tie[0]}"
+            say(label)
+            a = fg.louvain(L.params(gain=gain[1], tie=tie[1], bucket_max_deg=[0] * 7 + [2]),
+                           hash_trace=True)
+            compare_runs(a, ofg.louvain(O.params(gain=gain[2], tie=tie[2])), label)
+    for bad in ([5, 8, 16, 32, 64, 256, 1024, 4096], [4, 8, 16, 32, 64, 256, 1024, 1],
+                [4, 8, 16, 8, 64, 256, 1024, 4096], [4, 8, 16, 32, 64, 256, 1024, 5000]):
+        with pytest.raises(L.LouvainError) as e:
+            fg.louvain(L.params(bucket_max_deg=bad))
+        assert e.value.status == 1                    # LV_ERR_ARG
+
+
+def test_bucket_bound_tuner_keeps_results():
+    """autotune = 1 times the three bound sets on the community's first run
+    and keeps one; the run and a second one equal the oracle's."""
+    ea, eb, n = L.gen_rmat(17, 16, 0.57, 0.19, 0.19, seed=32)
+    g = L.Graph.from_edges(ea, eb)
+    ref = O.Graph.from_edges(ea, eb).louvain(O.params())
+    c = L.Community(g)
+    for rep in range(2):
+        c.reset()
+        r = c.run(L.params(autotune=1), hash_trace=True, n0=g.n)
+        compare_runs(r, ref, f"tuned run {rep}")
+    say(f"tuned bounds: {c.buckets()}")
+    assert c.buckets() in ([4, 8, 16, 32, 64, 256, 1024, 4096], [3, 6, 12, 24, 48, 192, 768, 3072],
+                           [2, 4, 8, 16, 32, 128, 512, 2048])

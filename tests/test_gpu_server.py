@@ -142,3 +142,30 @@ def test_same_graph_many_times_and_errors():
     with pytest.raises(L.LouvainError) as e:
         S.submit(big)
     assert e.value.status == 6
+
+
+def test_served_mix_with_rmat_jobs():
+    """The served configuration's mix in small: Barabasi-Albert jobs of the
+    batched configuration's distribution beside R-MAT jobs (scales 16-17,
+    edge factor 16, (0.57, 0.19, 0.19, 0.05): stand-ins for the 5% of R-MAT-20
+    requests, with hub rows above 4096 entries), submitted in waves while
+    earlier jobs are at later levels; each job's result equals the oracle's."""
+    rng = np.random.default_rng(55)
+    jobs = []
+    for i in range(10):
+        if i in (2, 7):
+            ea, eb, _ = L.gen_rmat(16 + (i == 7), 16, 0.57, 0.19, 0.19, seed=500 + i)
+        else:
+            n = int(np.exp(rng.uniform(np.log(200), np.log(200_000))))
+            ea, eb = L.gen_ba(n, int(rng.integers(2, 9)), seed=5_000_000 + i)
+        jobs.append((ea, eb, None))
+    graphs = [L.Graph.from_edges(a, b, w) for a, b, w in jobs]
+    say("served mix: " + ", ".join(f"n={g.n} E={g.E}" for g in graphs))
+    S = L.Server(L.params())
+    ids = []
+    for k, g in enumerate(graphs):
+        ids.append(S.submit(g))
+        if k % 3 == 2:
+            S.step()
+    _drain(S)
+    _check(S, ids, graphs, jobs, {}, {}, "served mix")
