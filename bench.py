@@ -9,7 +9,10 @@ One step = one full Louvain run to convergence (all levels, the reference's
 value = traversed directed entries per second over the whole run (sum over
 sweeps of the entries of the vertices examined, divided by the step time).
 The roofline block covers the dominant best-move kernel, timed with HIP
-events on the stream it runs on, and the whole best-move pass; the CPU
+events on the stream it runs on, and the whole best-move pass, and sets
+the pass beside the measured random-gather ceiling of the same graph
+(`gather_ceiling`: a kernel that does nothing but the pass's dependent
+neighbour-word gather); the CPU
 baseline is the single-threaded oracle run to convergence on the same CSR.
 `per_sweep` gives traversed entries per second of the best-move pass for
 sweep 0 of level 0 and the entry-weighted mean over all sweeps; `buckets`
@@ -587,6 +590,25 @@ def main():
                    for s in last["sweeps"]],
         "note": "e_active / HIP-event time of the sweep's best-move pass; the weighted mean is "
                 "total entries over total pass time of the timed runs"}
+    # the measured ceiling of the pass's dependent access on this graph: the
+    # rate of a kernel that only streams col and gathers arr[col[j]]
+    # (4 bytes: tot in the first sweep of a level; 8 bytes: comm | tot in the
+    # hash sweeps)
+    g4, g8 = c.gather_probe(4), c.gather_probe(8)
+    hs = [s for s in all_sw if s["kind"] != 0 and s["level"] == 0]
+    hash_teps = (sum(s["e_active"] for s in hs) / (sum(s["move_ms"] for s in hs) / 1e3)
+                 if hs else None)
+    gather_ceiling = {
+        "gathers_per_s_4B": g4, "gathers_per_s_8B": g8,
+        "sweep0_level0_frac": per_sweep["sweep0_level0_teps"] / g4,
+        "hash_sweeps_level0_teps": hash_teps,
+        "hash_sweeps_level0_frac": hash_teps / g8 if hash_teps else None,
+        "note": "lv_gather_probe on the level-0 CSR: entries/s of the best-move pass over "
+                "gathers/s of the bare stream-col + gather-arr[col] kernel (4 B for the first "
+                "sweep's totals, 8 B for the hash sweeps' packed community|total words)"}
+    log(f"gather ceiling: {g4 / 1e9:.1f} G/s (4 B), {g8 / 1e9:.1f} G/s (8 B); "
+        f"sweep 0 at {gather_ceiling['sweep0_level0_frac']:.2f}, level-0 hash sweeps at "
+        f"{gather_ceiling['hash_sweeps_level0_frac']}")
     if args.no_extras:
         reference_method, buckets = None, None
     else:
@@ -639,6 +661,7 @@ def main():
                      "kernel": L.kernel_variant_name(vi),
                      "launches": v_launch, "avg_launch_ms": v_ms, "avg_bytes": v_bytes,
                      "copy_kernel_gbs": copy_gbs,
+                     "gather_ceiling": gather_ceiling,
                      "best_move_pass": {"achieved_gbs": pass_gbs,
                                         "frac": pass_gbs / HBM_PEAK_GBS,
                                         "ms_per_run": ks["move_ms"] / args.steps,

@@ -267,6 +267,12 @@ lv_status lv_last_run_kernel_stats(lv_kernel_stats *s);
  * ceiling next to the nominal 8 TB/s, and the known-size reference for
  * FETCH_SIZE/WRITE_SIZE counters. */
 lv_status lv_bandwidth_probe(uint64_t bytes, int reps, double *gbs);
+/* Random-gather ceiling of a graph (no counterpart in the reference): the
+ * rate, in gathers per second over `reps` launches, of a kernel that streams
+ * the column ids of the community's input graph and loads arr[col[j]] for
+ * each (arr: `width` = 4 or 8 bytes per vertex), the dependent access of
+ * every best-move and commit pass with none of their other work. */
+lv_status lv_gather_probe(const lv_community *c, uint32_t width, int reps, double *gathers_per_s);
 /* Kernel name of best-move variant v for gain mode `gain` (as rocprofv3 lists it). */
 lv_status lv_kernel_variant_name(int variant, int gain, char *buf, size_t len);
 /* Test hook, no counterpart in the reference: on != 0 makes every commit keep

@@ -504,6 +504,39 @@ lv_status lv_bandwidth_probe(uint64_t bytes, int reps, double *gbs) {
     });
 }
 
+lv_status lv_gather_probe(const lv_community *c, uint32_t width, int reps,
+                          double *gathers_per_s) {
+    if (!c || (width != 4 && width != 8) || reps < 1 || !gathers_per_s) return LV_ERR_ARG;
+    const lv_status d = need_device();
+    if (d != LV_OK) return d;
+    return guarded([&] {
+        const GraphDevice &g = c->c.graph0;   // the input graph (graph is the current level)
+        const uint64_t E = g.n_entries;
+        if (!E) throw lv::Error(LV_ERR_ARG, "graph has no entries");
+        lv::DeviceArray<uint8_t> arr((size_t)g.n_nodes * width);
+        lv::DeviceArray<unsigned long long> out(lv::kGatherProbeWords);
+        hipStream_t s;
+        LV_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        hipEvent_t e0, e1;
+        LV_HIP(hipEventCreate(&e0));
+        LV_HIP(hipEventCreate(&e1));
+        lv::fill_u32(reinterpret_cast<uint32_t *>(arr.data()), 1u, (size_t)g.n_nodes * width / 4, s);
+        const uint32_t *col = g.edge_destination.data();
+        lv::gather_probe(col, E, arr.data(), (int)width, out.data(), s);   // warm
+        LV_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r) lv::gather_probe(col, E, arr.data(), (int)width, out.data(), s);
+        LV_HIP(hipEventRecord(e1, s));
+        LV_HIP(hipEventSynchronize(e1));
+        LV_HIP(hipGetLastError());
+        float ms = 0;
+        LV_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *gathers_per_s = (double)E * reps / (ms * 1e-3);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+    });
+}
+
 lv_status lv_debug_mark_bytes(int32_t on) {
     lv::set_mark_bytes(on != 0);
     return LV_OK;

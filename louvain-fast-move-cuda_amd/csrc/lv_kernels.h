@@ -173,6 +173,12 @@ void iota_u32(uint32_t *p, size_t n, hipStream_t s);
 void scatter_u32(uint32_t *dst, const uint32_t *idx, const uint32_t *val, size_t n, hipStream_t s);
 // plain device copy of `bytes` (a multiple of 16) with k_stream_copy
 void stream_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
+// gather probe: arr[col[j]] (width 4 or 8 bytes) over E column ids on a grid
+// of kGatherProbeBlocks workgroups of 256 lanes; out holds a word per lane
+constexpr uint32_t kGatherProbeBlocks = 16384;
+constexpr size_t kGatherProbeWords = (size_t)kGatherProbeBlocks * 256;
+void gather_probe(const uint32_t *col, uint64_t E, const void *arr, int width,
+                  unsigned long long *out, hipStream_t s);
 // number of non-self entries of every vertex
 void nonself_counts(const uint64_t *row, const uint32_t *col, uint32_t n, uint32_t *nsel,
                     hipStream_t s);

@@ -87,6 +87,35 @@ __global__ __launch_bounds__(kBlock) void k_stream_copy(const uint4 *src, uint4 
         dst[i] = src[i];
 }
 
+constexpr int kProbeIlp = 4;
+
+// Random-gather probe: each lane streams kProbeIlp column ids and gathers the
+// element arr[col[j]] of each (4 or 8 bytes), the one dependent access of
+// the best-move pass without its table work: the measured ceiling of that
+// access pattern on this graph.  The sum keeps the loads live.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gather_probe(const uint32_t *__restrict__ col,
+                                                         uint64_t E, const T *__restrict__ arr,
+                                                         unsigned long long *out) {
+    unsigned long long acc = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j0 < E;
+         j0 += stride * kProbeIlp) {
+        uint32_t x[kProbeIlp];
+#pragma unroll
+        for (int u = 0; u < kProbeIlp; ++u) {
+            const uint64_t j = j0 + (uint64_t)u * stride;
+            x[u] = j < E ? stream_load(&col[j]) : 0u;
+        }
+        T e[kProbeIlp];
+#pragma unroll
+        for (int u = 0; u < kProbeIlp; ++u) e[u] = arr[x[u]];
+#pragma unroll
+        for (int u = 0; u < kProbeIlp; ++u) acc += (unsigned long long)e[u];
+    }
+    out[blockIdx.x * (uint64_t)blockDim.x + threadIdx.x] = acc;
+}
+
 // Non-self entries of every vertex (rows of contracted levels are unsorted).
 __global__ void k_nonself_counts(const uint64_t *row, const uint32_t *col, uint32_t n,
                                  uint32_t *nsel) {
@@ -1496,6 +1525,18 @@ void stream_copy(const void *src, void *dst, size_t bytes, hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(k_stream_copy, dim3(grid_for(n, kBlock * 8, 1u << 20)), dim3(kBlock), 0, s,
                        (const uint4 *)src, (uint4 *)dst, n);
+}
+
+void gather_probe(const uint32_t *col, uint64_t E, const void *arr, int width,
+                  unsigned long long *out, hipStream_t s) {
+    static_assert(kGatherProbeWords == (size_t)kGatherProbeBlocks * kBlock, "a word per lane");
+    if (!E) return;
+    if (width == 8)
+        hipLaunchKernelGGL(k_gather_probe<unsigned long long>, dim3(kGatherProbeBlocks),
+                           dim3(kBlock), 0, s, col, E, (const unsigned long long *)arr, out);
+    else
+        hipLaunchKernelGGL(k_gather_probe<uint32_t>, dim3(kGatherProbeBlocks), dim3(kBlock), 0, s,
+                           col, E, (const uint32_t *)arr, out);
 }
 
 void nonself_counts(const uint64_t *row, const uint32_t *col, uint32_t n, uint32_t *nsel,

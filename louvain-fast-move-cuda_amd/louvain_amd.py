@@ -149,6 +149,7 @@ SIGNATURES = {
     "lv_kernel_variant_name": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.c_size_t]),
     "lv_debug_mark_bytes": (C.c_int, [C.c_int32]),
     "lv_bandwidth_probe": (C.c_int, [C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
+    "lv_gather_probe": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.POINTER(C.c_double)]),
     "lv_batch_create": (C.c_int, [C.POINTER(_vp), C.c_uint32, _vp, C.POINTER(_vp)]),
     "lv_batch_destroy": (None, [_vp]),
     "lv_batch_info": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
@@ -380,6 +381,13 @@ class Community:
         out = np.zeros(8, np.uint32)
         _check(lib.lv_community_get_buckets(self.h, _p(out)))
         return out.tolist()
+
+    def gather_probe(self, width=8, reps=5):
+        """Gathers per second of arr[col[j]] over this community's input graph
+        (lv_gather_probe): the ceiling of the best-move pass's dependent access."""
+        g = C.c_double()
+        _check(lib.lv_gather_probe(self.h, int(width), int(reps), C.byref(g)))
+        return g.value
 
     def modularity_parts(self):
         si = C.c_int64(); s = C.c_uint64(); q = C.c_double()

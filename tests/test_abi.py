@@ -1,6 +1,7 @@
 """CPU tests of the C ABI library: it loads, exports every declared entry
 point, and its host-side parts (.edge parsing, CSR construction, generators,
 error mapping) agree with the oracle.  No kernel is launched here."""
+import ctypes as C
 import os
 import re
 
@@ -102,6 +103,8 @@ def test_community_without_device_reports_no_device(fixture_path):
     with pytest.raises(L.LouvainError) as e:
         L.Server()
     assert e.value.status == 8
+    g8 = C.c_double()
+    assert L.lib.lv_gather_probe(None, 8, 1, C.byref(g8)) == 1       # LV_ERR_ARG
 
 
 def test_server_footprint_and_argument_errors(fixture_path):

@@ -96,3 +96,18 @@ def test_planted_1m_reference_method(planted):
     say(f"planted 1e6: Q exact {q_exact:.9f}, Q reference method {q_ref:.9f}, "
         f"relative difference {(q_exact - q_ref) / abs(q_ref):+.3%}")
     assert abs(q_exact - q_ref) <= 0.005 * abs(q_ref)
+
+
+def test_gather_probe_rates_are_sane(planted):
+    """lv_gather_probe on the planted 1e6 graph: both widths give a finite
+    positive rate below one gather per byte of the 8 TB/s peak, and widths
+    other than 4 and 8 are refused."""
+    import ctypes as C
+    ea, eb, w, _ = planted
+    c = L.Community(L.Graph.from_edges(ea, eb, w))
+    g4, g8 = c.gather_probe(4), c.gather_probe(8)
+    say(f"gather probe planted-1e6: {g4 / 1e9:.1f} G/s (4 B), {g8 / 1e9:.1f} G/s (8 B)")
+    for g in (g4, g8):
+        assert np.isfinite(g) and 1e9 < g < 8e12
+    out = C.c_double()
+    assert L.lib.lv_gather_probe(c.h, 3, 1, C.byref(out)) == 1
