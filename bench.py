@@ -80,7 +80,7 @@ def cpu_baseline(row, col, mode, threshold):
                       f"CPU {cpu_model()}, nproc {os.cpu_count()}"}
 
 
-def extra_runs(args, L, c, info, q_ex):
+def extra_runs(args, L, c, info, q_ex, bounds):
     """Runs outside the timed steps: the reference's own method on the same
     resident graph, and one run with the best-move pass on a single stream so
     that every degree bucket is timed alone."""
@@ -107,7 +107,8 @@ def extra_runs(args, L, c, info, q_ex):
     log(f"reference method: Q={q_rf:.9f} (default {q_ex:.9f}), {ref_s * 1e3:.1f} ms")
     # each bucket alone: one run with the pass on a single stream, every bucket timed
     prm3 = L.params(mode=args.mode, sweep_threshold=args.threshold,
-                    level_threshold=args.threshold, profile=3, autotune=0)
+                    level_threshold=args.threshold, profile=3, autotune=0,
+                    bucket_max_deg=bounds)   # the bounds of the timed runs
     c.reset()
     c.run(prm3, best=False, n0=info["n"])
     kb = c.kernel_stats()
@@ -524,6 +525,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="rmat: only the timed runs (no reference-method or per-bucket runs), "
                          "for kernel traces")
+    ap.add_argument("--bucket-max-deg", default=None,
+                    help="rmat: explicit degree bounds of the 8 best-move buckets, comma "
+                         "separated (default: the library's default or tuned bounds)")
     ap.add_argument("--loop-sample", type=int, default=1,
                     help="batched: time the per-graph loop on every k-th graph only")
     args = ap.parse_args()
@@ -551,8 +555,10 @@ def main():
     t = time.perf_counter()
     c = L.Community(g)
     log(f"upload + level setup ({time.perf_counter() - t:.2f}s)")
+    bounds = {} if not args.bucket_max_deg else {
+        "bucket_max_deg": [int(x) for x in args.bucket_max_deg.split(",")]}
     prm = L.params(mode=args.mode, sweep_threshold=args.threshold,
-                   level_threshold=args.threshold, profile=2, autotune=args.autotune)
+                   level_threshold=args.threshold, profile=2, autotune=args.autotune, **bounds)
     for i in range(args.warmup):
         c.reset()
         r = c.run(prm, best=False, n0=info["n"])
@@ -569,6 +575,7 @@ def main():
         log(f"step {i}: {dt * 1e3:.1f} ms (in-library {r['seconds'] * 1e3:.1f} ms), "
             f"Q={r['modularity']:.6f}, comms={r['n_comm']}, levels={r['levels']}, "
             f"sweeps={r['total_sweeps']}")
+    timed_bounds = c.buckets()   # the bucket bounds of the timed runs
     total = sum(times)
     traversed = sum(r["traversed_entries"] for r in results)
     last = results[-1]
@@ -585,7 +592,8 @@ def main():
         "sweeps": [{"level": s["level"], "sweep": s["sweep"], "kind": s["kind"],
                     "e_active": s["e_active"], "n_active": s["n_active"],
                     "candidates": s["candidates"], "move_ms": s["move_ms"],
-                    "sweep_ms": s["sweep_ms"],
+                    "sweep_ms": s["sweep_ms"], "commit_ms": s["commit_ms"],
+                    "mover_entries": s["mover_entries"],
                     "move_gbs": s["move_bytes"] / s["move_ms"] / 1e6 if s["move_ms"] > 0 else 0.0}
                    for s in last["sweeps"]],
         "note": "e_active / HIP-event time of the sweep's best-move pass; the weighted mean is "
@@ -612,9 +620,39 @@ def main():
     if args.no_extras:
         reference_method, buckets = None, None
     else:
-        reference_method, buckets = extra_runs(args, L, c, info, last["modularity"])
+        reference_method, buckets = extra_runs(args, L, c, info, last["modularity"],
+                                               timed_bounds)
     move_s = ks["move_ms"] / 1e3
     pass_gbs = ks["move_bytes"] / move_s / 1e9 if move_s > 0 else 0.0
+    # the commit (apply + push-form marking, the incremental change of
+    # sigma_in and the next lists) and the contraction, by their per-unit byte
+    # figures over their HIP-event time.  The commit's incremental sigma_in
+    # replaces the reference's full modularity pass after every sweep
+    # (beta*E + 16*n bytes); the second figure counts that pass's bytes too.
+    commit_s = ks["commit_ms"] / 1e3
+    bq = sum((12 if sw["level"] > 0 else 8) * sw["E"] + 16 * sw["n"]
+             for r in results for sw in r["sweeps"])
+    commit_pass = {
+        "achieved_gbs": ks["commit_bytes"] / commit_s / 1e9 if commit_s > 0 else 0.0,
+        "frac": ks["commit_bytes"] / commit_s / 1e9 / HBM_PEAK_GBS if commit_s > 0 else 0.0,
+        "achieved_gbs_incl_modularity_pass":
+            (ks["commit_bytes"] + bq) / commit_s / 1e9 if commit_s > 0 else 0.0,
+        "ms_per_run": ks["commit_ms"] / args.steps,
+        "mover_entries_per_run": ks["commit_entries"] / args.steps,
+        "note": "bytes 8*n_A + 24*n_moved + 5*mover_entries + n per sweep; time = sweep - "
+                "best-move pass (commit kernels and the finalize pass)"}
+    contract_s = ks["contract_ms"] / 1e3
+    contraction = {
+        "achieved_gbs": ks["contract_bytes"] / contract_s / 1e9 if contract_s > 0 else 0.0,
+        "frac": ks["contract_bytes"] / contract_s / 1e9 / HBM_PEAK_GBS if contract_s > 0 else 0.0,
+        "ms_per_run": ks["contract_ms"] / args.steps,
+        "contractions_per_run": ks["contractions"] / args.steps,
+        "levels": [{"level": lv["level"], "E": lv["E"], "E_new": lv["E_new"],
+                    "contract_ms": lv["contract_ms"],
+                    "gbs": lv["contract_bytes"] / lv["contract_ms"] / 1e6
+                    if lv["contract_ms"] > 0 else 0.0} for lv in last["level_recs"]],
+        "note": "bytes beta*E + 8*E_new + 12*n + 12*n_comm per contraction; time from "
+                "renumbering to the contracted graph, host planning included"}
     # dominant best-move kernel: the degree 257-1024 bucket's variant (one
     # kernel launch per sweep) with the most event-timed time over the timed
     # runs.  That bucket runs alone on the device; the others overlap on side
@@ -649,7 +687,7 @@ def main():
         "config": {"workload": f"plain Louvain, R-MAT scale {args.scale}, edge factor 16, "
                                f"(0.57,0.19,0.19,0.05), seed 2, symmetrised, unit weights",
                    "mode": args.mode, "threshold": args.threshold, "n": info["n"],
-                   "E": info["E"], "bucket_max_deg": c.buckets()},
+                   "E": info["E"], "bucket_max_deg": timed_bounds},
         "s_to_converge": total / args.steps,
         "modularity": last["modularity"],
         "n_communities": last["n_comm"],
@@ -665,7 +703,9 @@ def main():
                      "best_move_pass": {"achieved_gbs": pass_gbs,
                                         "frac": pass_gbs / HBM_PEAK_GBS,
                                         "ms_per_run": ks["move_ms"] / args.steps,
-                                        "sweeps_per_run": ks["move_launches"] / args.steps}},
+                                        "sweeps_per_run": ks["move_launches"] / args.steps},
+                     "commit_pass": commit_pass,
+                     "contraction": contraction},
         "per_sweep": per_sweep,
         "buckets": buckets,
         "reference_method": reference_method,

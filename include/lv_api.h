@@ -96,6 +96,13 @@ typedef struct lv_sweep_rec {
      * weights, 12 weighted) */
     double move_ms, sweep_ms;
     uint64_t candidates, move_bytes;
+    /* mover_entries (always): entries of the rows of the vertices that moved.
+     * With profile set: HIP-event time of the commit (apply, marking, the
+     * change of sigma_in, next lists: sweep_ms - move_ms) and its algorithmic
+     * bytes 8*n_active + 24*n_moved + 5*mover_entries + n (apply + push-form
+     * marking). */
+    double commit_ms;
+    uint64_t commit_bytes, mover_entries;
 } lv_sweep_rec;
 
 typedef struct lv_level_rec {
@@ -104,6 +111,15 @@ typedef struct lv_level_rec {
     uint64_t E;
     double q_start, q_end;
     uint64_t best_hash;        /* 0 unless requested */
+    /* with profile set and a contraction after the level (else 0): HIP-event
+     * time from renumbering to the contracted graph (host planning steps
+     * included), the contracted graph's size, and the algorithmic bytes
+     * beta*E + 8*E_new + 12*n + 12*n_comm (beta = 8 at unit weights, 12
+     * weighted: col, w, new id of the destination; the contracted entries;
+     * per vertex its row offset and new id, per community its offset and
+     * degree) */
+    double contract_ms;
+    uint64_t E_new, contract_bytes;
 } lv_level_rec;
 
 typedef struct lv_result {
@@ -143,6 +159,15 @@ typedef struct lv_kernel_stats {
     double bucket_ms[18];
     uint64_t bucket_bytes[18];
     uint64_t bucket_entries[18];
+    /* commit passes (lv_sweep_rec.commit_*) and contractions
+     * (lv_level_rec.contract_*) summed over the timed sweeps and levels;
+     * commit_entries = mover entries, contract_entries = entries contracted */
+    uint64_t commit_passes;
+    double commit_ms;
+    uint64_t commit_bytes, commit_entries;
+    uint64_t contractions;
+    double contract_ms;
+    uint64_t contract_bytes, contract_entries;
 } lv_kernel_stats;
 
 typedef struct lv_graph lv_graph;          /* GraphHost (graphHost.cuh:7-18): host CSR */

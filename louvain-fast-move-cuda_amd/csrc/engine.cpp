@@ -479,6 +479,7 @@ static SweepArgs sweep_args(Community &c, const lv_params &p, int kind, uint32_t
     a.movers = st.movers.data();
     a.n_movers = st.n_movers.data();
     a.ct = st.ct.data();
+    a.wide5 = st.bounds.max_deg[5] > kBucketMaxDeg[5];
     if (c.batch) a.bt = c.batch->view();
     return a;
 }
@@ -636,10 +637,11 @@ int OptimizationPhase::sweep_loop(Community &c, const lv_params &p, int64_t exec
         const uint64_t moved = sum_slots(h, ST_MOVED);
         const int64_t dsig = (int64_t)sum_slots(h, ST_DSIGMA);
         const uint64_t sq = sum_slots(h, ST_SUMSQ);
+        const uint64_t ment = sum_slots(h, ST_MOVER_ENTRIES);
         uint32_t next_counts[kNumBuckets];
         std::memcpy(next_counts, h + kStatWordsOffset, sizeof(next_counts));
         float ms_move = 0, ms_sweep = 0;
-        uint64_t sweep_cand = 0, sweep_bytes = 0;
+        uint64_t sweep_cand = 0, sweep_bytes = 0, commit_bytes = 0;
         if (p.profile) {
             LV_HIP(hipEventElapsedTime(&ms_move, st.ev[0], st.ev[1]));
             LV_HIP(hipEventElapsedTime(&ms_sweep, st.ev[0], st.ev[2]));
@@ -686,6 +688,15 @@ int OptimizationPhase::sweep_loop(Community &c, const lv_params &p, int64_t exec
             st.acc.move_entries += e_active;
             st.acc.sweep_ms += ms_sweep;
             st.acc.sweeps += 1;
+            // the commit by the per-unit figures of the roofline basis: per
+            // listed vertex its community and target (8), per mover the two
+            // total updates and its community write (24), per mover entry the
+            // column id and the neighbour's flag (5), a flag per vertex (1)
+            commit_bytes = 8ull * n_active + 24ull * moved + 5ull * ment + n;
+            st.acc.commit_passes += 1;
+            st.acc.commit_ms += ms_sweep - ms_move;
+            st.acc.commit_bytes += commit_bytes;
+            st.acc.commit_entries += ment;
         }
         // fast_move_update + compute_modularity (phaseOptimization.cuh:169-172)
         c.sigma_in += dsig;
@@ -711,6 +722,9 @@ int OptimizationPhase::sweep_loop(Community &c, const lv_params &p, int64_t exec
             r.sweep_ms = ms_sweep;
             r.candidates = sweep_cand;
             r.move_bytes = sweep_bytes;
+            r.commit_ms = ms_sweep - ms_move;
+            r.commit_bytes = commit_bytes;
+            r.mover_entries = ment;
             if (hash_trace) {
                 comm_host.resize(n);
                 LV_HIP(hipMemcpy(comm_host.data(), c.communities.data(), n * sizeof(uint32_t),
@@ -1066,7 +1080,24 @@ void ModularityAlgorithms::run(Community &C, const lv_params &p, lv_result *r) {
         cont = (C.modularity - old_q) > p.level_threshold;            // :52
         if (p.max_levels > 0 && lvl + 1 >= p.max_levels) cont = false;
         const double q_end = C.modularity;
+        const bool wtd = !C.graph.unit_weights;
+        if (p.profile) LV_HIP(hipEventRecord(C.state->ev[3], s));
         AggregationPhase::run(C, cont, (MODE)p.mode);                 // :53
+        float ms_contract = 0;
+        uint64_t contract_bytes = 0, e_new = 0;
+        if (cont) e_new = C.graph.n_entries;
+        if (p.profile && cont) {
+            LV_HIP(hipEventRecord(C.state->ev[4], s));
+            LV_HIP(hipEventSynchronize(C.state->ev[4]));
+            LV_HIP(hipEventElapsedTime(&ms_contract, C.state->ev[3], C.state->ev[4]));
+            contract_bytes = (wtd ? 12ull : 8ull) * e_lvl + 8ull * e_new + 12ull * n_lvl +
+                             12ull * C.n_of_best_communities;
+            KernelStatsAcc &acc = C.state->acc;
+            acc.contractions += 1;
+            acc.contract_ms += ms_contract;
+            acc.contract_bytes += contract_bytes;
+            acc.contract_entries += e_lvl;
+        }
         if (r && r->level_recs && nlev < r->levels_cap) {
             lv_level_rec &L = r->level_recs[nlev];
             std::memset(&L, 0, sizeof(L));
@@ -1077,6 +1108,9 @@ void ModularityAlgorithms::run(Community &C, const lv_params &p, lv_result *r) {
             L.E = e_lvl;
             L.q_start = old_q;
             L.q_end = q_end;
+            L.contract_ms = ms_contract;
+            L.E_new = e_new;
+            L.contract_bytes = contract_bytes;
             if (r->hash_trace) {
                 best_host.resize(C.n_original);
                 LV_HIP(hipMemcpy(best_host.data(), C.best_communities.data(),

@@ -38,6 +38,13 @@ struct KernelStatsAcc {
     double bk_ms[kNumBuckets][2] = {};
     uint64_t bk_bytes[kNumBuckets][2] = {};
     uint64_t bk_entries[kNumBuckets][2] = {};
+    // commit passes and contractions
+    uint64_t commit_passes = 0;
+    double commit_ms = 0;
+    uint64_t commit_bytes = 0, commit_entries = 0;
+    uint64_t contractions = 0;
+    double contract_ms = 0;
+    uint64_t contract_bytes = 0, contract_entries = 0;
 };
 
 extern KernelStatsAcc g_last_run_stats;
@@ -137,7 +144,7 @@ struct SweepState {
     SideStreams side;
     const SideStreams *side_streams();
     // profiling
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // [3], [4]: contraction
     hipEvent_t bev[2 * kNumBuckets] = {};
     KernelStatsAcc acc;
 

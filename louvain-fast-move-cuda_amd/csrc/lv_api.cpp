@@ -475,6 +475,14 @@ static void fill_stats(lv_kernel_stats *s, const lv::KernelStatsAcc &a) {
             s->bucket_bytes[2 * b + f] = a.bk_bytes[b][f];
             s->bucket_entries[2 * b + f] = a.bk_entries[b][f];
         }
+    s->commit_passes = a.commit_passes;
+    s->commit_ms = a.commit_ms;
+    s->commit_bytes = a.commit_bytes;
+    s->commit_entries = a.commit_entries;
+    s->contractions = a.contractions;
+    s->contract_ms = a.contract_ms;
+    s->contract_bytes = a.contract_bytes;
+    s->contract_entries = a.contract_entries;
 }
 
 lv_status lv_bandwidth_probe(uint64_t bytes, int reps, double *gbs) {

@@ -30,8 +30,11 @@ constexpr uint32_t kBucketMaxDeg[kNumBuckets - 1] = {4, 8, 16, 32, 64, 256, 1024
 // Degree bounds of the LDS buckets at run time: a row of d entries goes to the
 // first bucket b with d <= max_deg[b], to the hubs above max_deg[7].  Each
 // bound is at most kBucketMaxDeg[b] (so a bucket's tables and tiles hold its
-// rows), the bounds do not decrease, and max_deg[7] >= 2 (a row in the hub
-// path then has a non-self entry).  Results do not depend on the bounds.
+// rows), except bucket 5's, which may reach kWide5MaxDeg (its table kernels
+// then use tables of table_slots(kWide5MaxDeg) slots); the bounds do not
+// decrease, and max_deg[7] >= 2 (a row in the hub path then has a non-self
+// entry).  Results do not depend on the bounds.
+constexpr uint32_t kWide5MaxDeg = 512;
 struct BucketBounds {
     uint32_t max_deg[kNumBuckets - 1] = {4, 8, 16, 32, 64, 256, 1024, 4096};
 };
@@ -81,7 +84,8 @@ enum Stat : int {
     ST_DSIGMA = 4,       // change of sigma_in (two's complement)
     ST_SUMSQ = 5,        // sum over communities of tot^2
     ST_TWRITES = 6,      // target writes of the best-move pass (movers)
-    kNumStats = 7
+    ST_MOVER_ENTRIES = 7,  // entries of the movers' rows, read by the commit
+    kNumStats = 8
 };
 
 struct LevelView {
@@ -145,6 +149,7 @@ struct SweepArgs {
     unsigned long long *ct;      // [n] scratch: comm | tot[comm] << 32 during a hash sweep's
                                  // best-move pass, then comm | target << 32 for its commit
     bool counters_zeroed = false; // n_movers and the next-list counts were zeroed by sweep_reset
+    bool wide5 = false;          // bucket 5's bound is above kBucketMaxDeg[5] (wider tables)
 };
 
 // ct <- comm | tot[comm] << 32 of every vertex: the packed operand of the hash

@@ -1268,7 +1268,8 @@ struct SmallLists {
 
 template <int G, bool WEIGHTED, int CM>
 __device__ inline int64_t commit_group(const CommitCtx &a, const uint32_t *__restrict__ list,
-                                       uint32_t count, uint32_t wb, uint32_t &v) {
+                                       uint32_t count, uint32_t wb, uint32_t &v,
+                                       uint64_t &ment) {
     constexpr int GPB = kBlock / G;
     const int lane = threadIdx.x & (G - 1);
     const uint32_t idx = wb * GPB + threadIdx.x / G;
@@ -1279,8 +1280,10 @@ __device__ inline int64_t commit_group(const CommitCtx &a, const uint32_t *__res
         const uint32_t c = a.target[v];
         const uint64_t beg = a.row[v], end = a.row[v + 1];
         const uint32_t kv = a.kdeg[v];
-        if (c != own)
+        if (c != own) {
             dsig = commit_row<WEIGHTED, CM>(a, v, own, c, beg, end, kv, (uint32_t)lane, (uint32_t)G);
+            if (lane == 0) ment = end - beg;
+        }
     }
     return dsig;
 }
@@ -1292,16 +1295,17 @@ __global__ __launch_bounds__(kBlock) void k_commit(CommitCtx a, SmallLists sl) {
     const uint32_t wb = blockIdx.x - sl.wg[q];
     uint32_t v = LV_EMPTY;
     int64_t dsig = 0;
+    uint64_t ment = 0;
     switch (kSmallG[q]) {   // uniform over the workgroup
-        case 2: dsig = commit_group<2, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v); break;
-        case 4: dsig = commit_group<4, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v); break;
-        case 8: dsig = commit_group<8, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v); break;
-        case 16: dsig = commit_group<16, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v); break;
-        case 32: dsig = commit_group<32, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v); break;
-        default: dsig = commit_group<64, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v); break;
+        case 2: dsig = commit_group<2, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v, ment); break;
+        case 4: dsig = commit_group<4, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v, ment); break;
+        case 8: dsig = commit_group<8, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v, ment); break;
+        case 16: dsig = commit_group<16, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v, ment); break;
+        case 32: dsig = commit_group<32, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v, ment); break;
+        default: dsig = commit_group<64, WEIGHTED, CM>(a, sl.list[q], sl.count[q], wb, v, ment); break;
     }
     if (a.gid) wave_keyed_add(a.g_dsig, v != LV_EMPTY ? a.gid[v] : 0u, (uint64_t)dsig, dsig != 0);
-    block_add_stats(a.stats, ST_DSIGMA, (uint64_t)dsig, -1, 0, -1, 0);
+    block_add_stats(a.stats, ST_DSIGMA, (uint64_t)dsig, ST_MOVER_ENTRIES, ment, -1, 0);
 }
 
 // Large rows (bucket 7, hubs): most listed vertices stay after the
@@ -1355,6 +1359,7 @@ __global__ __launch_bounds__(kBlock) void k_commit_big(CommitCtx a, BigLists bl,
         total += nm[q] * kBigParts[q];
     }
     int64_t dsum = 0;
+    uint64_t ment = 0;
     for (uint32_t it = blockIdx.x; it < total; it += gridDim.x) {
         uint32_t q = 0, j = it;
         while (j >= nm[q] * kBigParts[q]) { j -= nm[q] * kBigParts[q]; ++q; }
@@ -1366,8 +1371,9 @@ __global__ __launch_bounds__(kBlock) void k_commit_big(CommitCtx a, BigLists bl,
                                                   kBlock * kBigParts[q]);
         if (a.gid) wave_keyed_add(a.g_dsig, a.gid[v], (uint64_t)dsig, dsig != 0);
         dsum += dsig;
+        if (part == 0 && threadIdx.x == 0) ment += a.row[v + 1] - a.row[v];
     }
-    block_add_stats(a.stats, ST_DSIGMA, (uint64_t)dsum, -1, 0, -1, 0);
+    block_add_stats(a.stats, ST_DSIGMA, (uint64_t)dsum, ST_MOVER_ENTRIES, ment, -1, 0);
 }
 
 // ----------------------------------------------------------- finalize pass
@@ -1625,7 +1631,7 @@ void radix_sort_pairs(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, si
 bool bounds_valid(const uint32_t *max_deg) {
     if (max_deg[kNumBuckets - 2] < 2) return false;
     for (int b = 0; b < kNumBuckets - 1; ++b) {
-        if (max_deg[b] > kBucketMaxDeg[b]) return false;
+        if (max_deg[b] > (b == 5 ? kWide5MaxDeg : kBucketMaxDeg[b])) return false;
         if (b > 0 && max_deg[b] < max_deg[b - 1]) return false;
     }
     return true;
@@ -1732,8 +1738,11 @@ void launch_move_fast(const MoveCtx &m, bool weighted, int gain, const uint32_t 
 }
 
 // The (G, T) instantiations behind kTileG.
+constexpr int kTableT5Wide = (int)table_slots(kWide5MaxDeg);
+static_assert(kTableT5Wide == 772 && kTableT5Wide <= kPackedMaxT, "wide bucket-5 table");
+
 void launch_table_tile(int b, int G, const MoveCtx &m, bool wtd, int gain, const uint32_t *L,
-                       uint32_t cnt, hipStream_t s) {
+                       uint32_t cnt, hipStream_t s, bool wide5) {
 #define LV_T(GG, TT) launch_move_table<GG, TT>(m, wtd, gain, L, cnt, s)
     switch (b) {
         case 0: if (G == 1) LV_T(1, kTableT[0]); else if (G == 4) LV_T(4, kTableT[0]); else LV_T(2, kTableT[0]); break;
@@ -1741,7 +1750,13 @@ void launch_table_tile(int b, int G, const MoveCtx &m, bool wtd, int gain, const
         case 2: if (G == 4) LV_T(4, kTableT[2]); else if (G == 16) LV_T(16, kTableT[2]); else LV_T(8, kTableT[2]); break;
         case 3: if (G == 8) LV_T(8, kTableT[3]); else if (G == 32) LV_T(32, kTableT[3]); else LV_T(16, kTableT[3]); break;
         case 4: if (G == 16) LV_T(16, kTableT[4]); else if (G == 64) LV_T(64, kTableT[4]); else LV_T(32, kTableT[4]); break;
-        case 5: if (G == 32) LV_T(32, kTableT[5]); else if (G == 256) LV_T(256, kTableT[5]); else LV_T(64, kTableT[5]); break;
+        case 5:
+            if (wide5) {
+                if (G == 32) LV_T(32, kTableT5Wide); else if (G == 256) LV_T(256, kTableT5Wide); else LV_T(64, kTableT5Wide);
+            } else {
+                if (G == 32) LV_T(32, kTableT[5]); else if (G == 256) LV_T(256, kTableT[5]); else LV_T(64, kTableT[5]);
+            }
+            break;
         case 6: LV_T(256, kTableT[6]); break;
         default: LV_T(256, kTableT[7]); break;
     }
@@ -1787,7 +1802,7 @@ void move_bucket(const SweepArgs &a, int b, int sel, const uint32_t *list, uint3
     m.stats = a.bstats + (size_t)b * kNumStats * kStatSlots;
     const int G = kTileG[b][sel];
     if (a.kind == 0) launch_fast_tile(G, m, a.g.wt != nullptr, a.gain, list, count, s);
-    else launch_table_tile(b, G, m, a.g.wt != nullptr, a.gain, list, count, s);
+    else launch_table_tile(b, G, m, a.g.wt != nullptr, a.gain, list, count, s, a.wide5);
 }
 
 void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
@@ -1848,7 +1863,8 @@ void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
         } else {
             switch (b) {
                 case 0: case 1: case 2: case 3: case 4: case 5: case 6: case 7:
-                    launch_table_tile(b, kTileG[b][a.tiles.sel[b][0]], m, wtd, a.gain, L, cnt, s);
+                    launch_table_tile(b, kTileG[b][a.tiles.sel[b][0]], m, wtd, a.gain, L, cnt, s,
+                                      a.wide5);
                     break;
                 default: {
                     HubCtx h{m, hub};

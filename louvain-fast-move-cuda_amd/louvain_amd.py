@@ -47,13 +47,15 @@ class SweepRec(C.Structure):
                 ("E", C.c_uint64), ("e_active", C.c_uint64), ("sigma_in", C.c_int64),
                 ("sum_tot_sq", C.c_uint64), ("q", C.c_double), ("comm_hash", C.c_uint64),
                 ("move_ms", C.c_double), ("sweep_ms", C.c_double), ("candidates", C.c_uint64),
-                ("move_bytes", C.c_uint64)]
+                ("move_bytes", C.c_uint64), ("commit_ms", C.c_double),
+                ("commit_bytes", C.c_uint64), ("mover_entries", C.c_uint64)]
 
 
 class LevelRec(C.Structure):
     _fields_ = [("level", C.c_int32), ("sweeps", C.c_int32), ("n", C.c_uint32),
                 ("n_comm", C.c_uint32), ("E", C.c_uint64), ("q_start", C.c_double),
-                ("q_end", C.c_double), ("best_hash", C.c_uint64)]
+                ("q_end", C.c_double), ("best_hash", C.c_uint64), ("contract_ms", C.c_double),
+                ("E_new", C.c_uint64), ("contract_bytes", C.c_uint64)]
 
 
 class Result(C.Structure):
@@ -72,7 +74,11 @@ class KernelStats(C.Structure):
                 ("variant_launches", C.c_uint64 * 96), ("variant_ms", C.c_double * 96),
                 ("variant_bytes", C.c_uint64 * 96), ("variant_entries", C.c_uint64 * 96),
                 ("bucket_launches", C.c_uint64 * 18), ("bucket_ms", C.c_double * 18),
-                ("bucket_bytes", C.c_uint64 * 18), ("bucket_entries", C.c_uint64 * 18)]
+                ("bucket_bytes", C.c_uint64 * 18), ("bucket_entries", C.c_uint64 * 18),
+                ("commit_passes", C.c_uint64), ("commit_ms", C.c_double),
+                ("commit_bytes", C.c_uint64), ("commit_entries", C.c_uint64),
+                ("contractions", C.c_uint64), ("contract_ms", C.c_double),
+                ("contract_bytes", C.c_uint64), ("contract_entries", C.c_uint64)]
 
 
 class BatchResult(C.Structure):

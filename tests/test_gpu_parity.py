@@ -564,7 +564,8 @@ def test_refine_matches_oracle(fixture_path, mode, gain):
 
 
 BOUND_SETS = [[2, 6, 12, 24, 48, 128, 512, 2048], [4, 8, 16, 32, 64, 256, 1024, 1024],
-              [1, 3, 9, 27, 64, 200, 1000, 4000], [3, 6, 12, 24, 48, 192, 768, 3072]]
+              [1, 3, 9, 27, 64, 200, 1000, 4000], [3, 6, 12, 24, 48, 192, 768, 3072],
+              [4, 8, 16, 32, 64, 512, 1024, 4096], [2, 6, 12, 24, 48, 300, 300, 4000]]
 
 
 def test_perturbed_bucket_bounds_equal_oracle(fixture_path):
@@ -593,7 +594,8 @@ def test_perturbed_bucket_bounds_equal_oracle(fixture_path):
                            hash_trace=True)
             compare_runs(a, ofg.louvain(O.params(gain=gain[2], tie=tie[2])), label)
     for bad in ([5, 8, 16, 32, 64, 256, 1024, 4096], [4, 8, 16, 32, 64, 256, 1024, 1],
-                [4, 8, 16, 8, 64, 256, 1024, 4096], [4, 8, 16, 32, 64, 256, 1024, 5000]):
+                [4, 8, 16, 8, 64, 256, 1024, 4096], [4, 8, 16, 32, 64, 256, 1024, 5000],
+                [4, 8, 16, 32, 64, 513, 1024, 4096]):
         with pytest.raises(L.LouvainError) as e:
             fg.louvain(L.params(bucket_max_deg=bad))
         assert e.value.status == 1                    # LV_ERR_ARG
@@ -613,3 +615,38 @@ def test_bucket_bound_tuner_keeps_results():
     say(f"tuned bounds: {c.buckets()}")
     assert c.buckets() in ([4, 8, 16, 32, 64, 256, 1024, 4096], [3, 6, 12, 24, 48, 192, 768, 3072],
                            [2, 4, 8, 16, 32, 128, 512, 2048])
+
+
+def test_commit_and_contraction_records():
+    """The commit's count of mover entries and the per-sweep / per-level
+    measurement records.  The first sweep runs at the identity partition with
+    every vertex listed, so its movers are the vertices whose oracle fast-sweep
+    target is not themselves, and its mover entries their row lengths (hubs
+    of 7,000 entries go through the large-row commit)."""
+    rng = np.random.default_rng(31)
+    ea, eb = _random_graph(rng, n=15000, m=60000, hubs=2, hub_deg=7000)
+    g = L.Graph.from_edges(ea, eb)
+    og = O.Graph.from_edges(ea, eb)
+    n = g.n
+    row = g.csr()[0].astype(np.int64)
+    to, _, _, _ = og.best_moves(np.arange(n, dtype=np.uint32), np.ones(n, np.uint8),
+                                O.SWEEP_FAST, O.params())
+    movers = to != np.arange(n, dtype=np.uint32)
+    want = int(np.diff(row)[movers].sum())
+    c = L.Community(g)
+    r = c.run(L.params(profile=1), best=False, n0=n)
+    s0 = r["sweeps"][0]
+    say(f"sweep 0: movers {int(movers.sum())}, mover entries {s0['mover_entries']} (want {want})")
+    assert s0["n_moved"] == int(movers.sum())
+    assert s0["mover_entries"] == want
+    for s in r["sweeps"]:
+        assert s["mover_entries"] <= s["E"]
+        assert (s["mover_entries"] > 0) == (s["n_moved"] > 0)
+        assert s["commit_ms"] > 0 and s["commit_bytes"] > 0
+    lv = r["level_recs"]
+    for a, b in zip(lv[:-1], lv[1:]):
+        assert a["E_new"] == b["E"] and a["contract_ms"] > 0 and a["contract_bytes"] > 0
+    assert lv[-1]["contract_ms"] == 0 and lv[-1]["E_new"] == 0
+    k = c.kernel_stats()
+    assert k["contractions"] == len(lv) - 1 and k["commit_passes"] == len(r["sweeps"])
+    assert k["commit_entries"] == sum(s["mover_entries"] for s in r["sweeps"])
