@@ -74,8 +74,9 @@ typedef struct lv_params {
     uint32_t bucket_max_deg[8]; /* degree bounds of the best-move kernels' LDS buckets 0-7
                                    (rows above bucket_max_deg[7] take the hub path):
                                    non-decreasing, each at most the default
-                                   {4, 8, 16, 32, 64, 256, 1024, 4096}, the last at least 2
-                                   (else LV_ERR_ARG).  All zero: the default, or with
+                                   {4, 8, 16, 32, 64, 256, 1024, 4096} except bucket 5's,
+                                   which may reach 512 (its tables then take 772 slots),
+                                   the last at least 2 (else LV_ERR_ARG).  All zero: the default, or with
                                    autotune = 1 the fastest of three sets, timed once per
                                    community on its level-0 fast sweep.  Results do not
                                    depend on the bounds.  Not for the server (default). */
