@@ -77,15 +77,16 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
     uint32_t *K = s_key + gi * T;
     uint32_t *V = s_val + gi * T;
     uint64_t k0 = 0;
-    uint32_t lg = 1;
+    uint32_t tv = 4u;
     // the community's words together: row bounds and member range
     const uint32_t m0 = valid ? a.mstart[c] : 0u;
     const uint32_t m1 = valid ? a.mstart[c + 1] : 0u;
     if (valid) {
         k0 = a.crow[c];
-        lg = table_log2(a.crow[c + 1] - k0);   // at most 2x the virtual entries, <= T
+        // at most two thirds full with an empty slot (the virtual entries
+        // bound the distinct destinations); <= T for the bucket's entries
+        tv = table_slots(a.crow[c + 1] - k0);
     }
-    const uint32_t tv = 1u << lg, mask = tv - 1u;
     for (uint32_t s = lane; s < tv; s += G) { K[s] = LV_EMPTY; V[s] = 0u; }
     if (lane == 0) { s_cnt[gi] = 0u; s_pos[gi] = 0u; }
     // a community's table is its group's alone
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
             uint32_t hs[kAggIlp], pv[kAggIlp];
 #pragma unroll
             for (int u = 0; u < kAggIlp; ++u) {
-                hs[u] = slot_hash_dyn(ds[u], lg);
+                hs[u] = slot_hash_mod(ds[u], tv);
                 pv[u] = ds[u] != LV_EMPTY ? atomicCAS(&K[hs[u]], LV_EMPTY, ds[u]) : LV_EMPTY;
             }
 #pragma unroll
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
                 if (d == LV_EMPTY) continue;
                 uint32_t h = hs[u], prev = pv[u];
                 while (prev != LV_EMPTY && prev != d) {
-                    h = (h + 1) & mask;
+                    h = h + 1 == tv ? 0u : h + 1;
                     prev = atomicCAS(&K[h], LV_EMPTY, d);
                 }
                 if (prev == LV_EMPTY) atomicAdd(&s_cnt[gi], 1u);
@@ -546,15 +547,17 @@ void agg_tables(const AggArgs &x, uint32_t *const *lists, const uint32_t *counts
         if (wtd) hipLaunchKernelGGL((k_agg_table<GG, TT, true>), grid, dim3(kBlock), 0, s, a, L, cnt); \
         else hipLaunchKernelGGL((k_agg_table<GG, TT, false>), grid, dim3(kBlock), 0, s, a, L, cnt); \
     } while (0)
+        // tables of table_slots(the bucket's largest community), the slots
+        // of the best-move tables of the same degree bound
         switch (b) {
-            case 0: LV_AGG(2, 8); break;
-            case 1: LV_AGG(4, 16); break;
-            case 2: LV_AGG(8, 32); break;
-            case 3: LV_AGG(16, 64); break;
-            case 4: LV_AGG(32, 128); break;
-            case 5: LV_AGG(64, 512); break;
-            case 6: LV_AGG(256, 2048); break;
-            default: LV_AGG(256, 8192); break;
+            case 0: LV_AGG(2, kTableT[0]); break;
+            case 1: LV_AGG(4, kTableT[1]); break;
+            case 2: LV_AGG(8, kTableT[2]); break;
+            case 3: LV_AGG(16, kTableT[3]); break;
+            case 4: LV_AGG(32, kTableT[4]); break;
+            case 5: LV_AGG(64, kTableT[5]); break;
+            case 6: LV_AGG(256, kTableT[6]); break;
+            default: LV_AGG(256, kTableT[7]); break;
         }
 #undef LV_AGG
     }

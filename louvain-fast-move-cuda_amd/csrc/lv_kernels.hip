@@ -532,6 +532,7 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
     using S = typename SC::T;
     constexpr int GPB = kBlock / G;
     constexpr bool PERSIST = G == kBlock;
+    constexpr int SI = PERSIST && PK ? 2 : kIlp;
     __shared__ uint32_t s_key[GPB * T];
     __shared__ uint32_t s_val[GPB * T];
     __shared__ uint32_t s_tot[PK ? GPB * T : 1];
@@ -545,7 +546,7 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
     uint32_t *TT = s_tot + (PK ? gi * T : 0);
     uint32_t idx = PERSIST ? blockIdx.x : blockIdx.x * GPB + gi;
     MoveWords nw = move_words<PK>(a, idx < count ? list[idx] : 0u, idx < count);
-    uint64_t acc_self = 0, acc_cand = 0, acc_cons = 0, acc_tw = 0;
+    uint32_t acc_self = 0, acc_cand = 0, acc_cons = 0, acc_tw = 0;
     for (;;) {
         const MoveWords cw = nw;
         const uint32_t v = cw.v, own = cw.own, kv = cw.kv, tot_own = cw.tot_own;
@@ -560,7 +561,7 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
         // a vertex's table is its group's alone: groups inside one wave wait
         // for no other wave of the workgroup
         group_barrier<G>();
-        uint64_t nself = 0;
+        uint32_t nself = 0;
         if (live) {
             // kIlp entries per lane in flight: loads first, then gathers, then inserts
             for (uint64_t jb = beg + lane; jb < end; jb += (uint64_t)G * kIlp) {
@@ -620,23 +621,23 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
                 if (c == LV_EMPTY) break;
                 h = h + 1 == tv ? 0u : h + 1;
             }
-            for (uint32_t sb = lane; sb < tv; sb += (uint32_t)G * kIlp) {
-                uint32_t cs[kIlp], Ws[kIlp], ts[kIlp];
+            for (uint32_t sb = lane; sb < tv; sb += (uint32_t)G * SI) {
+                uint32_t cs[SI], Ws[SI], ts[SI];
 #pragma unroll
-                for (int u = 0; u < kIlp; ++u) {
+                for (int u = 0; u < SI; ++u) {
                     const uint32_t s2 = sb + (uint32_t)u * G;
                     const uint32_t c = s2 < tv ? K[s2] : LV_EMPTY;
                     cs[u] = c == own ? LV_EMPTY : c;
                     Ws[u] = cs[u] != LV_EMPTY ? V[s2] : 0u;
                 }
 #pragma unroll
-                for (int u = 0; u < kIlp; ++u) {
+                for (int u = 0; u < SI; ++u) {
                     const uint32_t s2 = sb + (uint32_t)u * G;
                     if constexpr (PK) ts[u] = cs[u] != LV_EMPTY ? TT[s2] : 0u;
                     else ts[u] = cs[u] != LV_EMPTY ? a.tot[cs[u]] : 0u;
                 }
 #pragma unroll
-                for (int u = 0; u < kIlp; ++u) {
+                for (int u = 0; u < SI; ++u) {
                     if (cs[u] == LV_EMPTY) continue;
                     ++acc_cand;
                     const S sc = SC::of(Ws[u], self, kv, tot_own, ts[u], cw.m2, cw.tw);
