@@ -41,22 +41,123 @@ __device__ inline int64_t shfl_xor_s<int64_t>(int64_t v, int off) { return shfl_
 template <>
 __device__ inline float shfl_xor_s<float>(float v, int off) { return shfl_xor_f32(v, off); }
 
-// Butterfly reduction of (score, community) inside aligned groups of G lanes
-// (G <= 64); every lane of the group ends with the group's best.
+// ---- reductions inside aligned groups of G <= 64 lanes ----
+//
+// Inside a row of 16 lanes the partner of a butterfly step comes by DPP (a
+// VALU operand modifier, no LDS round trip): quad permutations for the
+// partners at distance 1 and 2, then the half-row and row mirrors (lane i with
+// 7 - i, then with 15 - i), which pair every lane with one of the other half
+// as a butterfly must.  After the four steps every lane of a row holds the
+// row's result; rows are combined through scalar reads of lanes 0, 16, 32, 48.
+// The operations reduced are commutative and associative (sums, the best of a
+// total order), so the pairing order does not change the result.  Every lane
+// of the wave must be active.
+constexpr int kDppXor1 = 0xB1;          // quad_perm(1, 0, 3, 2)
+constexpr int kDppXor2 = 0x4E;          // quad_perm(2, 3, 0, 1)
+constexpr int kDppHalfMirror = 0x141;   // row_half_mirror
+constexpr int kDppMirror = 0x140;       // row_mirror
+
+template <int CTRL>
+__device__ inline uint32_t dpp_u32(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, CTRL, 0xF, 0xF, false);
+}
+
+template <int CTRL>
+__device__ inline uint64_t dpp_u64(uint64_t x) {
+    return (uint64_t)dpp_u32<CTRL>((uint32_t)x) |
+           ((uint64_t)dpp_u32<CTRL>((uint32_t)(x >> 32)) << 32);
+}
+
+__device__ inline uint32_t lane_u32(uint32_t x, int lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, lane);
+}
+
+__device__ inline uint64_t lane_u64(uint64_t x, int lane) {
+    return (uint64_t)lane_u32((uint32_t)x, lane) | ((uint64_t)lane_u32((uint32_t)(x >> 32), lane) << 32);
+}
+
+// score words: int64 (exact gains) or f32 (the reference's rounding)
+template <typename S>
+__device__ inline uint64_t s_bits(S s) {
+    if constexpr (sizeof(S) == 8) return (uint64_t)s;
+    else return (uint64_t)__float_as_uint(s);
+}
+template <typename S>
+__device__ inline S s_from(uint64_t b) {
+    if constexpr (sizeof(S) == 8) return (S)b;
+    else return __uint_as_float((uint32_t)b);
+}
+
+template <int CTRL, typename S>
+__device__ inline void best_step(S &s, uint32_t &c, bool highest) {
+    S os;
+    if constexpr (sizeof(S) == 8) os = s_from<S>(dpp_u64<CTRL>(s_bits(s)));
+    else os = s_from<S>(dpp_u32<CTRL>((uint32_t)s_bits(s)));
+    const uint32_t oc = dpp_u32<CTRL>(c);
+    if (better_candidate(os, oc, s, c, highest)) { s = os; c = oc; }
+}
+
+template <typename S>
+__device__ inline void best_of_lane(S &s, uint32_t &c, const S &s0, uint32_t c0, int lane,
+                                    bool highest) {
+    S os;
+    if constexpr (sizeof(S) == 8) os = s_from<S>(lane_u64(s_bits(s0), lane));
+    else os = s_from<S>(lane_u32((uint32_t)s_bits(s0), lane));
+    const uint32_t oc = lane_u32(c0, lane);
+    if (better_candidate(os, oc, s, c, highest)) { s = os; c = oc; }
+}
+
+// (score, community) best of each aligned group of G lanes (G <= 64); every
+// lane of the group ends with the group's best.
 template <int G, typename S>
 __device__ inline void group_best(S &s, uint32_t &c, bool highest) {
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) {
-        const S os = shfl_xor_s<S>(s, off);
-        const uint32_t oc = (uint32_t)__shfl_xor((int)c, off);
-        if (better_candidate(os, oc, s, c, highest)) { s = os; c = oc; }
+    static_assert(G >= 1 && G <= 64 && (G & (G - 1)) == 0, "groups of 1-64 lanes");
+    if constexpr (G >= 2) best_step<kDppXor1>(s, c, highest);
+    if constexpr (G >= 4) best_step<kDppXor2>(s, c, highest);
+    if constexpr (G >= 8) best_step<kDppHalfMirror>(s, c, highest);
+    if constexpr (G >= 16) best_step<kDppMirror>(s, c, highest);
+    if constexpr (G >= 32) {
+        const S rs = s;
+        const uint32_t rc = c;
+        const bool up = (threadIdx.x & 32) != 0;
+        // G == 32: rows 0-1 for the lower half, rows 2-3 for the upper
+        S a = s_from<S>(0), b = s_from<S>(0);
+        uint32_t ac = 0, bc = 0;
+        if constexpr (sizeof(S) == 8) {
+            a = s_from<S>(lane_u64(s_bits(rs), 0)); b = s_from<S>(lane_u64(s_bits(rs), 32));
+        } else {
+            a = s_from<S>(lane_u32((uint32_t)s_bits(rs), 0));
+            b = s_from<S>(lane_u32((uint32_t)s_bits(rs), 32));
+        }
+        ac = lane_u32(rc, 0);
+        bc = lane_u32(rc, 32);
+        best_of_lane(a, ac, rs, rc, 16, highest);
+        best_of_lane(b, bc, rs, rc, 48, highest);
+        if constexpr (G == 64) {
+            if (better_candidate(b, bc, a, ac, highest)) { a = b; ac = bc; }
+            s = a;
+            c = ac;
+        } else {
+            s = up ? b : a;
+            c = up ? bc : ac;
+        }
     }
 }
 
+// Sum of each aligned group of G lanes (G <= 64), in every lane of the group.
 template <int G>
 __device__ inline uint64_t group_sum_u64(uint64_t x) {
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) x += shfl_xor_u64(x, off);
+    static_assert(G >= 1 && G <= 64 && (G & (G - 1)) == 0, "groups of 1-64 lanes");
+    if constexpr (G >= 2) x += dpp_u64<kDppXor1>(x);
+    if constexpr (G >= 4) x += dpp_u64<kDppXor2>(x);
+    if constexpr (G >= 8) x += dpp_u64<kDppHalfMirror>(x);
+    if constexpr (G >= 16) x += dpp_u64<kDppMirror>(x);
+    if constexpr (G >= 32) {
+        const uint64_t lo = lane_u64(x, 0) + lane_u64(x, 16);
+        const uint64_t hi = lane_u64(x, 32) + lane_u64(x, 48);
+        if constexpr (G == 64) x = lo + hi;
+        else x = (threadIdx.x & 32) ? hi : lo;
+    }
     return x;
 }
 
