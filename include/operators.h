@@ -24,7 +24,10 @@ __host__ __device__ inline uint32_t count_not_zero(uint32_t tot) { return tot > 
  * better than the incumbent (bs, bc). */
 template <typename S>
 __host__ __device__ inline bool better_candidate(S s, uint32_t c, S bs, uint32_t bc, bool highest) {
-    return s > bs || (s == bs && (highest ? c > bc : c < bc));
+    // both comparisons evaluated (bitwise, no short circuit): on the GPU a
+    // short-circuit form becomes branches and execution-mask bookkeeping
+    const bool tie = highest ? c > bc : c < bc;
+    return (s > bs) | ((s == bs) & tie);
 }
 
 /* Square (operators.h:62-70), exact in 64 bits for totals < 2^31. */

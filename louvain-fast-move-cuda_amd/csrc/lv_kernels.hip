@@ -526,13 +526,14 @@ __device__ inline MoveWords move_words(const MoveCtx &a, uint32_t v, bool valid)
 // are reduced once per workgroup.  The grid is a few rounds of the
 // workgroups the device holds at once (kPersistRounds).
 template <int G, int T, int GAIN, bool WEIGHTED, bool PK>
-__global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t *__restrict__ list,
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 && !WEIGHTED ? 8 : 1))) void k_move_table(MoveCtx a, const uint32_t *__restrict__ list,
                                                        uint32_t count) {
     using SC = Score<GAIN>;
     using S = typename SC::T;
     constexpr int GPB = kBlock / G;
     constexpr bool PERSIST = G == kBlock;
-    constexpr int SI = PERSIST && PK ? 2 : kIlp;
+    constexpr int SI = PERSIST && PK ? 1 : kIlp;
     __shared__ uint32_t s_key[GPB * T];
     __shared__ uint32_t s_val[GPB * T];
     __shared__ uint32_t s_tot[PK ? GPB * T : 1];
@@ -547,6 +548,17 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
     uint32_t idx = PERSIST ? blockIdx.x : blockIdx.x * GPB + gi;
     MoveWords nw = move_words<PK>(a, idx < count ? list[idx] : 0u, idx < count);
     uint32_t acc_self = 0, acc_cand = 0, acc_cons = 0, acc_tw = 0;
+    // persistent: the first kIlp columns per lane of the next vertex's row,
+    // loaded while this vertex is scored (its weights load beside the gathers)
+    uint32_t px[kIlp];
+    auto first_loads = [&](const MoveWords &w) {
+#pragma unroll
+        for (int u = 0; u < kIlp; ++u) {
+            const uint64_t j = w.beg + (uint64_t)lane + (uint64_t)u * G;
+            px[u] = w.live && j < w.end ? stream_load(&a.col[j]) : w.v;
+        }
+    };
+    if constexpr (PERSIST) first_loads(nw);
     for (;;) {
         const MoveWords cw = nw;
         const uint32_t v = cw.v, own = cw.own, kv = cw.kv, tot_own = cw.tot_own;
@@ -566,20 +578,46 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
             // kIlp entries per lane in flight: loads first, then gathers, then inserts
             for (uint64_t jb = beg + lane; jb < end; jb += (uint64_t)G * kIlp) {
                 uint32_t xs[kIlp], ws[kIlp], cs[kIlp], ts[kIlp];
+                if (PERSIST && jb == beg + lane) {
 #pragma unroll
-                for (int u = 0; u < kIlp; ++u) {
-                    const uint64_t j = jb + (uint64_t)u * G;
-                    xs[u] = j < end ? stream_load(&a.col[j]) : v;
-                    ws[u] = (WEIGHTED && j < end) ? stream_load(&a.wt[j]) : 1u;
+                    for (int u = 0; u < kIlp; ++u) {
+                        const uint64_t j = jb + (uint64_t)u * G;
+                        xs[u] = px[u];
+                        if constexpr (WEIGHTED) ws[u] = stream_load(&a.wt[j < end ? j : end - 1]);
+                        else ws[u] = 1u;
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < kIlp; ++u) {
+                        const uint64_t j = jb + (uint64_t)u * G;
+                        if constexpr (G >= 64) {
+                            // past the row's end a lane loads the row's last
+                            // entry (no branch; the lanes of a group share that
+                            // address) and takes the vertex itself, a skipped entry
+                            const uint64_t jc = j < end ? j : end - 1;
+                            const uint32_t x = stream_load(&a.col[jc]);
+                            xs[u] = j < end ? x : v;
+                            if constexpr (WEIGHTED) ws[u] = stream_load(&a.wt[jc]);
+                            else ws[u] = 1u;
+                        } else {
+                            xs[u] = j < end ? stream_load(&a.col[j]) : v;
+                            ws[u] = (WEIGHTED && j < end) ? stream_load(&a.wt[j]) : 1u;
+                        }
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < kIlp; ++u) {
+                    // groups of a wave or more: every xs[u] is a vertex id and
+                    // the words are gathered without a branch (those of
+                    // skipped entries are not used; the group's skipped lanes
+                    // share one address)
+                    const bool g = G >= 64 || xs[u] != v;
                     if constexpr (PK) {
-                        const unsigned long long p = xs[u] != v ? a.ctot[xs[u]] : 0ull;
+                        const unsigned long long p = g ? a.ctot[xs[u]] : 0ull;
                         cs[u] = (uint32_t)p;
                         ts[u] = (uint32_t)(p >> 32);
                     } else {
-                        cs[u] = xs[u] != v ? a.comm[xs[u]] : 0u;
+                        cs[u] = g ? a.comm[xs[u]] : 0u;
                         ts[u] = 0u;
                     }
                 }
@@ -610,6 +648,7 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
         }
         if constexpr (PERSIST) nw = move_words<PK>(a, nv, nvalid);
         group_barrier<G>();
+        if constexpr (PERSIST) first_loads(nw);
         S best = SC::none();
         uint32_t best_c = a.highest ? 0u : LV_EMPTY;
         uint32_t self = 0;
@@ -621,27 +660,32 @@ __global__ __launch_bounds__(kBlock) void k_move_table(MoveCtx a, const uint32_t
                 if (c == LV_EMPTY) break;
                 h = h + 1 == tv ? 0u : h + 1;
             }
+            // branch free: slots past the table read slot 0, and a slot that
+            // holds no other community is scored but never taken
             for (uint32_t sb = lane; sb < tv; sb += (uint32_t)G * SI) {
                 uint32_t cs[SI], Ws[SI], ts[SI];
+                bool ok[SI];
 #pragma unroll
                 for (int u = 0; u < SI; ++u) {
                     const uint32_t s2 = sb + (uint32_t)u * G;
-                    const uint32_t c = s2 < tv ? K[s2] : LV_EMPTY;
-                    cs[u] = c == own ? LV_EMPTY : c;
-                    Ws[u] = cs[u] != LV_EMPTY ? V[s2] : 0u;
+                    const uint32_t si = s2 < tv ? s2 : 0u;
+                    cs[u] = K[si];
+                    Ws[u] = V[si];
+                    ok[u] = (s2 < tv) & (cs[u] != own) & (cs[u] != LV_EMPTY);
                 }
 #pragma unroll
                 for (int u = 0; u < SI; ++u) {
                     const uint32_t s2 = sb + (uint32_t)u * G;
-                    if constexpr (PK) ts[u] = cs[u] != LV_EMPTY ? TT[s2] : 0u;
-                    else ts[u] = cs[u] != LV_EMPTY ? a.tot[cs[u]] : 0u;
+                    if constexpr (PK) ts[u] = TT[s2 < tv ? s2 : 0u];
+                    else ts[u] = ok[u] ? a.tot[cs[u]] : 0u;
                 }
 #pragma unroll
                 for (int u = 0; u < SI; ++u) {
-                    if (cs[u] == LV_EMPTY) continue;
-                    ++acc_cand;
+                    acc_cand += ok[u];
                     const S sc = SC::of(Ws[u], self, kv, tot_own, ts[u], cw.m2, cw.tw);
-                    if (better_candidate(sc, cs[u], best, best_c, a.highest)) { best = sc; best_c = cs[u]; }
+                    const bool b = ok[u] & better_candidate(sc, cs[u], best, best_c, a.highest);
+                    best = b ? sc : best;
+                    best_c = b ? cs[u] : best_c;
                 }
             }
         }
