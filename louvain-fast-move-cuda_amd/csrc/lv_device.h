@@ -161,27 +161,60 @@ __device__ inline uint64_t group_sum_u64(uint64_t x) {
     return x;
 }
 
+// Sum of each aligned group of G lanes (G <= 64) of 32-bit values, in every
+// lane of the group; for counts whose group total stays below 2^32.
+template <int G>
+__device__ inline uint32_t group_sum_u32(uint32_t x) {
+    static_assert(G >= 1 && G <= 64 && (G & (G - 1)) == 0, "groups of 1-64 lanes");
+    if constexpr (G >= 2) x += dpp_u32<kDppXor1>(x);
+    if constexpr (G >= 4) x += dpp_u32<kDppXor2>(x);
+    if constexpr (G >= 8) x += dpp_u32<kDppHalfMirror>(x);
+    if constexpr (G >= 16) x += dpp_u32<kDppMirror>(x);
+    if constexpr (G >= 32) {
+        const uint32_t lo = lane_u32(x, 0) + lane_u32(x, 16);
+        const uint32_t hi = lane_u32(x, 32) + lane_u32(x, 48);
+        if constexpr (G == 64) x = lo + hi;
+        else x = (threadIdx.x & 32) ? hi : lo;
+    }
+    return x;
+}
+
+template <typename U>
+__device__ inline U wave_total(U x) {
+    if constexpr (sizeof(U) == 4) return group_sum_u32<64>(x);
+    else return group_sum_u64<64>(x);
+}
+
+template <typename U>
+struct same_type { using type = U; };
+
 // Block-wide sum of up to 4 counters; thread 0 adds them to the spread slots
-// (a negative stat index skips a counter).
-__device__ inline void block_add_stats(unsigned long long *stats, int st0, uint64_t a0, int st1,
-                                       uint64_t a1, int st2, uint64_t a2, int st3 = -1,
-                                       uint64_t a3 = 0) {
+// (a negative stat index skips a counter).  32-bit counters (U = uint32_t,
+// wave totals below 2^32) take half the cross-lane steps of 64-bit ones.
+template <typename U>
+__device__ inline void block_add_stats(unsigned long long *stats, int st0, U a0, int st1,
+                                       typename same_type<U>::type a1, int st2,
+                                       typename same_type<U>::type a2, int st3 = -1,
+                                       typename same_type<U>::type a3 = 0) {
     __shared__ uint64_t red[4][kBlock / 64];
-    a0 = group_sum_u64<64>(a0);
-    a1 = group_sum_u64<64>(a1);
-    a2 = group_sum_u64<64>(a2);
-    a3 = group_sum_u64<64>(a3);
+    const int st[4] = {st0, st1, st2, st3};
+    const U a[4] = {a0, a1, a2, a3};
     const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { red[0][w] = a0; red[1][w] = a1; red[2][w] = a2; red[3][w] = a3; }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (st[k] < 0) continue;
+        const U t = wave_total(a[k]);
+        if ((threadIdx.x & 63) == 0) red[k][w] = (uint64_t)t;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint64_t t[4] = {0, 0, 0, 0};
-        for (int i = 0; i < kBlock / 64; ++i)
-            for (int k = 0; k < 4; ++k) t[k] += red[k][i];
-        const int st[4] = {st0, st1, st2, st3};
         const int slot = blockIdx.x & (kStatSlots - 1);
-        for (int k = 0; k < 4; ++k)
-            if (st[k] >= 0 && t[k]) atomicAdd(&stats[st[k] * kStatSlots + slot], (unsigned long long)t[k]);
+        for (int k = 0; k < 4; ++k) {
+            if (st[k] < 0) continue;
+            uint64_t t = 0;
+            for (int i = 0; i < kBlock / 64; ++i) t += red[k][i];
+            if (t) atomicAdd(&stats[st[k] * kStatSlots + slot], (unsigned long long)t);
+        }
     }
     __syncthreads();
 }

@@ -458,6 +458,19 @@ __global__ __launch_bounds__(kBlock) void k_identity_parts(LevelView g,
     block_add_stats(stats, ST_DSIGMA, sig, ST_SUMSQ, sq, -1, 0);
 }
 
+// Whether any lane of each aligned group of G lanes (G <= 64) has `x` set:
+// one ballot, no cross-lane arithmetic.
+template <int G>
+__device__ inline bool group_any(bool x) {
+    const unsigned long long b = __ballot(x);
+    if constexpr (G == 64) {
+        return b != 0;
+    } else {
+        const uint32_t base = (threadIdx.x & 63) & ~(uint32_t)(G - 1);
+        return ((b >> base) & ((1ull << G) - 1)) != 0;
+    }
+}
+
 // ---------------------------------------------------------- best-move pass
 
 struct MoveCtx {
@@ -527,7 +540,7 @@ __device__ inline MoveWords move_words(const MoveCtx &a, uint32_t v, bool valid)
 // workgroups the device holds at once (kPersistRounds).
 template <int G, int T, int GAIN, bool WEIGHTED, bool PK>
 __global__ __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 && !WEIGHTED ? 8 : 1))) void k_move_table(MoveCtx a, const uint32_t *__restrict__ list,
+__attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? (WEIGHTED ? 7 : 8) : 1))) void k_move_table(MoveCtx a, const uint32_t *__restrict__ list,
                                                        uint32_t count) {
     using SC = Score<GAIN>;
     using S = typename SC::T;
@@ -539,7 +552,7 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 && !WEIGHTED ?
     __shared__ uint32_t s_tot[PK ? GPB * T : 1];
     __shared__ S r_s[kBlock / 64];
     __shared__ uint32_t r_c[kBlock / 64];
-    __shared__ uint64_t r_n[kBlock / 64];
+    __shared__ uint32_t r_n[kBlock / 64];
     const int lane = threadIdx.x & (G - 1);
     const int gi = threadIdx.x / G;
     uint32_t *K = s_key + gi * T;
@@ -692,9 +705,9 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 && !WEIGHTED ?
         acc_self += nself;
         if constexpr (G <= 64) {
             group_best<G, S>(best, best_c, a.highest);
-            const uint64_t gself = group_sum_u64<G>(nself);
+            const bool gself = group_any<G>(nself > 0);
             if (live && lane == 0) {
-                acc_cons += gself > 0;
+                acc_cons += gself;
                 if (best != SC::none() && SC::positive(best, self, kv, tot_own, cw.m2)) {
                     a.target[v] = best_c;
                     ++acc_tw;
@@ -702,12 +715,12 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 && !WEIGHTED ?
             }
         } else {
             group_best<64, S>(best, best_c, a.highest);
-            const uint64_t wself = group_sum_u64<64>(nself);
+            const uint32_t wself = group_any<64>(nself > 0);
             const int w = threadIdx.x >> 6;
             if ((threadIdx.x & 63) == 0) { r_s[w] = best; r_c[w] = best_c; r_n[w] = wself; }
             __syncthreads();
             if (threadIdx.x == 0 && live) {
-                uint64_t tn = 0;
+                uint32_t tn = 0;
                 for (int i = 0; i < kBlock / 64; ++i) {
                     tn += r_n[i];
                     if (better_candidate(r_s[i], r_c[i], best, best_c, a.highest)) {
@@ -749,7 +762,7 @@ __global__ __launch_bounds__(kBlock) void k_move_fast(MoveCtx a, const uint32_t 
     const bool live = valid && v < a.v_end && vertex_graph(a.bt, v, a.m2, a.tw, m2, tw);
     S best = SC::none();
     uint32_t best_c = LV_EMPTY;
-    uint64_t nself = 0;
+    uint32_t nself = 0;
     uint32_t kv = 0, tot_own = 0;
     if (live) {
         kv = a.kdeg[v];
@@ -774,12 +787,12 @@ __global__ __launch_bounds__(kBlock) void k_move_fast(MoveCtx a, const uint32_t 
             }
         }
     }
-    uint64_t considered = 0, twrite = 0;
+    uint32_t considered = 0, twrite = 0;
     if constexpr (G <= 64) {
         group_best<G, S>(best, best_c, false);
-        const uint64_t gself = group_sum_u64<G>(nself);
+        const bool gself = group_any<G>(nself > 0);
         if (live && lane == 0) {
-            considered = gself > 0;
+            considered = gself;
             if (best != SC::none() && SC::positive(best, 0u, kv, tot_own, m2) && v < best_c) {
                 a.target[v] = best_c;
                 twrite = 1;
@@ -788,14 +801,14 @@ __global__ __launch_bounds__(kBlock) void k_move_fast(MoveCtx a, const uint32_t 
     } else {
         __shared__ S r_s[kBlock / 64];
         __shared__ uint32_t r_c[kBlock / 64];
-        __shared__ uint64_t r_n[kBlock / 64];
+        __shared__ uint32_t r_n[kBlock / 64];
         group_best<64, S>(best, best_c, false);
-        const uint64_t wself = group_sum_u64<64>(nself);
+        const uint32_t wself = group_any<64>(nself > 0);
         const int w = threadIdx.x >> 6;
         if ((threadIdx.x & 63) == 0) { r_s[w] = best; r_c[w] = best_c; r_n[w] = wself; }
         __syncthreads();
         if (threadIdx.x == 0 && live) {
-            uint64_t tn = 0;
+            uint32_t tn = 0;
             for (int i = 0; i < kBlock / 64; ++i) {
                 tn += r_n[i];
                 if (better_candidate(r_s[i], r_c[i], best, best_c, false)) {
