@@ -70,7 +70,9 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
     __shared__ uint32_t s_cnt[GPB];
     __shared__ uint32_t s_pos[GPB];
     const int lane = threadIdx.x & (G - 1);
-    const int gi = threadIdx.x / G;
+    // a group of a whole wave or more is wave uniform: said so, its words
+    // load through the scalar unit
+    const int gi = G >= 64 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / G)) : (int)(threadIdx.x / G);
     const uint32_t idx = blockIdx.x * GPB + gi;
     const bool valid = idx < count;
     const uint32_t c = valid ? list[idx] : 0u;
