@@ -42,9 +42,22 @@ struct AggCtx {
     uint32_t *rcount;          // [n_new] distinct destinations per community
 };
 
-// Largest i in [0, n) with p[i] <= k, for a staged LDS prefix (p[0] <= k).
-__device__ inline uint32_t lds_member(const uint64_t *p, uint32_t n, uint64_t k) {
-    uint32_t lo = 0, hi = n;
+// Largest i in [i0, n) with p[i] <= k, for a staged LDS prefix with
+// p[i0] <= k: a gallop forward from i0, then a binary search in the bracket.
+// A lane walks its entries in increasing order, so i0 (the member of its
+// previous entry) is at most a few members behind: one to three LDS reads
+// instead of log2(n).
+__device__ inline uint32_t lds_member_from(const uint64_t *p, uint32_t n, uint64_t k, uint32_t i0) {
+    uint32_t lo = i0, hi = n, step = 1;
+    while (lo + step < n) {
+        if (p[lo + step] <= k) {
+            lo += step;
+            step <<= 1;
+        } else {
+            hi = lo + step;
+            break;
+        }
+    }
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
         if (p[mid] <= k) lo = mid; else hi = mid;
@@ -70,9 +83,7 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
     __shared__ uint32_t s_cnt[GPB];
     __shared__ uint32_t s_pos[GPB];
     const int lane = threadIdx.x & (G - 1);
-    // a group of a whole wave or more is wave uniform: said so, its words
-    // load through the scalar unit
-    const int gi = G >= 64 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / G)) : (int)(threadIdx.x / G);
+    const int gi = (int)group_index<G>();
     const uint32_t idx = blockIdx.x * GPB + gi;
     const bool valid = idx < count;
     const uint32_t c = valid ? list[idx] : 0u;
@@ -106,6 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
         if (lane == 0) s_mp[gi][nm] = a.P[mb + nm];
         group_barrier<G>();
         const uint64_t e0 = s_mp[gi][0], e1 = s_mp[gi][nm];
+        uint32_t mi = 0;   // member of the lane's previous entry
         for (uint64_t kb = e0 + lane; kb < e1; kb += (uint64_t)G * kAggIlp) {
             uint32_t xs[kAggIlp], ws[kAggIlp], ds[kAggIlp];
 #pragma unroll
@@ -114,7 +126,8 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
                 xs[u] = LV_EMPTY;
                 ws[u] = 1u;
                 if (k < e1) {
-                    const uint32_t i = lds_member(s_mp[gi], nm, k);
+                    const uint32_t i = lds_member_from(s_mp[gi], nm, k, mi);
+                    mi = i;
                     const uint64_t j = s_mrow[gi][i] + (k - s_mp[gi][i]);
                     xs[u] = stream_load(&a.col[j]);
                     if (WEIGHTED) ws[u] = stream_load(&a.wt[j]);
@@ -229,6 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_giant_gather(GiantCtx h) {
         __syncthreads();
         const uint64_t e0 = s_mp[0] > s0 ? s_mp[0] : s0;
         const uint64_t e1 = s_mp[nm] < s1 ? s_mp[nm] : s1;
+        uint32_t mi = 0;   // member of the lane's previous entry
         for (uint64_t kb = e0 + threadIdx.x; kb < e1; kb += (uint64_t)kBlock * kGiantIlp) {
             uint32_t xs[kGiantIlp], ws[kGiantIlp], ds[kGiantIlp];
 #pragma unroll
@@ -237,7 +251,8 @@ __global__ __launch_bounds__(kBlock) void k_giant_gather(GiantCtx h) {
                 xs[u] = LV_EMPTY;
                 ws[u] = 1u;
                 if (k < e1) {
-                    const uint32_t i = lds_member(s_mp, nm, k);
+                    const uint32_t i = lds_member_from(s_mp, nm, k, mi);
+                    mi = i;
                     const uint64_t j = s_mrow[i] + (k - s_mp[i]);
                     xs[u] = stream_load(&a.col[j]);
                     if (WEIGHTED) ws[u] = stream_load(&a.wt[j]);

@@ -41,6 +41,16 @@ __device__ inline int64_t shfl_xor_s<int64_t>(int64_t v, int off) { return shfl_
 template <>
 __device__ inline float shfl_xor_s<float>(float v, int off) { return shfl_xor_f32(v, off); }
 
+// Index of a lane's group of G lanes in its workgroup.  A group of a whole
+// wave or more is wave uniform, which the compiler is told (readfirstlane):
+// the group's words then load through the scalar unit.
+template <int G>
+__device__ inline uint32_t group_index() {
+    const uint32_t g = threadIdx.x / (uint32_t)G;
+    if constexpr (G >= 64) return (uint32_t)__builtin_amdgcn_readfirstlane((int)g);
+    else return g;
+}
+
 // ---- reductions inside aligned groups of G <= 64 lanes ----
 //
 // Inside a row of 16 lanes the partner of a butterfly step comes by DPP (a

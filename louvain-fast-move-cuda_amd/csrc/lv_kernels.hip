@@ -554,9 +554,7 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? (WEIGHTED ? 
     __shared__ uint32_t r_c[kBlock / 64];
     __shared__ uint32_t r_n[kBlock / 64];
     const int lane = threadIdx.x & (G - 1);
-    // a group of a whole wave or more is wave uniform: said so, its words
-    // load through the scalar unit
-    const int gi = G >= 64 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / G)) : (int)(threadIdx.x / G);
+    const int gi = (int)group_index<G>();
     uint32_t *K = s_key + gi * T;
     uint32_t *V = s_val + gi * T;
     uint32_t *TT = s_tot + (PK ? gi * T : 0);
@@ -755,9 +753,7 @@ __global__ __launch_bounds__(kBlock) void k_move_fast(MoveCtx a, const uint32_t 
     using S = typename SC::T;
     constexpr int GPB = kBlock / G;
     const int lane = threadIdx.x & (G - 1);
-    // a group of a whole wave or more is wave uniform: said so, its words
-    // load through the scalar unit
-    const int gi = G >= 64 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / G)) : (int)(threadIdx.x / G);
+    const int gi = (int)group_index<G>();
     const uint32_t idx = blockIdx.x * GPB + gi;
     const bool valid = idx < count;
     const uint32_t v = valid ? list[idx] : 0u;
@@ -1334,7 +1330,7 @@ __device__ inline int64_t commit_group(const CommitCtx &a, const uint32_t *__res
                                        uint64_t &ment) {
     constexpr int GPB = kBlock / G;
     const int lane = threadIdx.x & (G - 1);
-    const uint32_t idx = wb * GPB + threadIdx.x / G;
+    const uint32_t idx = wb * GPB + group_index<G>();
     int64_t dsig = 0;
     if (idx < count) {
         v = list[idx];

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_gather(SortCtx a, const uint32_
         idx = blockIdx.x / parts;
         part = blockIdx.x % parts;
     } else {
-        idx = blockIdx.x * GPB + threadIdx.x / G;
+        idx = blockIdx.x * GPB + group_index<G>();
         part = 0;
     }
     uint64_t nself = 0;
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_decide(SortCtx a, const uint32_
     using S = typename SC::T;
     constexpr int GPB = kBlock / G;
     const int lane = threadIdx.x & (G - 1);
-    const uint32_t idx = blockIdx.x * GPB + threadIdx.x / G;
+    const uint32_t idx = blockIdx.x * GPB + group_index<G>();
     const bool valid = idx < count;
     const uint32_t v = valid ? list[idx] : 0u;
     int64_t m2 = a.m2;
