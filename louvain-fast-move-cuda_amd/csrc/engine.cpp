@@ -824,6 +824,14 @@ static int bits_for(uint64_t x) {   // smallest b with 2^b >= x (x >= 1)
 }  // namespace lv
 
 namespace lv {
+// The main stream waits for the contraction's row copy (no host wait).
+void join_row_copy(Community &c) {
+    SweepState &st = *c.state;
+    if (!st.row_copy_pending) return;
+    LV_HIP(hipStreamWaitEvent(c.stream(), st.side_streams()->join[0], 0));
+    st.row_copy_pending = false;
+}
+
 // community_map = inclusive_scan(tot > 0) (phaseAggregation.cuh:27-43), then
 // best_communities composed with it (:45); returns the device map.
 uint32_t *renumber(Community &c, uint32_t &n_new) {
@@ -849,12 +857,16 @@ void AggregationPhase::run(Community &c, bool aggregate, MODE mode) {
     if (!aggregate) return;
     const int64_t sigma_in = c.sigma_in;
     const uint64_t sum_tot_sq = c.sum_tot_sq;
+    c.state->defer_copy_join = true;
     contract(c, map, n_new);
+    c.state->defer_copy_join = false;
     c.level += 1;
     // community.update() (:174): identity partition on the contracted graph.
     // Its modularity parts equal the ones just reached (self-loop weight of a
     // community = its internal weight, new degree = old total), so they carry.
+    // The update reads the new row offsets only: it runs beside the row copy.
     c.update(sigma_in, sum_tot_sq);
+    join_row_copy(c);
 }
 
 // Contraction by community hashing (phaseAggregation.cuh:51-170).
@@ -1001,13 +1013,23 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
     wait_stream(s);
     ng.edge_destination.ensure(std::max<size_t>(e_new, 1));
     ng.weights.ensure(std::max<size_t>(e_new, 1));
-    agg_copy(x, lp, cnt, orow, ng.edge_destination.data(), ng.weights.data(), s);
-    agg_giant_copy(x, ga, n_parts, orow, ng.edge_destination.data(), ng.weights.data(), s);
+    // the rows are copied on a side stream: the next level's setup (degrees,
+    // buckets, lists, hub tables and their host round trips) needs the new
+    // offsets only and runs beside the copy when the caller defers the join
+    const SideStreams *cs = c.state->side_streams();
+    LV_HIP(hipEventRecord(cs->fork, s));
+    LV_HIP(hipStreamWaitEvent(cs->s[0], cs->fork, 0));
+    agg_copy(x, lp, cnt, orow, ng.edge_destination.data(), ng.weights.data(), cs->s[0]);
+    agg_giant_copy(x, ga, n_parts, orow, ng.edge_destination.data(), ng.weights.data(), cs->s[0]);
+    LV_HIP(hipEventRecord(cs->join[0], cs->s[0]));
+    c.state->row_copy_pending = true;
     new_degrees(c.communities_weight.data(), map, n, ng.tot_weight_per_nodes.data(), s);
     ng.n_entries = e_new;
     ng.n_links = e_new / 2;                                           // :97
     c.graph = GraphDevice::view_of(ng);
+    if (!c.state->defer_copy_join) join_row_copy(c);
 }
+
 
 // ------------------------------------------------------------ RefinePhase
 

@@ -132,6 +132,12 @@ struct SweepState {
     // level l, held in the other one or in the resident level-0 graph, is read)
     GraphDevice level_buf[2];
     int next_level_buf = 0;
+    // the contraction's row copy runs on side stream 0; its join event is
+    // waited for by the main stream at the end of the contraction, or, when
+    // the caller defers it (AggregationPhase::run), after the next level's
+    // setup, which reads the new row offsets only (join_row_copy)
+    bool defer_copy_join = false;
+    bool row_copy_pending = false;
     // tiles of the best-move kernels (tuned once per community when asked)
     TileChoice tiles;
     bool tuned[2] = {false, false};   // [fast sweep]
