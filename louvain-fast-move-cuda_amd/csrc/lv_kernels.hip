@@ -1258,7 +1258,7 @@ constexpr uint32_t kMarkBitMaxN = 0x80000000u;   // the bit path needs every id 
 
 // The caller loads v's words (communities, row bounds, degree) together, so
 // their latencies overlap.
-template <bool WEIGHTED, int CM>
+template <bool WEIGHTED, int CM, int ILP = kIlp>
 __device__ inline int64_t commit_row(const CommitCtx &a, uint32_t v, uint32_t own, uint32_t c,
                                      uint64_t beg, uint64_t end, uint32_t kv, uint32_t first,
                                      uint32_t stride) {
@@ -1267,16 +1267,16 @@ __device__ inline int64_t commit_row(const CommitCtx &a, uint32_t v, uint32_t ow
         atomicSub(&a.tot[own], kv);
     }
     int64_t dsig = 0;
-    for (uint64_t jb = beg + first; jb < end; jb += (uint64_t)stride * kIlp) {
-        uint32_t xs[kIlp], ws[kIlp], ox[kIlp], nx[kIlp];
+    for (uint64_t jb = beg + first; jb < end; jb += (uint64_t)stride * ILP) {
+        uint32_t xs[ILP], ws[ILP], ox[ILP], nx[ILP];
 #pragma unroll
-        for (int u = 0; u < kIlp; ++u) {
+        for (int u = 0; u < ILP; ++u) {
             const uint64_t j = jb + (uint64_t)u * stride;
             xs[u] = j < end ? stream_load(&a.col[j]) : v;
             ws[u] = (WEIGHTED && j < end) ? stream_load(&a.wt[j]) : 1u;
         }
 #pragma unroll
-        for (int u = 0; u < kIlp; ++u) {
+        for (int u = 0; u < ILP; ++u) {
             if (CM == kCommitIdent) {
                 ox[u] = xs[u];
                 nx[u] = xs[u] != v ? a.target[xs[u]] : 0u;
@@ -1287,7 +1287,7 @@ __device__ inline int64_t commit_row(const CommitCtx &a, uint32_t v, uint32_t ow
             }
         }
 #pragma unroll
-        for (int u = 0; u < kIlp; ++u) {
+        for (int u = 0; u < ILP; ++u) {
             if (xs[u] == v) continue;
             const uint32_t hi = nx[u];
             if (CM == kCommitPackedBit) nx[u] &= 0x7FFFFFFFu;
@@ -1339,7 +1339,10 @@ __device__ inline int64_t commit_group(const CommitCtx &a, const uint32_t *__res
         const uint64_t beg = a.row[v], end = a.row[v + 1];
         const uint32_t kv = a.kdeg[v];
         if (c != own) {
-            dsig = commit_row<WEIGHTED, CM>(a, v, own, c, beg, end, kv, (uint32_t)lane, (uint32_t)G);
+            // rows of up to 16 entries per lane (groups of 16 and 64 lanes):
+            // eight entries per lane in flight, half the round trips of four
+            dsig = commit_row<WEIGHTED, CM, (G >= 16 ? 2 * kIlp : kIlp)>(a, v, own, c, beg, end, kv,
+                                                                       (uint32_t)lane, (uint32_t)G);
             if (lane == 0) ment = end - beg;
         }
     }
