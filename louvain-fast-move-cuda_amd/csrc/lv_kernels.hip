@@ -561,6 +561,10 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? (WEIGHTED ? 
     uint32_t idx = PERSIST ? blockIdx.x : blockIdx.x * GPB + gi;
     MoveWords nw = move_words<PK>(a, idx < count ? list[idx] : 0u, idx < count);
     uint32_t acc_self = 0, acc_cand = 0, acc_cons = 0, acc_tw = 0;
+    // persistent: thread 0 counts its decisions in LDS (two registers fewer
+    // in every lane)
+    __shared__ uint32_t s_cons, s_tw;
+    if (PERSIST && threadIdx.x == 0) { s_cons = 0u; s_tw = 0u; }
     // persistent: the first kIlp columns per lane of the next vertex's row,
     // loaded while this vertex is scored (its weights load beside the gathers)
     uint32_t px[kIlp];
@@ -728,10 +732,10 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? (WEIGHTED ? 
                         best_c = r_c[i];
                     }
                 }
-                acc_cons += tn > 0;
+                s_cons += tn > 0;
                 if (best != SC::none() && SC::positive(best, self, kv, tot_own, cw.m2)) {
                     a.target[v] = best_c;
-                    ++acc_tw;
+                    ++s_tw;
                 }
             }
         }
@@ -739,6 +743,7 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? (WEIGHTED ? 
         if (!nvalid) break;
         idx = nidx;
     }
+    if (PERSIST && threadIdx.x == 0) { acc_cons = s_cons; acc_tw = s_tw; }
     block_add_stats(a.stats, ST_E_ACTIVE, acc_self, ST_CANDIDATES, acc_cand, ST_CONSIDERED,
                     acc_cons, ST_TWRITES, acc_tw);
 }
