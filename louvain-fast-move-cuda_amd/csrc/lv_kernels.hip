@@ -576,6 +576,18 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? (WEIGHTED ? 
         }
     };
     if constexpr (PERSIST) first_loads(nw);
+    // FUSED (persistent, exact gains): the table is cleared once here, then
+    // by the scoring loop as it reads each slot (every slot belongs to one
+    // lane there), so a vertex starts on a clean table with no clearing loop
+    // and no barrier of its own; the own community's weight, needed only by
+    // the final test, is posted by the lane that meets it
+    constexpr bool FUSED = PERSIST && GAIN == 0;
+    __shared__ uint32_t s_selfw;
+    if constexpr (FUSED) {
+        for (uint32_t s = lane; s < (uint32_t)T; s += G) { K[s] = LV_EMPTY; V[s] = 0u; }
+        if (threadIdx.x == 0) s_selfw = 0u;
+        __syncthreads();
+    }
     for (;;) {
         const MoveWords cw = nw;
         const uint32_t v = cw.v, own = cw.own, kv = cw.kv, tot_own = cw.tot_own;
@@ -586,10 +598,12 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? (WEIGHTED ? 
         const uint32_t nv = nvalid ? list[nidx] : 0u;
         // the vertex's table: table_slots(degree) <= T slots
         const uint32_t tv = live ? table_slots(end - beg) : 4u;
-        for (uint32_t s = lane; s < tv; s += G) { K[s] = LV_EMPTY; V[s] = 0u; }
-        // a vertex's table is its group's alone: groups inside one wave wait
-        // for no other wave of the workgroup
-        group_barrier<G>();
+        if constexpr (!FUSED) {
+            for (uint32_t s = lane; s < tv; s += G) { K[s] = LV_EMPTY; V[s] = 0u; }
+            // a vertex's table is its group's alone: groups inside one wave
+            // wait for no other wave of the workgroup
+            group_barrier<G>();
+        }
         uint32_t nself = 0;
         if (live) {
             // kIlp entries per lane in flight: loads first, then gathers, then inserts
@@ -670,12 +684,14 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? (WEIGHTED ? 
         uint32_t best_c = a.highest ? 0u : LV_EMPTY;
         uint32_t self = 0;
         if (live) {
-            uint32_t h = slot_hash_mod(own, tv);
-            for (;;) {
-                const uint32_t c = K[h];
-                if (c == own) { self = V[h]; break; }
-                if (c == LV_EMPTY) break;
-                h = h + 1 == tv ? 0u : h + 1;
+            if constexpr (!FUSED) {
+                uint32_t h = slot_hash_mod(own, tv);
+                for (;;) {
+                    const uint32_t c = K[h];
+                    if (c == own) { self = V[h]; break; }
+                    if (c == LV_EMPTY) break;
+                    h = h + 1 == tv ? 0u : h + 1;
+                }
             }
             // branch free: slots past the table read slot 0, and a slot that
             // holds no other community is scored but never taken
@@ -689,6 +705,13 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? (WEIGHTED ? 
                     cs[u] = K[si];
                     Ws[u] = V[si];
                     ok[u] = (s2 < tv) & (cs[u] != own) & (cs[u] != LV_EMPTY);
+                    if constexpr (FUSED) {
+                        if (s2 < tv) {
+                            if (cs[u] == own) s_selfw = Ws[u];
+                            K[s2] = LV_EMPTY;
+                            V[s2] = 0u;
+                        }
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < SI; ++u) {
@@ -731,6 +754,10 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? (WEIGHTED ? 
                         best = r_s[i];
                         best_c = r_c[i];
                     }
+                }
+                if constexpr (FUSED) {
+                    self = s_selfw;
+                    s_selfw = 0u;
                 }
                 s_cons += tn > 0;
                 if (best != SC::none() && SC::positive(best, self, kv, tot_own, cw.m2)) {
