@@ -31,6 +31,7 @@
 #include <atomic>
 #include <climits>
 #include <cmath>
+#include <type_traits>
 
 #include "lv_device.h"
 
@@ -605,76 +606,87 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? (WEIGHTED ? 
             group_barrier<G>();
         }
         uint32_t nself = 0;
-        if (live) {
-            // kIlp entries per lane in flight: loads first, then gathers, then inserts
-            for (uint64_t jb = beg + lane; jb < end; jb += (uint64_t)G * kIlp) {
-                uint32_t xs[kIlp], ws[kIlp], cs[kIlp], ts[kIlp];
-                if (PERSIST && jb == beg + lane) {
+        // I entries per lane in flight: loads first, then gathers, then
+        // inserts; `first`: the row's first step, whose columns the persistent
+        // kernels prefetched
+        auto step = [&](auto ilp, uint64_t jb, bool first) {
+            constexpr int I = decltype(ilp)::value;
+            uint32_t xs[I], ws[I], cs[I], ts[I];
+            if (PERSIST && first) {
 #pragma unroll
-                    for (int u = 0; u < kIlp; ++u) {
-                        const uint64_t j = jb + (uint64_t)u * G;
-                        xs[u] = px[u];
-                        if constexpr (WEIGHTED) ws[u] = stream_load(&a.wt[j < end ? j : end - 1]);
+                for (int u = 0; u < I; ++u) {
+                    const uint64_t j = jb + (uint64_t)u * G;
+                    xs[u] = px[u];
+                    if constexpr (WEIGHTED) ws[u] = stream_load(&a.wt[j < end ? j : end - 1]);
+                    else ws[u] = 1u;
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < I; ++u) {
+                    const uint64_t j = jb + (uint64_t)u * G;
+                    if constexpr (G >= 64) {
+                        // past the row's end a lane loads the row's last
+                        // entry (no branch; the lanes of a group share that
+                        // address) and takes the vertex itself, a skipped entry
+                        const uint64_t jc = j < end ? j : end - 1;
+                        const uint32_t x = stream_load(&a.col[jc]);
+                        xs[u] = j < end ? x : v;
+                        if constexpr (WEIGHTED) ws[u] = stream_load(&a.wt[jc]);
                         else ws[u] = 1u;
-                    }
-                } else {
-#pragma unroll
-                    for (int u = 0; u < kIlp; ++u) {
-                        const uint64_t j = jb + (uint64_t)u * G;
-                        if constexpr (G >= 64) {
-                            // past the row's end a lane loads the row's last
-                            // entry (no branch; the lanes of a group share that
-                            // address) and takes the vertex itself, a skipped entry
-                            const uint64_t jc = j < end ? j : end - 1;
-                            const uint32_t x = stream_load(&a.col[jc]);
-                            xs[u] = j < end ? x : v;
-                            if constexpr (WEIGHTED) ws[u] = stream_load(&a.wt[jc]);
-                            else ws[u] = 1u;
-                        } else {
-                            xs[u] = j < end ? stream_load(&a.col[j]) : v;
-                            ws[u] = (WEIGHTED && j < end) ? stream_load(&a.wt[j]) : 1u;
-                        }
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < kIlp; ++u) {
-                    // groups of a wave or more: every xs[u] is a vertex id and
-                    // the words are gathered without a branch (those of
-                    // skipped entries are not used; the group's skipped lanes
-                    // share one address)
-                    const bool g = G >= 64 || xs[u] != v;
-                    if constexpr (PK) {
-                        const unsigned long long p = g ? a.ctot[xs[u]] : 0ull;
-                        cs[u] = (uint32_t)p;
-                        ts[u] = (uint32_t)(p >> 32);
                     } else {
-                        cs[u] = g ? a.comm[xs[u]] : 0u;
-                        ts[u] = 0u;
+                        xs[u] = j < end ? stream_load(&a.col[j]) : v;
+                        ws[u] = (WEIGHTED && j < end) ? stream_load(&a.wt[j]) : 1u;
                     }
                 }
-                // the first probe of all kIlp entries is issued back to back (one
-                // LDS round trip instead of kIlp); a lane's own entries are still
-                // inserted in order, so an entry sees the keys its predecessors
-                // placed, and the rare occupied first slots probe on afterwards
-                uint32_t hs[kIlp], pv[kIlp];
+            }
 #pragma unroll
-                for (int u = 0; u < kIlp; ++u) {
-                    hs[u] = slot_hash_mod(cs[u], tv);
-                    pv[u] = xs[u] != v ? atomicCAS(&K[hs[u]], LV_EMPTY, cs[u]) : LV_EMPTY;
+            for (int u = 0; u < I; ++u) {
+                // groups of a wave or more: every xs[u] is a vertex id and
+                // the words are gathered without a branch (those of
+                // skipped entries are not used; the group's skipped lanes
+                // share one address)
+                const bool g = G >= 64 || xs[u] != v;
+                if constexpr (PK) {
+                    const unsigned long long p = g ? a.ctot[xs[u]] : 0ull;
+                    cs[u] = (uint32_t)p;
+                    ts[u] = (uint32_t)(p >> 32);
+                } else {
+                    cs[u] = g ? a.comm[xs[u]] : 0u;
+                    ts[u] = 0u;
                 }
+            }
+            // the first probe of all I entries is issued back to back (one
+            // LDS round trip instead of I); a lane's own entries are still
+            // inserted in order, so an entry sees the keys its predecessors
+            // placed, and the rare occupied first slots probe on afterwards
+            uint32_t hs[I], pv[I];
 #pragma unroll
-                for (int u = 0; u < kIlp; ++u) {
-                    if (xs[u] == v) continue;
-                    ++nself;
-                    const uint32_t c = cs[u];
-                    uint32_t h = hs[u], prev = pv[u];
-                    while (prev != LV_EMPTY && prev != c) {
-                        h = h + 1 == tv ? 0u : h + 1;
-                        prev = atomicCAS(&K[h], LV_EMPTY, c);
-                    }
-                    if (PK && prev == LV_EMPTY) TT[h] = ts[u];
-                    atomicAdd(&V[h], ws[u]);
+            for (int u = 0; u < I; ++u) {
+                hs[u] = slot_hash_mod(cs[u], tv);
+                pv[u] = xs[u] != v ? atomicCAS(&K[hs[u]], LV_EMPTY, cs[u]) : LV_EMPTY;
+            }
+#pragma unroll
+            for (int u = 0; u < I; ++u) {
+                if (xs[u] == v) continue;
+                ++nself;
+                const uint32_t c = cs[u];
+                uint32_t h = hs[u], prev = pv[u];
+                while (prev != LV_EMPTY && prev != c) {
+                    h = h + 1 == tv ? 0u : h + 1;
+                    prev = atomicCAS(&K[h], LV_EMPTY, c);
                 }
+                if (PK && prev == LV_EMPTY) TT[h] = ts[u];
+                atomicAdd(&V[h], ws[u]);
+            }
+        };
+        if (live) {
+            if (PERSIST && end - beg <= (uint64_t)G * (kIlp / 2)) {
+                // short rows of a persistent kernel's bucket (group-uniform
+                // test): half the entries per lane, no masked half step
+                step(std::integral_constant<int, kIlp / 2>{}, beg + lane, true);
+            } else {
+                for (uint64_t jb = beg + lane; jb < end; jb += (uint64_t)G * kIlp)
+                    step(std::integral_constant<int, kIlp>{}, jb, jb == beg + lane);
             }
         }
         if constexpr (PERSIST) nw = move_words<PK>(a, nv, nvalid);
