@@ -812,6 +812,7 @@ __global__ __launch_bounds__(kBlock) void k_move_fast(MoveCtx a, const uint32_t 
         kv = a.kdeg[v];
         tot_own = a.tot[v];
         const uint64_t beg = a.row[v], end = a.row[v + 1];
+        // candidates tested without branches
         for (uint64_t jb = beg + lane; jb < end; jb += (uint64_t)G * kIlp) {
             uint32_t xs[kIlp], ws[kIlp], ts[kIlp];
 #pragma unroll
@@ -824,10 +825,12 @@ __global__ __launch_bounds__(kBlock) void k_move_fast(MoveCtx a, const uint32_t 
             for (int u = 0; u < kIlp; ++u) ts[u] = xs[u] != v ? a.tot[xs[u]] : 0u;
 #pragma unroll
             for (int u = 0; u < kIlp; ++u) {
-                if (xs[u] == v) continue;
-                ++nself;
+                const bool ok = xs[u] != v;
+                nself += ok;
                 const S sc = SC::of(ws[u], 0u, kv, tot_own, ts[u], m2, tw);
-                if (better_candidate(sc, xs[u], best, best_c, false)) { best = sc; best_c = xs[u]; }
+                const bool b = ok & better_candidate(sc, xs[u], best, best_c, false);
+                best = b ? sc : best;
+                best_c = b ? xs[u] : best_c;
             }
         }
     }
