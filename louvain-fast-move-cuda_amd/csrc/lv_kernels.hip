@@ -548,16 +548,18 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? (WEIGHTED ? 
     constexpr int GPB = kBlock / G;
     constexpr bool PERSIST = G == kBlock;
     constexpr int SI = PERSIST && PK ? 1 : kIlp;
-    __shared__ uint32_t s_key[GPB * T];
-    __shared__ uint32_t s_val[GPB * T];
+    // a slot is (key, summed weight), adjacent: one 8-byte LDS access reads
+    // or clears both
+    __shared__ uint64_t s_kv[GPB * T];
     __shared__ uint32_t s_tot[PK ? GPB * T : 1];
     __shared__ S r_s[kBlock / 64];
     __shared__ uint32_t r_c[kBlock / 64];
     __shared__ uint32_t r_n[kBlock / 64];
     const int lane = threadIdx.x & (G - 1);
     const int gi = (int)group_index<G>();
-    uint32_t *K = s_key + gi * T;
-    uint32_t *V = s_val + gi * T;
+    uint64_t *KV = s_kv + gi * T;
+    uint32_t *const KW = reinterpret_cast<uint32_t *>(KV);   // 2h: key, 2h + 1: weight
+    constexpr uint64_t kEmptySlot = (uint64_t)LV_EMPTY;       // key LV_EMPTY, weight 0
     uint32_t *TT = s_tot + (PK ? gi * T : 0);
     uint32_t idx = PERSIST ? blockIdx.x : blockIdx.x * GPB + gi;
     MoveWords nw = move_words<PK>(a, idx < count ? list[idx] : 0u, idx < count);
@@ -585,7 +587,7 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? (WEIGHTED ? 
     constexpr bool FUSED = PERSIST && GAIN == 0;
     __shared__ uint32_t s_selfw;
     if constexpr (FUSED) {
-        for (uint32_t s = lane; s < (uint32_t)T; s += G) { K[s] = LV_EMPTY; V[s] = 0u; }
+        for (uint32_t s = lane; s < (uint32_t)T; s += G) KV[s] = kEmptySlot;
         if (threadIdx.x == 0) s_selfw = 0u;
         __syncthreads();
     }
@@ -600,7 +602,7 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? (WEIGHTED ? 
         // the vertex's table: table_slots(degree) <= T slots
         const uint32_t tv = live ? table_slots(end - beg) : 4u;
         if constexpr (!FUSED) {
-            for (uint32_t s = lane; s < tv; s += G) { K[s] = LV_EMPTY; V[s] = 0u; }
+            for (uint32_t s = lane; s < tv; s += G) KV[s] = kEmptySlot;
             // a vertex's table is its group's alone: groups inside one wave
             // wait for no other wave of the workgroup
             group_barrier<G>();
@@ -663,7 +665,7 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? (WEIGHTED ? 
 #pragma unroll
             for (int u = 0; u < I; ++u) {
                 hs[u] = slot_hash_mod(cs[u], tv);
-                pv[u] = xs[u] != v ? atomicCAS(&K[hs[u]], LV_EMPTY, cs[u]) : LV_EMPTY;
+                pv[u] = xs[u] != v ? atomicCAS(&KW[2 * hs[u]], LV_EMPTY, cs[u]) : LV_EMPTY;
             }
 #pragma unroll
             for (int u = 0; u < I; ++u) {
@@ -673,10 +675,10 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? (WEIGHTED ? 
                 uint32_t h = hs[u], prev = pv[u];
                 while (prev != LV_EMPTY && prev != c) {
                     h = h + 1 == tv ? 0u : h + 1;
-                    prev = atomicCAS(&K[h], LV_EMPTY, c);
+                    prev = atomicCAS(&KW[2 * h], LV_EMPTY, c);
                 }
                 if (PK && prev == LV_EMPTY) TT[h] = ts[u];
-                atomicAdd(&V[h], ws[u]);
+                atomicAdd(&KW[2 * h + 1], ws[u]);
             }
         };
         if (live) {
@@ -699,8 +701,9 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? (WEIGHTED ? 
             if constexpr (!FUSED) {
                 uint32_t h = slot_hash_mod(own, tv);
                 for (;;) {
-                    const uint32_t c = K[h];
-                    if (c == own) { self = V[h]; break; }
+                    const uint64_t kv = KV[h];
+                    const uint32_t c = (uint32_t)kv;
+                    if (c == own) { self = (uint32_t)(kv >> 32); break; }
                     if (c == LV_EMPTY) break;
                     h = h + 1 == tv ? 0u : h + 1;
                 }
@@ -714,14 +717,14 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? (WEIGHTED ? 
                 for (int u = 0; u < SI; ++u) {
                     const uint32_t s2 = sb + (uint32_t)u * G;
                     const uint32_t si = s2 < tv ? s2 : 0u;
-                    cs[u] = K[si];
-                    Ws[u] = V[si];
+                    const uint64_t kv = KV[si];
+                    cs[u] = (uint32_t)kv;
+                    Ws[u] = (uint32_t)(kv >> 32);
                     ok[u] = (s2 < tv) & (cs[u] != own) & (cs[u] != LV_EMPTY);
                     if constexpr (FUSED) {
                         if (s2 < tv) {
                             if (cs[u] == own) s_selfw = Ws[u];
-                            K[s2] = LV_EMPTY;
-                            V[s2] = 0u;
+                            KV[s2] = kEmptySlot;
                         }
                     }
                 }
