@@ -813,7 +813,7 @@ namespace lv {
 enum AggSlot : size_t {
     W_MAP, W_SCAN32, W_NEWC, W_VK0, W_VK1, W_MEMBERS, W_VV1, W_HIST, W_HIST_SCAN, W_MSTART, W_P,
     W_SCAN64, W_CROW, W_CBUCKET, W_CCOUNT, W_CLISTS, W_TCOL, W_TW, W_RCOUNT, W_HSIZE, W_GINFO,
-    W_GTILES, W_GITEMS, W_GP0, W_GP1, W_GPCNT, W_GPOFF, W_GDCNT, W_GDOFF, W_GBASE
+    W_GTILES, W_GITEMS, W_GP0, W_GP1, W_GPCNT, W_GPOFF, W_GDCNT, W_GDOFF, W_GBASE, W_GTCNT
 };
 
 static int bits_for(uint64_t x) {   // smallest b with 2^b >= x (x >= 1)
@@ -981,6 +981,7 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
         ga.n_items = (uint32_t)items.size();
         ga.p0 = ws.get<unsigned long long>(W_GP0, off);
         ga.p1 = ws.get<unsigned long long>(W_GP1, off);
+        ga.tcnt = ws.get<uint32_t>(W_GTCNT, tiles.size());
         ga.pcnt = ws.get<uint32_t>(W_GPCNT, pbase);
         ga.poff = ws.get<uint32_t>(W_GPOFF, pbase);
         ga.dcnt = ws.get<uint32_t>(W_GDCNT, pbase);

@@ -180,20 +180,31 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
 // bits of hub_mix (one per kGiantPartEntries entries, at most 4096) and its
 // virtual entries into tiles of 2^lgt (at least 16 entries per partition, so
 // a tile reserves at most one run per 16 entries).  k_giant_gather writes the
-// (destination, weight) pair of every entry in tile order, folding repeats of
-// the destinations it caches (a community's own id and the giants it is
-// bound to repeat hundreds of thousands of times) into one pair per tile, and
-// reserves each (tile, partition) run; with poff = exclusive scan of the
-// partition counts (every pair is in one partition, so poff is the pair
-// buffers' position of a partition), k_giant_scatter moves the pairs to
-// partition order;
+// (destination, weight) pairs of a tile, packed at the start of the tile's
+// range, folding repeats of the destinations it caches (a community's own id
+// and the giants it is bound to repeat hundreds of thousands of times) into
+// one pair per tile, and reserves each (tile, partition) run; with poff =
+// exclusive scan of the partition counts (every pair is in one partition, so
+// poff is the pair buffers' position of a partition), k_giant_scatter moves
+// a tile's pairs to partition order;
 // k_giant_reduce sums one partition per workgroup in the LDS table
 // (lv_device.h) and writes its distinct pairs back over the tile-order buffer;
 // k_giant_copy puts them at the community's final row.
 
 constexpr int kGiantIlp = 8;
 constexpr uint32_t kGiantCache = 1024;                 // destinations cached per tile
-constexpr unsigned long long kGiantHole = ~0ull;       // entry folded into a cached one
+
+// Appends `pair` (lanes with `emit`) at out[*n ...]: one LDS atomic per wave.
+__device__ inline void wave_append(bool emit, unsigned long long pair, uint32_t *n,
+                                   unsigned long long *out) {
+    const unsigned long long m = __ballot(emit);
+    if (m == 0) return;
+    const int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(n, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, leader);
+    if (emit) out[base + (uint32_t)__popcll(m & lanemask_lt())] = pair;
+}
 
 struct GiantCtx : GiantArgs {
     AggCtx a;
@@ -209,11 +220,11 @@ __global__ __launch_bounds__(kBlock) void k_giant_gather(GiantCtx h) {
     __shared__ uint64_t s_mrow[kBlock];
     __shared__ uint64_t s_mp[kBlock + 1];
     __shared__ uint32_t s_first;
-    // destinations cached for the tile: a repeat adds its weight and leaves a
-    // hole; each cached destination is written once, at its first entry
+    __shared__ uint32_t s_n;             // pairs written so far
+    // destinations cached for the tile: a repeat adds its weight; each cached
+    // destination is written once, after the tile's other pairs
     __shared__ uint32_t c_key[kGiantCache];
     __shared__ uint32_t c_val[kGiantCache];
-    __shared__ uint32_t c_pos[kGiantCache];
     const AggCtx &a = h.a;
     const uint64_t tile = h.tiles[blockIdx.x];
     const uint32_t lt = (uint32_t)tile;
@@ -225,9 +236,11 @@ __global__ __launch_bounds__(kBlock) void k_giant_gather(GiantCtx h) {
     const uint64_t s0 = k0 + ((uint64_t)lt << I.lgt);
     const uint64_t s1 = s0 + (1ull << I.lgt) < k1 ? s0 + (1ull << I.lgt) : k1;
     const uint32_t m0 = a.mstart[I.c], m1 = a.mstart[I.c + 1];
-    unsigned long long *out = h.p0 + I.off;
+    // the tile's range of the pair buffer: at most one pair per entry
+    unsigned long long *out = h.p0 + I.off + (s0 - k0);
     if (threadIdx.x == 0) {
         s_first = member_of(a.P, m0, m1, s0);
+        s_n = 0u;
         if (lt == 0) a.rcount[I.c] = 0u;   // the reduce adds the distinct counts
     }
     __syncthreads();
@@ -263,25 +276,19 @@ __global__ __launch_bounds__(kBlock) void k_giant_gather(GiantCtx h) {
 #pragma unroll
             for (int u = 0; u < kGiantIlp; ++u) {
                 const uint32_t d = ds[u];
-                if (d == LV_EMPTY) continue;
-                const uint64_t k = kb + (uint64_t)u * kBlock;
-                const uint32_t cs = hub_mix(d) & (kGiantCache - 1);
-                uint32_t cur = c_key[cs];
-                if (cur == LV_EMPTY) {
-                    cur = atomicCAS(&c_key[cs], LV_EMPTY, d);
-                    if (cur == LV_EMPTY) {   // cached here; written at the flush
-                        c_pos[cs] = (uint32_t)(k - k0);
+                bool emit = false;
+                if (d != LV_EMPTY) {
+                    const uint32_t cs = hub_mix(d) & (kGiantCache - 1);
+                    uint32_t cur = c_key[cs];
+                    if (cur == LV_EMPTY) cur = atomicCAS(&c_key[cs], LV_EMPTY, d);
+                    if (cur == LV_EMPTY || cur == d) {   // cached here, or a repeat
                         atomicAdd(&c_val[cs], ws[u]);
-                        continue;
+                    } else {
+                        emit = true;
+                        atomicAdd(&hist[giant_part(d, I.lgp)], 1u);
                     }
                 }
-                if (cur == d) {
-                    atomicAdd(&c_val[cs], ws[u]);
-                    out[k - k0] = kGiantHole;
-                    continue;
-                }
-                out[k - k0] = ((unsigned long long)d << 32) | ws[u];
-                atomicAdd(&hist[giant_part(d, I.lgp)], 1u);
+                wave_append(emit, ((unsigned long long)d << 32) | ws[u], &s_n, out);
             }
         }
         const bool done = s_mp[nm] >= s1;
@@ -289,13 +296,14 @@ __global__ __launch_bounds__(kBlock) void k_giant_gather(GiantCtx h) {
         if (done) break;
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < kGiantCache; i += kBlock) {
+    for (uint32_t i = threadIdx.x; i < kGiantCache; i += kBlock) {   // uniform trip count
         const uint32_t d = c_key[i];
-        if (d == LV_EMPTY) continue;
-        out[c_pos[i]] = ((unsigned long long)d << 32) | c_val[i];
-        atomicAdd(&hist[giant_part(d, I.lgp)], 1u);
+        const bool emit = d != LV_EMPTY;
+        if (emit) atomicAdd(&hist[giant_part(d, I.lgp)], 1u);
+        wave_append(emit, ((unsigned long long)d << 32) | c_val[i], &s_n, out);
     }
     __syncthreads();
+    if (threadIdx.x == 0) h.tcnt[blockIdx.x] = s_n;
     for (uint32_t p = threadIdx.x; p < np; p += kBlock) {
         const uint32_t c = hist[p];
         if (c) h.base[I.mbase + (uint64_t)lt * np + p] = atomicAdd(&h.pcnt[I.pbase + p], c);
@@ -316,20 +324,18 @@ __global__ __launch_bounds__(kBlock) void k_giant_scatter(GiantCtx h) {
         s_rank[p] = 0u;
     }
     __syncthreads();
-    const uint64_t E = h.a.crow[I.c + 1] - h.a.crow[I.c];
-    const uint64_t t0 = (uint64_t)lt << I.lgt;
-    const uint64_t t1 = t0 + (1ull << I.lgt) < E ? t0 + (1ull << I.lgt) : E;
-    const unsigned long long *in = h.p0 + I.off;
-    for (uint64_t kb = t0 + threadIdx.x; kb < t1; kb += (uint64_t)kBlock * kGiantIlp) {
+    const uint32_t cnt = h.tcnt[blockIdx.x];
+    const unsigned long long *in = h.p0 + I.off + ((uint64_t)lt << I.lgt);
+    for (uint32_t kb = threadIdx.x; kb < cnt; kb += kBlock * kGiantIlp) {
         unsigned long long pr[kGiantIlp];
 #pragma unroll
         for (int u = 0; u < kGiantIlp; ++u) {
-            const uint64_t k = kb + (uint64_t)u * kBlock;
-            pr[u] = k < t1 ? in[k] : kGiantHole;
+            const uint32_t k = kb + (uint32_t)u * kBlock;
+            pr[u] = k < cnt ? in[k] : 0ull;
         }
 #pragma unroll
         for (int u = 0; u < kGiantIlp; ++u) {
-            if (pr[u] == kGiantHole) continue;   // past the tile, or folded by the gather
+            if (kb + (uint32_t)u * kBlock >= cnt) continue;
             const uint32_t p = giant_part((uint32_t)(pr[u] >> 32), I.lgp);
             h.p1[s_off[p] + atomicAdd(&s_rank[p], 1u)] = pr[u];
         }

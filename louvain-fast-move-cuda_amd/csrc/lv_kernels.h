@@ -304,7 +304,9 @@ struct GiantArgs {
     uint32_t n_tiles;
     const uint64_t *items;          // [n_items] giant index << 32 | partition
     uint32_t n_items;
-    unsigned long long *p0;         // [entries] pairs in tile order, then distinct pairs
+    unsigned long long *p0;         // [entries] pairs of each tile at its range's start, then
+                                    // distinct pairs
+    uint32_t *tcnt;                 // [n_tiles] pairs each tile wrote
     unsigned long long *p1;         // [entries] pairs in partition order
     uint32_t *pcnt;                 // [partitions] entries per partition, zeroed
     uint32_t *poff;                 // [partitions] exclusive scan of pcnt: a partition's pairs
