@@ -91,7 +91,7 @@ void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
         hub_cbuf.ensure(hub_entries);
         hub_pair.ensure(hub_entries);
         hub_cnt.ensure((size_t)n_hubs * kHubParts);
-        hub_base.ensure((size_t)n_hubs * kHubParts * kHubParts);
+        hub_base.ensure((size_t)n_hubs * kHubChunks * kHubParts);
         hub_items.ensure((size_t)n_hubs * kHubParts);
         hub_n_items.ensure(1);
         fill_u32(hub_n_items.data(), 0u, 1, s);

@@ -20,7 +20,16 @@ namespace lv {
 constexpr int kNumBuckets = 9;        // 8 LDS-table buckets + global-table hubs
 constexpr int kHubBucket = 8;
 constexpr int kStatSlots = 64;        // spread of the per-block statistic atomics
-constexpr int kHubParts = 32;         // row chunks and community partitions per hub vertex
+constexpr int kHubParts = 32;         // community partitions per hub vertex
+#ifndef LV_HUB_CHUNKS
+#define LV_HUB_CHUNKS 16
+#endif
+// Row chunks per hub vertex, one workgroup each, in the hub kernels that walk
+// the row (counting, scattering, the fast sweep): a few entries per lane, so a
+// workgroup's fixed chain of loads is paid over many gathers (R-MAT-22, hash
+// sweeps' hub path on one stream: 2.73 ms per run with 32 chunks, 2.61 with
+// 16, 2.75 with 8; fast sweeps 0.59, 0.51, 0.53).
+constexpr int kHubChunks = LV_HUB_CHUNKS;
 constexpr int kBigBuckets = 2;        // large-row buckets (bucket 7, hubs) committed over movers
 constexpr uint32_t kMaxChunks = 4096; // reference chunks per level (STEP_ROUND emulation)
 
@@ -105,10 +114,11 @@ struct HubTables {
     uint32_t *cbuf;           // [hub entries] community of each entry (LV_EMPTY: self-loop)
     unsigned long long *pair; // [hub entries] (community << 32) | weight in partition order
     uint32_t *cnt;            // [n_hubs * kHubParts] entries per partition, zero between sweeps
-    uint32_t *base;           // [n_hubs * kHubParts * kHubParts] start of (chunk, partition)
+    uint32_t *base;           // [n_hubs * kHubChunks * kHubParts] start of (chunk, partition)
     uint64_t *items;          // [n_hubs * kHubParts] reduce work: vertex << 32 | slot << 8 | log2 groups << 5 | group
     uint32_t *n_items;        // [1] items posted this sweep, zero between sweeps
-    int64_t *part_s;          // [n_hubs * kHubParts] partial best scores
+    int64_t *part_s;          // [n_hubs * kHubParts] partial best scores (reduce groups, or
+                              // the fast sweep's chunks)
     uint32_t *part_c;         // [n_hubs * kHubParts] partial best communities
     uint32_t *self;           // [n_hubs] weight to own community, zero between sweeps
 };

@@ -879,7 +879,7 @@ __global__ __launch_bounds__(kBlock) void k_move_fast(MoveCtx a, const uint32_t 
 
 // ---- hubs: kHubParts workgroups per vertex, partitioned by community hash ----
 //
-// A hub row is cut into kHubParts contiguous chunks (one workgroup each) and
+// A hub row is cut into kHubChunks contiguous chunks (one workgroup each) and
 // its neighbour communities into kHubParts partitions by the top bits of a
 // bijective mix of the community id.  k_hub_count gathers the community of
 // every entry into the hub's c-buffer, sums the weight towards the hub's own
@@ -907,6 +907,7 @@ constexpr uint64_t kHubGroupEntries = 4096;                // entries per reduce
 constexpr uint32_t kHubReduceBlocks = 8192;
 constexpr int kHubIlp = 16;                                // pair loads / tot gathers in flight per lane
 static_assert(kHubParts == 32, "items pack the group in 5 bits");
+static_assert(kHubChunks >= 1 && kHubChunks <= kHubParts, "fast-sweep chunks use the group slots");
 
 __device__ inline uint32_t ilog2_u32(uint32_t x) { return 31u - (uint32_t)__builtin_clz(x); }
 
@@ -921,7 +922,7 @@ __device__ inline uint32_t hub_groups(uint64_t d) {
 __device__ inline void hub_chunk(uint64_t beg, uint64_t end, uint32_t q, uint64_t &cb,
                                  uint64_t &ce) {
     const uint64_t d = end - beg;
-    const uint64_t per = (d + kHubParts - 1) / kHubParts;
+    const uint64_t per = (d + kHubChunks - 1) / kHubChunks;
     cb = beg + min(d, per * q);
     ce = beg + min(d, per * (q + 1));
 }
@@ -940,13 +941,13 @@ __global__ __launch_bounds__(kBlock) void k_hub_count(HubCtx a, const uint32_t *
                                                       uint32_t count) {
     __shared__ uint32_t hist[kHubParts];
     __shared__ uint64_t r_self[kBlock / 64];
-    const uint32_t q = blockIdx.x % kHubParts;
+    const uint32_t q = blockIdx.x % kHubChunks;
     if (threadIdx.x < kHubParts) hist[threadIdx.x] = 0;
     __syncthreads();
     uint32_t v = 0, slot = 0;
     int64_t m2;
     double tw;
-    const bool run = hub_runs(a, list, count, blockIdx.x / kHubParts, v, m2, tw);
+    const bool run = hub_runs(a, list, count, blockIdx.x / kHubChunks, v, m2, tw);
     uint64_t nself = 0, selfw = 0;
     if (run) {
         slot = a.h.slot_of[v];
@@ -983,7 +984,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_count(HubCtx a, const uint32_t *
         if (threadIdx.x < kHubParts) {
             const uint32_t h = hist[threadIdx.x];
             if (h)
-                a.h.base[((size_t)slot * kHubParts + q) * kHubParts + threadIdx.x] =
+                a.h.base[((size_t)slot * kHubChunks + q) * kHubParts + threadIdx.x] =
                     atomicAdd(&a.h.cnt[(size_t)slot * kHubParts + threadIdx.x], h);
         }
         if (threadIdx.x == 0) {
@@ -1008,11 +1009,11 @@ __global__ __launch_bounds__(kBlock) void k_hub_scatter(HubCtx a,
                                                         uint32_t count) {
     __shared__ uint32_t s_off[kHubParts];
     __shared__ uint32_t s_rank[kHubParts];
-    const uint32_t q = blockIdx.x % kHubParts;
+    const uint32_t q = blockIdx.x % kHubChunks;
     uint32_t v = 0;
     int64_t m2;
     double tw;
-    const bool run = hub_runs(a, list, count, blockIdx.x / kHubParts, v, m2, tw);
+    const bool run = hub_runs(a, list, count, blockIdx.x / kHubChunks, v, m2, tw);
     if (!run) return;   // uniform over the workgroup
     const uint32_t slot = a.h.slot_of[v];
     if (threadIdx.x < 64) {
@@ -1028,7 +1029,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_scatter(HubCtx a,
         if (lane < kHubParts) {
             // base is only written for runs this chunk has; others stay unused
             s_off[lane] = (incl - c) +
-                          a.h.base[((size_t)slot * kHubParts + q) * kHubParts + lane];
+                          a.h.base[((size_t)slot * kHubChunks + q) * kHubParts + lane];
             s_rank[lane] = 0;
         }
     }
@@ -1160,8 +1161,8 @@ __global__ __launch_bounds__(kBlock) void k_hub_fast(HubCtx a, const uint32_t *_
                                                      uint32_t count) {
     using SC = Score<GAIN>;
     using S = typename SC::T;
-    const uint32_t hi = blockIdx.x / kHubParts;
-    const uint32_t part = blockIdx.x % kHubParts;
+    const uint32_t hi = blockIdx.x / kHubChunks;
+    const uint32_t part = blockIdx.x % kHubChunks;
     S best = SC::none();
     uint32_t best_c = LV_EMPTY;
     uint64_t nself = 0;
@@ -1177,7 +1178,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_fast(HubCtx a, const uint32_t *_
             const uint32_t tot_own = a.m.tot[v];
             const uint64_t beg = a.m.row[v], end = a.m.row[v + 1];
             for (uint64_t j = beg + (uint64_t)part * kBlock + threadIdx.x; j < end;
-                 j += (uint64_t)kHubParts * kBlock) {
+                 j += (uint64_t)kHubChunks * kBlock) {
                 const uint32_t x = a.m.col[j];
                 if (x == v) continue;
                 ++nself;
@@ -1222,7 +1223,7 @@ __global__ void k_hub_decide(HubCtx a, const uint32_t *__restrict__ list, uint32
     const uint32_t slot = a.h.slot_of[v];
     const int lane = threadIdx.x;
     const bool highest = fast ? false : a.m.highest;
-    const uint32_t parts = fast ? kHubParts : hub_groups(a.m.row[v + 1] - a.m.row[v]);
+    const uint32_t parts = fast ? kHubChunks : hub_groups(a.m.row[v + 1] - a.m.row[v]);
     S best = SC::none();
     uint32_t best_c = highest ? 0u : LV_EMPTY;
     if (lane < (int)parts) {
@@ -1422,7 +1423,10 @@ __global__ __launch_bounds__(kBlock) void k_commit(CommitCtx a, SmallLists sl) {
 // Large rows (bucket 7, hubs): most listed vertices stay after the
 // first sweep, so the movers are compacted first (per bucket, one atomic per
 // wave) and the commit runs over (mover, part) items on a persistent grid.
-constexpr uint32_t kBigParts[kBigBuckets] = {4u, (uint32_t)kHubParts};
+// Parts per mover (kBlock lanes each): a hub mover of ~9,000 entries then
+// gives its lanes four or five entries each (measured: 8 parts committed
+// R-MAT-22 slightly faster than 16 or 32).
+constexpr uint32_t kBigParts[kBigBuckets] = {4u, 8u};
 // Grid of k_commit_big: many rounds of resident workgroups, for the same
 // reason (measured faster than 2,048 to 16,384).
 constexpr uint32_t kCommitBlocks = 32768;
@@ -1957,7 +1961,7 @@ void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
                     break;
                 default: {
                     HubCtx h{m, hub};
-                    const dim3 grid(cnt * kHubParts), blk(kBlock);
+                    const dim3 grid(cnt * kHubChunks), blk(kBlock);
                     if (a.gain == 0) {
                         if (wtd) hipLaunchKernelGGL((k_hub_fast<0, true>), grid, blk, 0, s, h, L, cnt);
                         else hipLaunchKernelGGL((k_hub_fast<0, false>), grid, blk, 0, s, h, L, cnt);
@@ -1979,7 +1983,7 @@ void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
                     break;
                 default: {
                     HubCtx h{m, hub};
-                    const dim3 grid(cnt * kHubParts), blk(kBlock);
+                    const dim3 grid(cnt * kHubChunks), blk(kBlock);
                     if (wtd) {
                         hipLaunchKernelGGL(k_hub_count<true>, grid, blk, 0, s, h, L, cnt);
                         hipLaunchKernelGGL(k_hub_scatter<true>, grid, blk, 0, s, h, L, cnt);
