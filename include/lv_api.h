@@ -83,7 +83,11 @@ typedef struct lv_params {
 } lv_params;
 
 typedef struct lv_sweep_rec {
-    int32_t level, sweep, kind, _pad;
+    /* pre_decided: 1 when the sweep's best moves were decided by the
+     * contraction that built the level (the first, fast sweep of a level
+     * above 0 in lv_community_run): no best-move pass ran, move_ms and
+     * move_bytes are 0 and that work is in the contraction's time */
+    int32_t level, sweep, kind, pre_decided;
     uint32_t n, n_active, n_moved, n_considered;
     uint64_t E, e_active;      /* e_active: entries of active vertices with src != dst */
     int64_t sigma_in;          /* weight of entries inside communities */

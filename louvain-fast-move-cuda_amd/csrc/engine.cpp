@@ -578,18 +578,29 @@ void autotune_tiles(Community &c, const lv_params &p, int kind, uint32_t v_end) 
 
 // One sweep from the current lists; returns stats.  Kernels only; the caller
 // syncs.  Used by the sweep loop and by lv_best_moves.
-void launch_sweep(Community &c, const lv_params &p, int kind, bool apply, uint32_t v_end) {
+void launch_sweep(Community &c, const lv_params &p, int kind, bool apply, uint32_t v_end,
+                  bool pre_decided) {
     SweepState &st = *c.state;
     hipStream_t s = c.stream();
-    if (p.autotune) autotune_tiles(c, p, kind, v_end);
+    if (p.autotune && !pre_decided) autotune_tiles(c, p, kind, v_end);
     SweepArgs a = sweep_args(c, p, kind, v_end);
     sweep_reset(st.stats.data(), st.stats.size(), a.n_movers, st.next_counts(), s);
     a.counters_zeroed = true;
     uint32_t *const *cur = st.lists(st.cur);
     if (p.profile) LV_HIP(hipEventRecord(st.ev[0], s));
-    if (kind == LV_SWEEP_SORT) sort_sweep(a, cur, st.count, st.bucket_off, st.ws, s);
-    else move_pass(a, st.hub_tables(), cur, st.count, s,
-                   p.profile == 3 ? nullptr : st.side_streams());
+    if (pre_decided) {
+        // the contraction's decisions and best-move statistics (one section)
+        LV_HIP(hipMemcpyAsync(st.stats.data(), st.pre_stats.data(),
+                              kSection * sizeof(unsigned long long), hipMemcpyDeviceToDevice, s));
+        if (a.g.n)
+            LV_HIP(hipMemcpyAsync(st.target.data(), st.pre_target.data(), a.g.n * sizeof(uint32_t),
+                                  hipMemcpyDeviceToDevice, s));
+    } else if (kind == LV_SWEEP_SORT) {
+        sort_sweep(a, cur, st.count, st.bucket_off, st.ws, s);
+    } else {
+        move_pass(a, st.hub_tables(), cur, st.count, s,
+                  p.profile == 3 ? nullptr : st.side_streams());
+    }
     if (p.profile) LV_HIP(hipEventRecord(st.ev[1], s));
     if (apply) {
         commit_pass(a, cur, st.count, s, st.side_streams());
@@ -622,14 +633,21 @@ int OptimizationPhase::sweep_loop(Community &c, const lv_params &p, int64_t exec
     double delta;
     int nrec = 0;
     std::vector<uint32_t> comm_host;
+    // the first sweep of a level reached by a contraction that decided it
+    // (every vertex listed, identity partition: the fast sweep it replaces)
+    const bool pre = exec0 == 0 && st.pre_valid && st.pre_level == c.level &&
+                     st.pre_gain == p.gain && !p.chunk_emulation &&
+                     sweep_kind(p.mode, exec0) == LV_SWEEP_FAST;
+    st.pre_valid = false;
     do {
         const double old_q = c.modularity;
         const int kind = sweep_kind(p.mode, exec);
+        const bool pre_decided = pre && exec == exec0;
         uint32_t n_active = 0;
         for (int b = 0; b < kNumBuckets; ++b) n_active += st.count[b];
         uint32_t v_end = n;
         if (p.chunk_emulation && kind != LV_SWEEP_HASH) v_end = chunk_vertex_end(c, p, kind, nullptr);
-        launch_sweep(c, p, kind, true, v_end);
+        launch_sweep(c, p, kind, true, v_end, pre_decided);
         wait_stream(s);
         const unsigned long long *h = st.host_stats;
         const uint64_t e_active = all_bucket_slots(h, ST_E_ACTIVE);
@@ -642,7 +660,18 @@ int OptimizationPhase::sweep_loop(Community &c, const lv_params &p, int64_t exec
         std::memcpy(next_counts, h + kStatWordsOffset, sizeof(next_counts));
         float ms_move = 0, ms_sweep = 0;
         uint64_t sweep_cand = 0, sweep_bytes = 0, commit_bytes = 0;
-        if (p.profile) {
+        if (p.profile && pre_decided) {
+            // no best-move pass ran: its work is in the contraction's time
+            LV_HIP(hipEventElapsedTime(&ms_sweep, st.ev[0], st.ev[2]));
+            for (int b = 0; b < kNumBuckets; ++b) sweep_cand += bucket_slots(h, b, ST_CANDIDATES);
+            st.acc.sweep_ms += ms_sweep;
+            st.acc.sweeps += 1;
+            commit_bytes = 8ull * n_active + 24ull * moved + 5ull * ment + n;
+            st.acc.commit_passes += 1;
+            st.acc.commit_ms += ms_sweep;
+            st.acc.commit_bytes += commit_bytes;
+            st.acc.commit_entries += ment;
+        } else if (p.profile) {
             LV_HIP(hipEventElapsedTime(&ms_move, st.ev[0], st.ev[1]));
             LV_HIP(hipEventElapsedTime(&ms_sweep, st.ev[0], st.ev[2]));
             const bool fast = kind == LV_SWEEP_FAST;
@@ -709,6 +738,7 @@ int OptimizationPhase::sweep_loop(Community &c, const lv_params &p, int64_t exec
             r.level = rec_level;
             r.sweep = (int32_t)exec;
             r.kind = kind;
+            r.pre_decided = pre_decided ? 1 : 0;
             r.n = n;
             r.n_active = n_active;
             r.n_moved = (uint32_t)moved;
@@ -813,7 +843,7 @@ namespace lv {
 enum AggSlot : size_t {
     W_MAP, W_SCAN32, W_NEWC, W_VK0, W_VK1, W_MEMBERS, W_VV1, W_HIST, W_HIST_SCAN, W_MSTART, W_P,
     W_SCAN64, W_CROW, W_CBUCKET, W_CCOUNT, W_CLISTS, W_TCOL, W_TW, W_RCOUNT, W_HSIZE, W_GINFO,
-    W_GTILES, W_GITEMS, W_GP0, W_GP1, W_GPCNT, W_GPOFF, W_GDCNT, W_GDOFF, W_GBASE, W_GTCNT
+    W_GTILES, W_GITEMS, W_GP0, W_GP1, W_GPCNT, W_GPOFF, W_GDCNT, W_GDOFF, W_GBASE, W_GTCNT, W_NEWCK, W_DPS, W_DPC, W_DPN
 };
 
 static int bits_for(uint64_t x) {   // smallest b with 2^b >= x (x >= 1)
@@ -861,6 +891,11 @@ void AggregationPhase::run(Community &c, bool aggregate, MODE mode) {
     contract(c, map, n_new);
     c.state->defer_copy_join = false;
     c.level += 1;
+    if (c.state->first_sweep_gain >= 0 && !c.batch) {
+        c.state->pre_valid = true;
+        c.state->pre_level = c.level;
+        c.state->pre_gain = c.state->first_sweep_gain;
+    }
     // community.update() (:174): identity partition on the contracted graph.
     // Its modularity parts equal the ones just reached (self-loop weight of a
     // community = its internal weight, new degree = old total), so they carry.
@@ -889,7 +924,13 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
     uint32_t *newc = ws.get<uint32_t>(W_NEWC, n);
     uint64_t *vk0 = ws.get<uint64_t>(W_VK0, n), *vk1 = ws.get<uint64_t>(W_VK1, n);
     uint32_t *members = ws.get<uint32_t>(W_MEMBERS, n), *vv1 = ws.get<uint32_t>(W_VV1, n);
-    agg_newc(c.communities.data(), map, n, newc, vk0, members, s);
+    // the next level's first sweep decided here when the caller asks for it
+    // (single graphs; ModularityAlgorithms::run)
+    SweepState &st = *c.state;
+    const int dgain = c.batch ? -1 : st.first_sweep_gain;
+    unsigned long long *newck = dgain >= 0 ? ws.get<unsigned long long>(W_NEWCK, n) : nullptr;
+    agg_newc(c.communities.data(), map, n, newc, vk0, members, c.communities_weight.data(), newck,
+             s);
     radix_sort_pairs(vk0, members, vk1, vv1, n, bits_for(n_new),
                      ws.get<uint32_t>(W_HIST, radix_hist_elems(n)),
                      ws.get<uint32_t>(W_HIST_SCAN, scan_scratch_elems(radix_hist_elems(n))), s);
@@ -925,6 +966,20 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
     x.tcol = ws.get<uint32_t>(W_TCOL, E);
     x.tw = ws.get<uint32_t>(W_TW, E);
     x.rcount = ws.get<uint32_t>(W_RCOUNT, n_new);
+    // new degrees (= old community totals) first: the decisions read them
+    new_degrees(c.communities_weight.data(), map, n, ng.tot_weight_per_nodes.data(), s);
+    if (dgain >= 0) {
+        x.dgain = dgain;
+        x.m2 = (int64_t)(2.0 * c.graph.total_weight);   // as the sweeps take it
+        x.tw_total = c.graph.total_weight;
+        x.newck = newck;
+        x.knew = ng.tot_weight_per_nodes.data();
+        st.pre_target.ensure(std::max<size_t>(n_new, 1));
+        st.pre_stats.ensure(kSection);
+        x.dtarget = st.pre_target.data();
+        x.dstats = st.pre_stats.data();
+        fill_u64(st.pre_stats.data(), 0ull, kSection, s);
+    }
     // 4. giant communities: their entries in destination hash partitions,
     //    each partition summed per destination in an LDS table (planned on the
     //    host first, then run on a side stream beside step 3's kernels: the
@@ -989,6 +1044,12 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
         ga.base = ws.get<uint32_t>(W_GBASE, mbase);
         ga.scan_tmp = ws.get<uint32_t>(W_SCAN32, scan_scratch_elems(pbase));
         ga.max_parts = max_parts;
+        ga.n_giants = nh;
+        if (dgain >= 0) {
+            ga.dpart_s = ws.get<int64_t>(W_DPS, pbase);
+            ga.dpart_c = ws.get<uint32_t>(W_DPC, pbase);
+            ga.dpart_n = ws.get<uint32_t>(W_DPN, pbase);
+        }
         n_parts = pbase;
     }
     const SideStreams *side = nh ? c.state->side_streams() : nullptr;
@@ -1024,7 +1085,6 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
     agg_giant_copy(x, ga, n_parts, orow, ng.edge_destination.data(), ng.weights.data(), cs->s[0]);
     LV_HIP(hipEventRecord(cs->join[0], cs->s[0]));
     c.state->row_copy_pending = true;
-    new_degrees(c.communities_weight.data(), map, n, ng.tot_weight_per_nodes.data(), s);
     ng.n_entries = e_new;
     ng.n_links = e_new / 2;                                           // :97
     c.graph = GraphDevice::view_of(ng);
@@ -1105,7 +1165,11 @@ void ModularityAlgorithms::run(Community &C, const lv_params &p, lv_result *r) {
         const double q_end = C.modularity;
         const bool wtd = !C.graph.unit_weights;
         if (p.profile) LV_HIP(hipEventRecord(C.state->ev[3], s));
+        // the next level's first (fast) sweep is decided while its rows are
+        // built, unless chunked sweeps are emulated
+        C.state->first_sweep_gain = (cont && !p.chunk_emulation && !C.batch) ? p.gain : -1;
         AggregationPhase::run(C, cont, (MODE)p.mode);                 // :53
+        C.state->first_sweep_gain = -1;
         float ms_contract = 0;
         uint64_t contract_bytes = 0, e_new = 0;
         if (cont) e_new = C.graph.n_entries;

@@ -138,6 +138,17 @@ struct SweepState {
     // setup, which reads the new row offsets only (join_row_copy)
     bool defer_copy_join = false;
     bool row_copy_pending = false;
+    // The first sweep of the next level decided by the contraction (see
+    // AggArgs::dgain): asked with first_sweep_gain >= 0 before a contraction;
+    // the decisions and that sweep's best-move statistics are kept for the
+    // sweep loop of level pre_level, which uses them if its first sweep is a
+    // fast sweep over every vertex with the same gain mode.
+    int first_sweep_gain = -1;
+    bool pre_valid = false;
+    int32_t pre_level = -1;
+    int pre_gain = -1;
+    DeviceArray<uint32_t> pre_target;
+    DeviceArray<unsigned long long> pre_stats;
     // tiles of the best-move kernels (tuned once per community when asked)
     TileChoice tiles;
     bool tuned[2] = {false, false};   // [fast sweep]
@@ -221,7 +232,10 @@ uint64_t sum_slots(const unsigned long long *h, int stat);            // global 
 void apply_bucket_bounds(Community &C, const lv_params &p);
 uint64_t bucket_slots(const unsigned long long *h, int bucket, int stat);
 uint64_t all_bucket_slots(const unsigned long long *h, int stat);
-void launch_sweep(::Community &c, const lv_params &p, int kind, bool apply, uint32_t v_end);
+// pre_decided: the best-move pass is replaced by the decisions and statistics
+// the contraction kept (SweepState::pre_target, pre_stats)
+void launch_sweep(::Community &c, const lv_params &p, int kind, bool apply, uint32_t v_end,
+                  bool pre_decided = false);
 // Times each LDS bucket's tile candidates on the current lists (once per
 // community and sweep class) and keeps the fastest in state->tiles.
 void autotune_tiles(::Community &c, const lv_params &p, int kind, uint32_t v_end);

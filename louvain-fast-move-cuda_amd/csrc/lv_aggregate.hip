@@ -40,6 +40,35 @@ struct AggCtx {
     uint32_t *tcol;            // [E] staging: distinct destinations
     uint32_t *tw;              // [E] staging: their weights
     uint32_t *rcount;          // [n_new] distinct destinations per community
+    // the next level's first sweep decided from the rows (DEC kernels)
+    int64_t m2;
+    double tw_total;
+    const unsigned long long *newck;   // [n] newc | new degree << 32
+    const uint32_t *knew;              // [n_new] new degrees
+    uint32_t *dtarget;                 // [n_new]
+    unsigned long long *dstats;        // one statistics section
+};
+
+// The next level's first sweep, decided from a contracted row: that sweep is
+// a fast sweep at the identity partition (phaseOptimizationSort.cu:28-158 with
+// the v < c guard of phaseOptimizationUpdate.cu:99), so new vertex c's
+// candidates are the distinct destinations d != c of its row, each with the
+// row's summed weight w_cd and the total of d's singleton community, d's new
+// degree k_d; c's own total is k_c.  DEC = 1 + the gain mode.  The same
+// scores, order and move rule as the fast-sweep kernels (k_move_fast).
+template <int DEC>
+struct FirstSweep {
+    using SC = Score<DEC - 1>;
+    using S = typename SC::T;
+    __device__ static S score(const AggCtx &a, uint32_t w, uint32_t kc, uint32_t kd) {
+        return SC::of(w, 0u, kc, kc, kd, a.m2, a.tw_total);
+    }
+    // the decision for c from its best candidate (lane-uniform call)
+    __device__ static uint32_t decide(const AggCtx &a, uint32_t c, uint32_t kc, S best,
+                                      uint32_t best_c) {
+        const bool mv = best != SC::none() && SC::positive(best, 0u, kc, kc, a.m2) && c < best_c;
+        return mv ? best_c : c;
+    }
 };
 
 // Largest i in [i0, n) with p[i] <= k, for a staged LDS prefix with
@@ -74,12 +103,20 @@ __device__ inline uint32_t member_of(const uint64_t *P, uint32_t lo, uint32_t hi
     return lo;
 }
 
-template <int G, int T, bool WEIGHTED>
+// DEC = 0: the rows only.  DEC = 1 + gain: the rows and the next level's
+// first sweep (FirstSweep); tables of at most kPackedMaxT slots then gather
+// newck[x] (destination and its new degree in one word) and keep each
+// destination's degree beside it, larger ones gather the degree per
+// destination when they score.
+template <int G, int T, bool WEIGHTED, int DEC>
 __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *__restrict__ list,
                                                       uint32_t count) {
     constexpr int GPB = kBlock / G;
+    constexpr bool PK = DEC != 0 && T <= kPackedMaxT;
+    constexpr int TK = PK ? T : 1;
     __shared__ uint32_t s_key[GPB * T];
     __shared__ uint32_t s_val[GPB * T];
+    __shared__ uint32_t s_kd[GPB * TK];   // PK: new degree of each destination
     __shared__ uint32_t s_cnt[GPB];
     __shared__ uint32_t s_pos[GPB];
     const int lane = threadIdx.x & (G - 1);
@@ -89,6 +126,7 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
     const uint32_t c = valid ? list[idx] : 0u;
     uint32_t *K = s_key + gi * T;
     uint32_t *V = s_val + gi * T;
+    uint32_t *KD = s_kd + gi * TK;
     uint64_t k0 = 0;
     uint32_t tv = 4u;
     // the community's words together: row bounds and member range
@@ -119,7 +157,7 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
         const uint64_t e0 = s_mp[gi][0], e1 = s_mp[gi][nm];
         uint32_t mi = 0;   // member of the lane's previous entry
         for (uint64_t kb = e0 + lane; kb < e1; kb += (uint64_t)G * kAggIlp) {
-            uint32_t xs[kAggIlp], ws[kAggIlp], ds[kAggIlp];
+            uint32_t xs[kAggIlp], ws[kAggIlp], ds[kAggIlp], kds[kAggIlp];
 #pragma unroll
             for (int u = 0; u < kAggIlp; ++u) {
                 const uint64_t k = kb + (uint64_t)u * G;
@@ -134,7 +172,16 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
                 }
             }
 #pragma unroll
-            for (int u = 0; u < kAggIlp; ++u) ds[u] = xs[u] != LV_EMPTY ? a.newc[xs[u]] : LV_EMPTY;
+            for (int u = 0; u < kAggIlp; ++u) {
+                if constexpr (PK) {
+                    const unsigned long long p = xs[u] != LV_EMPTY ? a.newck[xs[u]] : ~0ull;
+                    ds[u] = (uint32_t)p;
+                    kds[u] = (uint32_t)(p >> 32);
+                } else {
+                    ds[u] = xs[u] != LV_EMPTY ? a.newc[xs[u]] : LV_EMPTY;
+                    kds[u] = 0u;
+                }
+            }
             // first probes issued back to back (one LDS round trip), then the
             // rare occupied first slots probe on; a lane's entries stay in order
             uint32_t hs[kAggIlp], pv[kAggIlp];
@@ -152,7 +199,10 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
                     h = h + 1 == tv ? 0u : h + 1;
                     prev = atomicCAS(&K[h], LV_EMPTY, d);
                 }
-                if (prev == LV_EMPTY) atomicAdd(&s_cnt[gi], 1u);
+                if (prev == LV_EMPTY) {
+                    atomicAdd(&s_cnt[gi], 1u);
+                    if constexpr (PK) KD[h] = kds[u];
+                }
                 atomicAdd(&V[h], ws[u]);
             }
         }
@@ -162,15 +212,92 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
     // distinct (destination, weight) pairs in table order; the row order is not
     // part of the result (every consumer is order independent; exported graphs
     // are put in destination order on the way out)
-    if (valid) {
-        if (lane == 0) a.rcount[c] = s_cnt[gi];
-        for (uint32_t s = lane; s < tv; s += G) {
-            const uint32_t d = K[s];
-            if (d == LV_EMPTY) continue;
-            const uint32_t pos = atomicAdd(&s_pos[gi], 1u);
-            a.tcol[k0 + pos] = d;
-            a.tw[k0 + pos] = V[s];
+    if constexpr (DEC == 0) {
+        if (valid) {
+            if (lane == 0) a.rcount[c] = s_cnt[gi];
+            for (uint32_t s = lane; s < tv; s += G) {
+                const uint32_t d = K[s];
+                if (d == LV_EMPTY) continue;
+                const uint32_t pos = atomicAdd(&s_pos[gi], 1u);
+                a.tcol[k0 + pos] = d;
+                a.tw[k0 + pos] = V[s];
+            }
         }
+    } else {
+        using FS = FirstSweep<DEC>;
+        using S = typename FS::S;
+        S best = FS::SC::none();
+        uint32_t best_c = LV_EMPTY;
+        uint32_t nself = 0;
+        const uint32_t kc = valid ? a.knew[c] : 0u;
+        if (valid) {
+            if (lane == 0) a.rcount[c] = s_cnt[gi];
+            // kAggIlp slots per lane at a time: without the packed degrees
+            // their destinations' degrees are gathered together
+            for (uint32_t sb = lane; sb < tv; sb += (uint32_t)G * kAggIlp) {
+                uint32_t ds[kAggIlp], ws[kAggIlp], kds[kAggIlp];
+#pragma unroll
+                for (int u = 0; u < kAggIlp; ++u) {
+                    const uint32_t sl = sb + (uint32_t)u * G;
+                    ds[u] = sl < tv ? K[sl] : LV_EMPTY;
+                    ws[u] = sl < tv ? V[sl] : 0u;
+                    if constexpr (PK) kds[u] = sl < tv ? KD[sl] : 0u;
+                }
+                if constexpr (!PK) {
+#pragma unroll
+                    for (int u = 0; u < kAggIlp; ++u)
+                        kds[u] = ds[u] != LV_EMPTY && ds[u] != c ? a.knew[ds[u]] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < kAggIlp; ++u) {
+                    const uint32_t d = ds[u];
+                    if (d == LV_EMPTY) continue;
+                    const uint32_t pos = atomicAdd(&s_pos[gi], 1u);
+                    a.tcol[k0 + pos] = d;
+                    a.tw[k0 + pos] = ws[u];
+                    if (d == c) continue;   // the self-loop is no candidate
+                    ++nself;
+                    const S sc = FS::score(a, ws[u], kc, kds[u]);
+                    if (better_candidate(sc, d, best, best_c, false)) { best = sc; best_c = d; }
+                }
+            }
+        }
+        uint32_t considered = 0, twrite = 0;
+        if constexpr (G <= 64) {
+            group_best<G, S>(best, best_c, false);
+            const bool any = group_any<G>(nself > 0);
+            if (valid && lane == 0) {
+                considered = any;
+                const uint32_t t = FS::decide(a, c, kc, best, best_c);
+                a.dtarget[c] = t;
+                twrite = t != c;
+            }
+        } else {
+            __shared__ S r_s[kBlock / 64];
+            __shared__ uint32_t r_c[kBlock / 64];
+            __shared__ uint32_t r_n[kBlock / 64];
+            group_best<64, S>(best, best_c, false);
+            const uint32_t wn = group_any<64>(nself > 0);
+            const int w = threadIdx.x >> 6;
+            if ((threadIdx.x & 63) == 0) { r_s[w] = best; r_c[w] = best_c; r_n[w] = wn; }
+            __syncthreads();
+            if (threadIdx.x == 0 && valid) {
+                uint32_t tn = 0;
+                for (int i = 0; i < kBlock / 64; ++i) {
+                    tn += r_n[i];
+                    if (better_candidate(r_s[i], r_c[i], best, best_c, false)) {
+                        best = r_s[i];
+                        best_c = r_c[i];
+                    }
+                }
+                considered = tn > 0;
+                const uint32_t t = FS::decide(a, c, kc, best, best_c);
+                a.dtarget[c] = t;
+                twrite = t != c;
+            }
+        }
+        block_add_stats(a.dstats, ST_E_ACTIVE, nself, ST_CANDIDATES, nself, ST_CONSIDERED,
+                        considered, ST_TWRITES, twrite);
     }
 }
 
@@ -342,7 +469,14 @@ __global__ __launch_bounds__(kBlock) void k_giant_scatter(GiantCtx h) {
     }
 }
 
+// DEC != 0: every distinct destination d != c of the partition is also scored
+// as a candidate of the next level's first sweep (FirstSweep; d's new degree
+// gathered), and the partition's best and non-self count are kept for
+// k_giant_decide.
+template <int DEC>
 __global__ __launch_bounds__(kBlock) void k_giant_reduce(GiantCtx h) {
+    using FS = FirstSweep<DEC == 0 ? 1 : DEC>;
+    using S = typename FS::S;
     __shared__ SumTable t;
     const uint64_t item = h.items[blockIdx.x];
     const uint32_t p = (uint32_t)item;
@@ -357,6 +491,10 @@ __global__ __launch_bounds__(kBlock) void k_giant_reduce(GiantCtx h) {
     uint64_t width = sum_first_width(span_lo, span_hi, np);
     uint64_t lo = span_lo;
     uint32_t n_out = 0;
+    S best = FS::SC::none();
+    uint32_t best_c = LV_EMPTY;
+    uint32_t nself = 0;
+    const uint32_t kc = DEC ? h.a.knew[I.c] : 0u;
     __syncthreads();
     while (np > 0 && lo < span_hi) {
         const uint64_t rhi = span_hi - lo < width ? span_hi : lo + width;
@@ -365,11 +503,42 @@ __global__ __launch_bounds__(kBlock) void k_giant_reduce(GiantCtx h) {
             width = (rhi - lo) / 2 > 0 ? (rhi - lo) / 2 : 1u;
             continue;
         }
-        for (uint32_t i = threadIdx.x; i < nu; i += kBlock) {
-            const uint32_t s = t.used[i];
-            out[n_out + i] = ((unsigned long long)t.keys[s] << 32) | t.vals[s];
-            t.keys[s] = LV_EMPTY;
-            t.vals[s] = 0u;
+        if constexpr (DEC == 0) {
+            for (uint32_t i = threadIdx.x; i < nu; i += kBlock) {
+                const uint32_t s = t.used[i];
+                out[n_out + i] = ((unsigned long long)t.keys[s] << 32) | t.vals[s];
+                t.keys[s] = LV_EMPTY;
+                t.vals[s] = 0u;
+            }
+        } else {
+            // kGiantIlp destinations per lane, their degrees gathered together
+            for (uint32_t ib = threadIdx.x; ib < nu; ib += kBlock * kGiantIlp) {
+                uint32_t ds[kGiantIlp], ws[kGiantIlp], kds[kGiantIlp];
+#pragma unroll
+                for (int u = 0; u < kGiantIlp; ++u) {
+                    const uint32_t i = ib + (uint32_t)u * kBlock;
+                    ds[u] = LV_EMPTY;
+                    ws[u] = 0u;
+                    if (i < nu) {
+                        const uint32_t s = t.used[i];
+                        ds[u] = t.keys[s];
+                        ws[u] = t.vals[s];
+                        out[n_out + i] = ((unsigned long long)ds[u] << 32) | ws[u];
+                        t.keys[s] = LV_EMPTY;
+                        t.vals[s] = 0u;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kGiantIlp; ++u)
+                    kds[u] = ds[u] != LV_EMPTY && ds[u] != I.c ? h.a.knew[ds[u]] : 0u;
+#pragma unroll
+                for (int u = 0; u < kGiantIlp; ++u) {
+                    if (ds[u] == LV_EMPTY || ds[u] == I.c) continue;
+                    ++nself;
+                    const S sc = FS::score(h.a, ws[u], kc, kds[u]);
+                    if (better_candidate(sc, ds[u], best, best_c, false)) { best = sc; best_c = ds[u]; }
+                }
+            }
         }
         sum_next_piece(t);
         n_out += nu;
@@ -380,6 +549,77 @@ __global__ __launch_bounds__(kBlock) void k_giant_reduce(GiantCtx h) {
         h.dcnt[I.pbase + p] = n_out;
         if (n_out) atomicAdd(&h.a.rcount[I.c], n_out);
     }
+    if constexpr (DEC != 0) {
+        __shared__ S r_s[kBlock / 64];
+        __shared__ uint32_t r_c[kBlock / 64];
+        __shared__ uint32_t r_n[kBlock / 64];
+        group_best<64, S>(best, best_c, false);
+        const uint32_t wn = group_sum_u32<64>(nself);
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) { r_s[w] = best; r_c[w] = best_c; r_n[w] = wn; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tn = 0;
+            for (int i = 0; i < kBlock / 64; ++i) {
+                tn += r_n[i];
+                if (better_candidate(r_s[i], r_c[i], best, best_c, false)) {
+                    best = r_s[i];
+                    best_c = r_c[i];
+                }
+            }
+            if constexpr (DEC == 1) h.dpart_s[I.pbase + p] = best;
+            else h.dpart_s[I.pbase + p] = (int64_t)__float_as_int(best);
+            h.dpart_c[I.pbase + p] = best_c;
+            h.dpart_n[I.pbase + p] = tn;
+        }
+    }
+}
+
+// One workgroup per giant community: its partitions' bests reduced, the next
+// level's first-sweep decision and statistics.
+template <int DEC>
+__global__ __launch_bounds__(kBlock) void k_giant_decide(GiantCtx h) {
+    using FS = FirstSweep<DEC>;
+    using S = typename FS::S;
+    const GiantInfo I = h.info[blockIdx.x];
+    const uint32_t np = 1u << I.lgp;
+    S best = FS::SC::none();
+    uint32_t best_c = LV_EMPTY;
+    uint32_t nself = 0;
+    for (uint32_t q = threadIdx.x; q < np; q += kBlock) {
+        const int64_t raw = h.dpart_s[I.pbase + q];
+        S sc;
+        if constexpr (DEC == 1) sc = raw;
+        else sc = __int_as_float((int)raw);
+        const uint32_t cq = h.dpart_c[I.pbase + q];
+        nself += h.dpart_n[I.pbase + q];
+        if (better_candidate(sc, cq, best, best_c, false)) { best = sc; best_c = cq; }
+    }
+    __shared__ S r_s[kBlock / 64];
+    __shared__ uint32_t r_c[kBlock / 64];
+    __shared__ uint32_t r_n[kBlock / 64];
+    group_best<64, S>(best, best_c, false);
+    const uint32_t wn = group_sum_u32<64>(nself);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { r_s[w] = best; r_c[w] = best_c; r_n[w] = wn; }
+    __syncthreads();
+    uint32_t tn = 0, considered = 0, twrite = 0;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < kBlock / 64; ++i) {
+            tn += r_n[i];
+            if (better_candidate(r_s[i], r_c[i], best, best_c, false)) {
+                best = r_s[i];
+                best_c = r_c[i];
+            }
+        }
+        const uint32_t c = I.c;
+        const uint32_t t = FS::decide(h.a, c, h.a.knew[c], best, best_c);
+        h.a.dtarget[c] = t;
+        considered = tn > 0;
+        twrite = t != c;
+    }
+    block_add_stats(h.a.dstats, ST_E_ACTIVE, tn, ST_CANDIDATES, tn, ST_CONSIDERED, considered,
+                    ST_TWRITES, twrite);
 }
 
 __global__ __launch_bounds__(kBlock) void k_giant_copy(GiantCtx h, const uint64_t *out_row,
@@ -477,13 +717,16 @@ __global__ __launch_bounds__(kBlock) void k_agg_copy(AggCtx a, const uint32_t *_
 }
 
 __global__ void k_newc(const uint32_t *comm, const uint32_t *map, uint32_t n, uint32_t *newc,
-                       uint64_t *key, uint32_t *val) {
+                       uint64_t *key, uint32_t *val, const uint32_t *tot,
+                       unsigned long long *newck) {
     for (size_t v = blockIdx.x * (size_t)blockDim.x + threadIdx.x; v < n;
          v += (size_t)gridDim.x * blockDim.x) {
-        const uint32_t c = renumbered(map, comm[v]);
+        const uint32_t cv = comm[v];
+        const uint32_t c = renumbered(map, cv);
         newc[v] = c;
         key[v] = c;
         val[v] = (uint32_t)v;
+        if (newck) newck[v] = (unsigned long long)c | ((unsigned long long)tot[cv] << 32);
     }
 }
 
@@ -517,10 +760,10 @@ __global__ void k_set_u64(uint64_t *p, uint64_t v) { *p = v; }
 }  // namespace
 
 void agg_newc(const uint32_t *comm, const uint32_t *map, uint32_t n, uint32_t *newc, uint64_t *key,
-              uint32_t *val, hipStream_t s) {
+              uint32_t *val, const uint32_t *tot, unsigned long long *newck, hipStream_t s) {
     if (!n) return;
     hipLaunchKernelGGL(k_newc, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, comm, map, n,
-                       newc, key, val);
+                       newc, key, val, tot, newck);
 }
 
 void agg_members(const uint64_t *skey, const uint32_t *members, const uint64_t *row, uint32_t n,
@@ -552,6 +795,8 @@ static AggCtx ctx_of(const AggArgs &x) {
     a.row = x.g.row; a.col = x.g.col; a.wt = x.g.wt;
     a.newc = x.newc; a.members = x.members; a.mstart = x.mstart; a.P = x.P; a.crow = x.crow;
     a.tcol = x.tcol; a.tw = x.tw; a.rcount = x.rcount;
+    a.m2 = x.m2; a.tw_total = x.tw_total; a.newck = x.newck; a.knew = x.knew;
+    a.dtarget = x.dtarget; a.dstats = x.dstats;
     return a;
 }
 
@@ -564,11 +809,17 @@ void agg_tables(const AggArgs &x, uint32_t *const *lists, const uint32_t *counts
         const uint32_t cnt = counts[b];
         if (!cnt) continue;
         const uint32_t *L = lists[b];
-#define LV_AGG(GG, TT)                                                                          \
+#define LV_AGG_D(GG, TT, DD)                                                                    \
     do {                                                                                        \
         const dim3 grid((cnt + kBlock / (GG) - 1) / (kBlock / (GG)));                           \
-        if (wtd) hipLaunchKernelGGL((k_agg_table<GG, TT, true>), grid, dim3(kBlock), 0, s, a, L, cnt); \
-        else hipLaunchKernelGGL((k_agg_table<GG, TT, false>), grid, dim3(kBlock), 0, s, a, L, cnt); \
+        if (wtd) hipLaunchKernelGGL((k_agg_table<GG, TT, true, DD>), grid, dim3(kBlock), 0, s, a, L, cnt); \
+        else hipLaunchKernelGGL((k_agg_table<GG, TT, false, DD>), grid, dim3(kBlock), 0, s, a, L, cnt); \
+    } while (0)
+#define LV_AGG(GG, TT)                                                                          \
+    do {                                                                                        \
+        if (x.dgain == 0) LV_AGG_D(GG, TT, 1);                                                  \
+        else if (x.dgain == 1) LV_AGG_D(GG, TT, 2);                                             \
+        else LV_AGG_D(GG, TT, 0);                                                               \
     } while (0)
         // tables of table_slots(the bucket's largest community), the slots
         // of the best-move tables of the same degree bound
@@ -583,6 +834,7 @@ void agg_tables(const AggArgs &x, uint32_t *const *lists, const uint32_t *counts
             default: LV_AGG(256, kTableT[7]); break;
         }
 #undef LV_AGG
+#undef LV_AGG_D
     }
 }
 
@@ -602,7 +854,15 @@ void agg_giant(const AggArgs &x, const GiantArgs &g, uint32_t n_parts, hipStream
     else hipLaunchKernelGGL(k_giant_gather<false>, dim3(g.n_tiles), dim3(kBlock), lds1, s, h);
     scan_u32(g.pcnt, g.poff, n_parts, false, g.scan_tmp, s);
     hipLaunchKernelGGL(k_giant_scatter, dim3(g.n_tiles), dim3(kBlock), 2 * lds1, s, h);
-    hipLaunchKernelGGL(k_giant_reduce, dim3(g.n_items), dim3(kBlock), 0, s, h);
+    if (x.dgain == 0) {
+        hipLaunchKernelGGL(k_giant_reduce<1>, dim3(g.n_items), dim3(kBlock), 0, s, h);
+        hipLaunchKernelGGL(k_giant_decide<1>, dim3(g.n_giants), dim3(kBlock), 0, s, h);
+    } else if (x.dgain == 1) {
+        hipLaunchKernelGGL(k_giant_reduce<2>, dim3(g.n_items), dim3(kBlock), 0, s, h);
+        hipLaunchKernelGGL(k_giant_decide<2>, dim3(g.n_giants), dim3(kBlock), 0, s, h);
+    } else {
+        hipLaunchKernelGGL(k_giant_reduce<0>, dim3(g.n_items), dim3(kBlock), 0, s, h);
+    }
     scan_u32(g.dcnt, g.doff, n_parts, false, g.scan_tmp, s);
 }
 

@@ -229,6 +229,19 @@ __device__ inline void block_add_stats(unsigned long long *stats, int st0, U a0,
     __syncthreads();
 }
 
+// Whether any lane of each aligned group of G lanes (G <= 64) has `x` set:
+// one ballot, no cross-lane arithmetic.
+template <int G>
+__device__ inline bool group_any(bool x) {
+    const unsigned long long b = __ballot(x);
+    if constexpr (G == 64) {
+        return b != 0;
+    } else {
+        const uint32_t base = (threadIdx.x & 63) & ~(uint32_t)(G - 1);
+        return ((b >> base) & ((1ull << G) - 1)) != 0;
+    }
+}
+
 // Adds val to base[key] for the lanes with `has`.  A wave whose contributing
 // lanes share one key (the usual case: lists and tiles follow vertex order and
 // a graph of a batch is a vertex range) issues one atomic.  Every lane of the

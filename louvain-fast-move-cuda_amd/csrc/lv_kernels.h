@@ -282,10 +282,24 @@ struct AggArgs {
     uint32_t *tcol;            // [E] staging
     uint32_t *tw;              // [E] staging
     uint32_t *rcount;          // [n_new], zeroed
+    // The next level's first sweep -- a fast sweep at the identity partition
+    // over the contracted rows -- decided while the rows are built (dgain: 0
+    // exact gains, 1 the reference's f32 rounding; -1 not asked).  Every new
+    // vertex c gets dtarget[c] (its best neighbour under the fast sweep's
+    // rule, or c), and the sweep's best-move statistics go to dstats.
+    int dgain = -1;
+    int64_t m2 = 0;                                 // 2m of the graph
+    double tw_total = 0;                            // m
+    const unsigned long long *newck = nullptr;      // [n] newc[v] | new degree of newc[v] << 32
+    const uint32_t *knew = nullptr;                 // [n_new] new degrees
+    uint32_t *dtarget = nullptr;                    // [n_new]
+    unsigned long long *dstats = nullptr;           // one statistics section, zeroed
 };
-// newc[v] = map[comm[v]]-1; key/val prepared for the vertex sort (key = newc, val = v)
+// newc[v] = map[comm[v]]-1; key/val prepared for the vertex sort (key = newc,
+// val = v); with newck, also newc[v] | tot[comm[v]] << 32 (tot[comm[v]] is the
+// new degree of newc[v])
 void agg_newc(const uint32_t *comm, const uint32_t *map, uint32_t n, uint32_t *newc, uint64_t *key,
-              uint32_t *val, hipStream_t s);
+              uint32_t *val, const uint32_t *tot, unsigned long long *newck, hipStream_t s);
 // member starts (mstart[n_new] = n) and member degrees (mdeg[n] = 0)
 void agg_members(const uint64_t *skey, const uint32_t *members, const uint64_t *row, uint32_t n,
                  uint32_t n_new, uint32_t *mstart, uint64_t *mdeg, hipStream_t s);
@@ -325,6 +339,12 @@ struct GiantArgs {
     uint32_t *base;                 // [(tile, partition) runs] start of a tile's run
     uint32_t *scan_tmp;             // scan scratch for the partition counts
     uint32_t max_parts;             // largest partition count of the plan (LDS per workgroup)
+    uint32_t n_giants;
+    // with AggArgs::dgain >= 0: each partition's best candidate and non-self
+    // destinations, reduced per giant afterwards
+    int64_t *dpart_s;               // [partitions]
+    uint32_t *dpart_c;              // [partitions]
+    uint32_t *dpart_n;              // [partitions]
 };
 // distinct (destination, weight) pairs of every giant community and its
 // rcount; then, once the new offsets are known, its row copied to place

@@ -459,19 +459,6 @@ __global__ __launch_bounds__(kBlock) void k_identity_parts(LevelView g,
     block_add_stats(stats, ST_DSIGMA, sig, ST_SUMSQ, sq, -1, 0);
 }
 
-// Whether any lane of each aligned group of G lanes (G <= 64) has `x` set:
-// one ballot, no cross-lane arithmetic.
-template <int G>
-__device__ inline bool group_any(bool x) {
-    const unsigned long long b = __ballot(x);
-    if constexpr (G == 64) {
-        return b != 0;
-    } else {
-        const uint32_t base = (threadIdx.x & 63) & ~(uint32_t)(G - 1);
-        return ((b >> base) & ((1ull << G) - 1)) != 0;
-    }
-}
-
 // ---------------------------------------------------------- best-move pass
 
 struct MoveCtx {

@@ -42,7 +42,7 @@ class Params(C.Structure):
 
 class SweepRec(C.Structure):
     _fields_ = [("level", C.c_int32), ("sweep", C.c_int32), ("kind", C.c_int32),
-                ("_pad", C.c_int32), ("n", C.c_uint32), ("n_active", C.c_uint32),
+                ("pre_decided", C.c_int32), ("n", C.c_uint32), ("n_active", C.c_uint32),
                 ("n_moved", C.c_uint32), ("n_considered", C.c_uint32),
                 ("E", C.c_uint64), ("e_active", C.c_uint64), ("sigma_in", C.c_int64),
                 ("sum_tot_sq", C.c_uint64), ("q", C.c_double), ("comm_hash", C.c_uint64),
