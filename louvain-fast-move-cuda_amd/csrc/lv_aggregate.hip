@@ -114,6 +114,9 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
     constexpr int GPB = kBlock / G;
     constexpr bool PK = DEC != 0 && T <= kPackedMaxT;
     constexpr int TK = PK ? T : 1;
+    // members staged at a time: G, or fewer where the degree array would
+    // otherwise cost a workgroup per CU (bucket 5 and 6 tables at 8 per CU)
+    constexpr int SM = !PK ? G : (G >= 256 ? 64 : (G >= 64 ? 16 : G));
     __shared__ uint32_t s_key[GPB * T];
     __shared__ uint32_t s_val[GPB * T];
     __shared__ uint32_t s_kd[GPB * TK];   // PK: new degree of each destination
@@ -142,11 +145,11 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
     if (lane == 0) { s_cnt[gi] = 0u; s_pos[gi] = 0u; }
     // a community's table is its group's alone
     group_barrier<G>();
-    // members staged G at a time: row start and first virtual entry in LDS
-    __shared__ uint64_t s_mrow[GPB][G];
-    __shared__ uint64_t s_mp[GPB][G + 1];
-    for (uint32_t mb = m0; mb < m1; mb += G) {
-        const uint32_t nm = m1 - mb < (uint32_t)G ? m1 - mb : (uint32_t)G;
+    // members staged SM at a time: row start and first virtual entry in LDS
+    __shared__ uint64_t s_mrow[GPB][SM];
+    __shared__ uint64_t s_mp[GPB][SM + 1];
+    for (uint32_t mb = m0; mb < m1; mb += SM) {
+        const uint32_t nm = m1 - mb < (uint32_t)SM ? m1 - mb : (uint32_t)SM;
         if ((uint32_t)lane < nm) {
             const uint32_t v = a.members[mb + lane];
             s_mrow[gi][lane] = a.row[v];

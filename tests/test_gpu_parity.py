@@ -473,6 +473,37 @@ def test_phase_api_matches_louvain(fixture_path):
     assert c.modularity == ref["modularity"]
 
 
+@pytest.mark.parametrize("gain", GAINS, ids=lambda x: x[0])
+def test_first_sweep_decided_by_contraction(gain):
+    """lv_community_run decides each level's first (fast) sweep while the
+    contraction builds the level (LDS-table rows and giant partitions alike):
+    those sweeps are flagged pre_decided, and the run equals the phase API's,
+    whose levels run that sweep's best-move pass (R-MAT scale 18: hub rows and
+    giant communities at levels 1-2), and the oracle's trace."""
+    ea, eb, n = L.gen_rmat(18, 16, 0.57, 0.19, 0.19, seed=18)
+    say(f"first sweep decided by the contraction, {gain[0]}")
+    g = L.Graph.from_edges(ea, eb)
+    prm = L.params(gain=gain[1])
+    a = g.louvain(prm, hash_trace=True)
+    assert a["levels"] >= 3
+    for sw in a["sweeps"]:
+        assert sw["pre_decided"] == (1 if sw["sweep"] == 0 and sw["level"] > 0 else 0), sw
+    b = O.Graph.from_edges(ea, eb).louvain(O.params(gain=gain[2]))
+    compare_runs(a, b, f"rmat18 first sweep {gain[0]}")
+    c = L.Community(g)
+    while True:
+        old = c.modularity
+        c.optimization_phase(prm)
+        cont = c.modularity - old > prm.level_threshold
+        c.aggregation_phase(cont, prm)
+        if not cont:
+            break
+    best, nc = c.best_communities(g.n)
+    assert np.array_equal(best, a["best"]) and nc == a["n_comm"]
+    assert c.modularity == a["modularity"]
+    c.close()
+
+
 def test_chunk_emulation_early_return_snapshot():
     """An empty first chunk stops the fast/sort sweeps but not the hash sweep."""
     n = 200
