@@ -88,7 +88,8 @@ void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
         hub_slot_of.ensure(n);
         fill_u32(hub_slot_of.data(), LV_EMPTY, n, s);
         hub_ent_off.ensure(n_hubs);
-        hub_cbuf.ensure(hub_entries);
+        hub_stage.ensure(hub_entries);
+        hub_ccnt.ensure((size_t)n_hubs * kHubChunks);
         hub_pair.ensure(hub_entries);
         hub_cnt.ensure((size_t)n_hubs * kHubParts);
         hub_base.ensure((size_t)n_hubs * kHubChunks * kHubParts);
@@ -127,7 +128,8 @@ HubTables SweepState::hub_tables() {
     HubTables h;
     h.slot_of = hub_slot_of.data();
     h.ent_off = hub_ent_off.data();
-    h.cbuf = hub_cbuf.data();
+    h.stage = hub_stage.data();
+    h.ccnt = hub_ccnt.data();
     h.pair = hub_pair.data();
     h.cnt = hub_cnt.data();
     h.base = hub_base.data();

@@ -111,7 +111,8 @@ struct SweepState {
     uint32_t n_hubs = 0;
     DeviceArray<uint32_t> hub_slot_of;
     DeviceArray<uint64_t> hub_ent_off;
-    DeviceArray<uint32_t> hub_cbuf;
+    DeviceArray<unsigned long long> hub_stage;
+    DeviceArray<uint32_t> hub_ccnt;
     DeviceArray<unsigned long long> hub_pair;
     DeviceArray<uint32_t> hub_cnt;
     DeviceArray<uint32_t> hub_base;

@@ -324,18 +324,6 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
 constexpr int kGiantIlp = 8;
 constexpr uint32_t kGiantCache = 1024;                 // destinations cached per tile
 
-// Appends `pair` (lanes with `emit`) at out[*n ...]: one LDS atomic per wave.
-__device__ inline void wave_append(bool emit, unsigned long long pair, uint32_t *n,
-                                   unsigned long long *out) {
-    const unsigned long long m = __ballot(emit);
-    if (m == 0) return;
-    const int leader = __ffsll((long long)m) - 1;
-    uint32_t base = 0;
-    if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(n, (uint32_t)__popcll(m));
-    base = (uint32_t)__shfl((int)base, leader);
-    if (emit) out[base + (uint32_t)__popcll(m & lanemask_lt())] = pair;
-}
-
 struct GiantCtx : GiantArgs {
     AggCtx a;
 };

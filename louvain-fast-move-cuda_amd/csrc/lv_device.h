@@ -242,6 +242,19 @@ __device__ inline bool group_any(bool x) {
     }
 }
 
+// Appends `pair` (lanes with `emit`) at out[*n ...] (n in LDS): one LDS atomic
+// per wave.  Lanes of a wave that are active call it together.
+__device__ inline void wave_append(bool emit, unsigned long long pair, uint32_t *n,
+                                   unsigned long long *out) {
+    const unsigned long long m = __ballot(emit);
+    if (m == 0) return;
+    const int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(n, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, leader);
+    if (emit) out[base + (uint32_t)__popcll(m & lanemask_lt())] = pair;
+}
+
 // Adds val to base[key] for the lanes with `has`.  A wave whose contributing
 // lanes share one key (the usual case: lists and tiles follow vertex order and
 // a graph of a batch is a vertex range) issues one atomic.  Every lane of the

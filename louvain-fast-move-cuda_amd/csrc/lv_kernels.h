@@ -107,11 +107,13 @@ struct LevelView {
 };
 
 // Hub rows (> 4096 entries) are reduced by community hash partition: hub h
-// owns entries [ent_off[h], ent_off[h] + degree) of cbuf and pair.
+// owns entries [ent_off[h], ent_off[h] + degree) of stage and pair.
 struct HubTables {
     uint32_t *slot_of;        // [n] hub slot of a vertex (LV_EMPTY if not a hub)
-    uint64_t *ent_off;        // [n_hubs] offset of the hub's entries in cbuf / pair
-    uint32_t *cbuf;           // [hub entries] community of each entry (LV_EMPTY: self-loop)
+    uint64_t *ent_off;        // [n_hubs] offset of the hub's entries in stage / pair
+    unsigned long long *stage;  // [hub entries] (community << 32) | weight pairs of each
+                                // chunk, packed at the start of the chunk's range
+    uint32_t *ccnt;           // [n_hubs * kHubChunks] pairs each chunk wrote
     unsigned long long *pair; // [hub entries] (community << 32) | weight in partition order
     uint32_t *cnt;            // [n_hubs * kHubParts] entries per partition, zero between sweeps
     uint32_t *base;           // [n_hubs * kHubChunks * kHubParts] start of (chunk, partition)

@@ -869,11 +869,12 @@ __global__ __launch_bounds__(kBlock) void k_move_fast(MoveCtx a, const uint32_t 
 // A hub row is cut into kHubChunks contiguous chunks (one workgroup each) and
 // its neighbour communities into kHubParts partitions by the top bits of a
 // bijective mix of the community id.  k_hub_count gathers the community of
-// every entry into the hub's c-buffer, sums the weight towards the hub's own
-// community, reserves each (chunk, partition) run and posts the hub's reduce
-// groups (one per kHubGroupEntries entries, each a run of consecutive
-// partitions); k_hub_scatter packs the (community, weight) pairs in partition
-// order; k_hub_reduce, a persistent grid over the posted groups, sums a
+// every entry, sums the weight towards the hub's own community, folds
+// repeats of the communities it caches, writes the chunk's (community,
+// weight) pairs packed at the start of the chunk's range of the stage buffer,
+// reserves each (chunk, partition) run and posts the hub's reduce groups (one
+// per kHubGroupEntries entries, each a run of consecutive partitions);
+// k_hub_scatter moves a chunk's pairs to partition order; k_hub_reduce, a persistent grid over the posted groups, sums a
 // group's weights per community in an LDS table and scores them; k_hub_decide
 // reduces the groups' bests.  Every entry costs streaming reads and writes and
 // LDS atomics only; hub rows see almost only distinct communities, so a
@@ -923,46 +924,79 @@ __device__ inline bool hub_runs(const HubCtx &a, const uint32_t *list, uint32_t 
     return v < a.m.v_end && vertex_graph(a.m.bt, v, a.m.m2, a.m.tw, m2, tw);
 }
 
+// Communities cached per chunk workgroup: a repeat adds its weight in LDS, so
+// a row whose neighbours share few communities (hubs of contracted levels)
+// passes few pairs on to the scatter and the reduce.
+constexpr uint32_t kHubCache = 1024;
+
 template <bool WEIGHTED>
 __global__ __launch_bounds__(kBlock) void k_hub_count(HubCtx a, const uint32_t *__restrict__ list,
                                                       uint32_t count) {
     __shared__ uint32_t hist[kHubParts];
     __shared__ uint64_t r_self[kBlock / 64];
+    __shared__ uint32_t c_key[kHubCache];
+    __shared__ uint32_t c_val[kHubCache];
+    __shared__ uint32_t s_n;   // pairs written so far
     const uint32_t q = blockIdx.x % kHubChunks;
     if (threadIdx.x < kHubParts) hist[threadIdx.x] = 0;
+    for (uint32_t i = threadIdx.x; i < kHubCache; i += kBlock) { c_key[i] = LV_EMPTY; c_val[i] = 0u; }
+    if (threadIdx.x == 0) s_n = 0u;
     __syncthreads();
     uint32_t v = 0, slot = 0;
     int64_t m2;
     double tw;
     const bool run = hub_runs(a, list, count, blockIdx.x / kHubChunks, v, m2, tw);
     uint64_t nself = 0, selfw = 0;
+    unsigned long long *out = nullptr;
     if (run) {
         slot = a.h.slot_of[v];
         const uint32_t own = a.m.comm[v];
         const uint64_t beg = a.m.row[v], end = a.m.row[v + 1];
-        uint32_t *cbuf = a.h.cbuf + a.h.ent_off[slot];
         uint64_t cb, ce;
         hub_chunk(beg, end, q, cb, ce);
+        // the chunk's range of the stage buffer: at most one pair per entry
+        out = a.h.stage + a.h.ent_off[slot] + (cb - beg);
         for (uint64_t jb = cb + threadIdx.x; jb < ce; jb += (uint64_t)kBlock * kIlp) {
-            uint32_t xs[kIlp], cs[kIlp];
+            uint32_t xs[kIlp], cs[kIlp], ws[kIlp];
 #pragma unroll
             for (int u = 0; u < kIlp; ++u) {
                 const uint64_t j = jb + (uint64_t)u * kBlock;
                 xs[u] = j < ce ? stream_load(&a.m.col[j]) : v;
+                ws[u] = (WEIGHTED && j < ce) ? stream_load(&a.m.wt[j]) : 1u;
             }
 #pragma unroll
             for (int u = 0; u < kIlp; ++u) cs[u] = xs[u] != v ? a.m.comm[xs[u]] : LV_EMPTY;
 #pragma unroll
             for (int u = 0; u < kIlp; ++u) {
-                const uint64_t j = jb + (uint64_t)u * kBlock;
-                if (j >= ce) continue;
-                cbuf[j - beg] = cs[u];
-                if (xs[u] == v) continue;
-                ++nself;
-                if (cs[u] == own) selfw += WEIGHTED ? a.m.wt[j] : 1u;
-                else atomicAdd(&hist[hub_mix(cs[u]) >> kHubSubBits], 1u);
+                const uint32_t c = cs[u];
+                bool emit = false;
+                if (c != LV_EMPTY) {
+                    ++nself;
+                    if (c == own) {
+                        selfw += ws[u];
+                    } else {
+                        const uint32_t k = hub_mix(c);
+                        const uint32_t cs_ = k & (kHubCache - 1);
+                        uint32_t cur = c_key[cs_];
+                        if (cur == LV_EMPTY) cur = atomicCAS(&c_key[cs_], LV_EMPTY, c);
+                        if (cur == LV_EMPTY || cur == c) {   // cached here, or a repeat
+                            atomicAdd(&c_val[cs_], ws[u]);
+                        } else {
+                            emit = true;
+                            atomicAdd(&hist[k >> kHubSubBits], 1u);
+                        }
+                    }
+                }
+                wave_append(emit, ((unsigned long long)c << 32) | ws[u], &s_n, out);
             }
         }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kHubCache; i += kBlock) {   // uniform trip count
+        const uint32_t c = c_key[i];
+        const bool emit = c != LV_EMPTY;
+        if (emit) atomicAdd(&hist[hub_mix(c) >> kHubSubBits], 1u);
+        wave_append(emit, ((unsigned long long)c << 32) | c_val[i], &s_n, out);
     }
     const uint64_t wsum = group_sum_u64<64>(selfw);
     if ((threadIdx.x & 63) == 0) r_self[threadIdx.x >> 6] = wsum;
@@ -975,6 +1009,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_count(HubCtx a, const uint32_t *
                     atomicAdd(&a.h.cnt[(size_t)slot * kHubParts + threadIdx.x], h);
         }
         if (threadIdx.x == 0) {
+            a.h.ccnt[(size_t)slot * kHubChunks + q] = s_n;
             uint64_t t = 0;
             for (int i = 0; i < kBlock / 64; ++i) t += r_self[i];
             if (t) atomicAdd(&a.h.self[slot], (uint32_t)t);
@@ -990,9 +1025,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_count(HubCtx a, const uint32_t *
     block_add_stats(a.m.stats, ST_E_ACTIVE, nself, -1, 0, -1, 0);
 }
 
-template <bool WEIGHTED>
-__global__ __launch_bounds__(kBlock) void k_hub_scatter(HubCtx a,
-                                                        const uint32_t *__restrict__ list,
+__global__ __launch_bounds__(kBlock) void k_hub_scatter(HubCtx a, const uint32_t *__restrict__ list,
                                                         uint32_t count) {
     __shared__ uint32_t s_off[kHubParts];
     __shared__ uint32_t s_rank[kHubParts];
@@ -1021,27 +1054,26 @@ __global__ __launch_bounds__(kBlock) void k_hub_scatter(HubCtx a,
         }
     }
     __syncthreads();
-    const uint32_t own = a.m.comm[v];
     const uint64_t beg = a.m.row[v], end = a.m.row[v + 1];
     const uint64_t eo = a.h.ent_off[slot];
-    const uint32_t *cbuf = a.h.cbuf + eo;
-    unsigned long long *pair = a.h.pair + eo;
     uint64_t cb, ce;
     hub_chunk(beg, end, q, cb, ce);
-    for (uint64_t jb = cb + threadIdx.x; jb < ce; jb += (uint64_t)kBlock * kIlp) {
-        uint32_t cs[kIlp], ws[kIlp];
+    const unsigned long long *in = a.h.stage + eo + (cb - beg);
+    const uint32_t n = a.h.ccnt[(size_t)slot * kHubChunks + q];
+    unsigned long long *pair = a.h.pair + eo;
+    for (uint32_t ib = threadIdx.x; ib < n; ib += kBlock * kIlp) {
+        unsigned long long pr[kIlp];
 #pragma unroll
         for (int u = 0; u < kIlp; ++u) {
-            const uint64_t j = jb + (uint64_t)u * kBlock;
-            cs[u] = j < ce ? __builtin_nontemporal_load(&cbuf[j - beg]) : LV_EMPTY;
-            ws[u] = (WEIGHTED && j < ce) ? stream_load(&a.m.wt[j]) : 1u;
+            const uint32_t i = ib + (uint32_t)u * kBlock;
+            pr[u] = i < n ? __builtin_nontemporal_load(&in[i]) : 0ull;
         }
 #pragma unroll
         for (int u = 0; u < kIlp; ++u) {
-            if (cs[u] == LV_EMPTY || cs[u] == own) continue;
-            const uint32_t p = hub_mix(cs[u]) >> kHubSubBits;
+            if (ib + (uint32_t)u * kBlock >= n) continue;
+            const uint32_t p = hub_mix((uint32_t)(pr[u] >> 32)) >> kHubSubBits;
             const uint32_t r = atomicAdd(&s_rank[p], 1u);
-            pair[s_off[p] + r] = ((unsigned long long)cs[u] << 32) | ws[u];
+            pair[s_off[p] + r] = pr[u];
         }
     }
 }
@@ -1973,10 +2005,10 @@ void move_pass(const SweepArgs &a, const HubTables &hub, uint32_t *const *lists,
                     const dim3 grid(cnt * kHubChunks), blk(kBlock);
                     if (wtd) {
                         hipLaunchKernelGGL(k_hub_count<true>, grid, blk, 0, s, h, L, cnt);
-                        hipLaunchKernelGGL(k_hub_scatter<true>, grid, blk, 0, s, h, L, cnt);
+                        hipLaunchKernelGGL(k_hub_scatter, grid, blk, 0, s, h, L, cnt);
                     } else {
                         hipLaunchKernelGGL(k_hub_count<false>, grid, blk, 0, s, h, L, cnt);
-                        hipLaunchKernelGGL(k_hub_scatter<false>, grid, blk, 0, s, h, L, cnt);
+                        hipLaunchKernelGGL(k_hub_scatter, grid, blk, 0, s, h, L, cnt);
                     }
                     // persistent: the items posted by k_hub_count
                     const dim3 rgrid(std::min<uint32_t>(cnt * kHubParts, kHubReduceBlocks));
