@@ -314,6 +314,7 @@ void GraphDevice::build_coo(hipStream_t s) {
 void Community::reset() {
     graph = GraphDevice::view_of(graph0);
     state->next_level_buf = 0;
+    state->pre_valid = false;   // decisions kept for a level of an earlier run
     level = 0;
     n_of_best_communities = graph.n_nodes;
     iota_u32(best_communities.data(), graph.n_nodes, stream());
