@@ -190,7 +190,10 @@ void GraphBatch::run(const lv_params &p, lv_batch_result *r) {
             const int kind = OptimizationPhase::sweep_kind(p.mode, exec);
             LV_HIP(hipMemcpyAsync(B.live_dev.data(), B.host_live, G, hipMemcpyHostToDevice, s));
             fill_u64(B.acc.data(), 0ull, 2 * (size_t)G, s);
-            launch_sweep(C, p, kind, true, C.graph.n_nodes);
+            // a level's first sweep decided by the contraction that built it
+            const bool pre = exec == 0 && st.pre_valid && kind == LV_SWEEP_FAST;
+            st.pre_valid = false;
+            launch_sweep(C, p, kind, true, C.graph.n_nodes, pre);
             LV_HIP(hipMemcpyAsync(B.host_acc, B.acc.data(), 2 * (size_t)G * sizeof(unsigned long long),
                                   hipMemcpyDeviceToHost, s));
             wait_stream(s);
@@ -236,7 +239,21 @@ void GraphBatch::run(const lv_params &p, lv_batch_result *r) {
         DeviceArray<uint32_t> &gn = B.gid_buf[B.next_gid];
         gn.ensure(std::max<size_t>(n_new, 1));
         batch_next_gid(C.communities_weight.data(), map, B.gid, n, gn.data(), s);
+        // the contraction decides the next level's first sweep for the graphs
+        // that go on (their live flags, as that sweep will upload them)
+        const bool decide = !p.chunk_emulation;
+        if (decide) {
+            for (uint32_t g = 0; g < G; ++g) B.host_live[g] = done[g] ? 0 : 1;
+            LV_HIP(hipMemcpyAsync(B.live_dev.data(), B.host_live, G, hipMemcpyHostToDevice, s));
+            BatchView v = B.view();
+            v.gid = gn.data();
+            st.first_sweep_bt = v;
+            st.first_sweep_gain = p.gain;
+        }
         AggregationPhase::contract(C, map, n_new);
+        st.first_sweep_gain = -1;
+        st.first_sweep_bt = BatchView();
+        st.pre_valid = decide;
         C.level += 1;
         B.gid = gn.data();
         B.next_gid ^= 1;

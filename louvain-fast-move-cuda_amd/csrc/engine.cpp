@@ -928,9 +928,9 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
     uint64_t *vk0 = ws.get<uint64_t>(W_VK0, n), *vk1 = ws.get<uint64_t>(W_VK1, n);
     uint32_t *members = ws.get<uint32_t>(W_MEMBERS, n), *vv1 = ws.get<uint32_t>(W_VV1, n);
     // the next level's first sweep decided here when the caller asks for it
-    // (single graphs; ModularityAlgorithms::run)
+    // (ModularityAlgorithms::run, batch runs)
     SweepState &st = *c.state;
-    const int dgain = c.batch ? -1 : st.first_sweep_gain;
+    const int dgain = st.first_sweep_gain;
     unsigned long long *newck = dgain >= 0 ? ws.get<unsigned long long>(W_NEWCK, n) : nullptr;
     agg_newc(c.communities.data(), map, n, newc, vk0, members, c.communities_weight.data(), newck,
              s);
@@ -981,6 +981,7 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
         st.pre_stats.ensure(kSection);
         x.dtarget = st.pre_target.data();
         x.dstats = st.pre_stats.data();
+        x.dbt = st.first_sweep_bt;
         fill_u64(st.pre_stats.data(), 0ull, kSection, s);
     }
     // 4. giant communities: their entries in destination hash partitions,

@@ -145,6 +145,7 @@ struct SweepState {
     // sweep loop of level pre_level, which uses them if its first sweep is a
     // fast sweep over every vertex with the same gain mode.
     int first_sweep_gain = -1;
+    BatchView first_sweep_bt;   // batches: the new level's graph ids, 2m, m, live flags
     bool pre_valid = false;
     int32_t pre_level = -1;
     int pre_gain = -1;

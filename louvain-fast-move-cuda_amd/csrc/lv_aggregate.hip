@@ -47,6 +47,7 @@ struct AggCtx {
     const uint32_t *knew;              // [n_new] new degrees
     uint32_t *dtarget;                 // [n_new]
     unsigned long long *dstats;        // one statistics section
+    BatchView dbt;                     // batches: graph, 2m, m and live flag per new vertex
 };
 
 // The next level's first sweep, decided from a contracted row: that sweep is
@@ -55,18 +56,20 @@ struct AggCtx {
 // candidates are the distinct destinations d != c of its row, each with the
 // row's summed weight w_cd and the total of d's singleton community, d's new
 // degree k_d; c's own total is k_c.  DEC = 1 + the gain mode.  The same
-// scores, order and move rule as the fast-sweep kernels (k_move_fast).
+// scores, order and move rule as the fast-sweep kernels (k_move_fast); in a
+// batch each vertex takes its graph's 2m and m, and a vertex of a graph that
+// does not sweep the next level is not examined (vertex_graph).
 template <int DEC>
 struct FirstSweep {
     using SC = Score<DEC - 1>;
     using S = typename SC::T;
-    __device__ static S score(const AggCtx &a, uint32_t w, uint32_t kc, uint32_t kd) {
-        return SC::of(w, 0u, kc, kc, kd, a.m2, a.tw_total);
+    __device__ static S score(uint32_t w, uint32_t kc, uint32_t kd, int64_t m2, double tw) {
+        return SC::of(w, 0u, kc, kc, kd, m2, tw);
     }
-    // the decision for c from its best candidate (lane-uniform call)
-    __device__ static uint32_t decide(const AggCtx &a, uint32_t c, uint32_t kc, S best,
-                                      uint32_t best_c) {
-        const bool mv = best != SC::none() && SC::positive(best, 0u, kc, kc, a.m2) && c < best_c;
+    // the decision for c from its best candidate
+    __device__ static uint32_t decide(uint32_t c, uint32_t kc, S best, uint32_t best_c,
+                                      int64_t m2) {
+        const bool mv = best != SC::none() && SC::positive(best, 0u, kc, kc, m2) && c < best_c;
         return mv ? best_c : c;
     }
 };
@@ -233,6 +236,9 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
         uint32_t best_c = LV_EMPTY;
         uint32_t nself = 0;
         const uint32_t kc = valid ? a.knew[c] : 0u;
+        int64_t m2 = a.m2;
+        double tw = a.tw_total;
+        const bool live = valid && vertex_graph(a.dbt, c, a.m2, a.tw_total, m2, tw);
         if (valid) {
             if (lane == 0) a.rcount[c] = s_cnt[gi];
             // kAggIlp slots per lane at a time: without the packed degrees
@@ -249,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
                 if constexpr (!PK) {
 #pragma unroll
                     for (int u = 0; u < kAggIlp; ++u)
-                        kds[u] = ds[u] != LV_EMPTY && ds[u] != c ? a.knew[ds[u]] : 0u;
+                        kds[u] = live && ds[u] != LV_EMPTY && ds[u] != c ? a.knew[ds[u]] : 0u;
                 }
 #pragma unroll
                 for (int u = 0; u < kAggIlp; ++u) {
@@ -258,9 +264,9 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
                     const uint32_t pos = atomicAdd(&s_pos[gi], 1u);
                     a.tcol[k0 + pos] = d;
                     a.tw[k0 + pos] = ws[u];
-                    if (d == c) continue;   // the self-loop is no candidate
+                    if (d == c || !live) continue;   // the self-loop is no candidate
                     ++nself;
-                    const S sc = FS::score(a, ws[u], kc, kds[u]);
+                    const S sc = FS::score(ws[u], kc, kds[u], m2, tw);
                     if (better_candidate(sc, d, best, best_c, false)) { best = sc; best_c = d; }
                 }
             }
@@ -271,7 +277,7 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
             const bool any = group_any<G>(nself > 0);
             if (valid && lane == 0) {
                 considered = any;
-                const uint32_t t = FS::decide(a, c, kc, best, best_c);
+                const uint32_t t = live ? FS::decide(c, kc, best, best_c, m2) : c;
                 a.dtarget[c] = t;
                 twrite = t != c;
             }
@@ -294,7 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
                     }
                 }
                 considered = tn > 0;
-                const uint32_t t = FS::decide(a, c, kc, best, best_c);
+                const uint32_t t = live ? FS::decide(c, kc, best, best_c, m2) : c;
                 a.dtarget[c] = t;
                 twrite = t != c;
             }
@@ -486,6 +492,9 @@ __global__ __launch_bounds__(kBlock) void k_giant_reduce(GiantCtx h) {
     uint32_t best_c = LV_EMPTY;
     uint32_t nself = 0;
     const uint32_t kc = DEC ? h.a.knew[I.c] : 0u;
+    int64_t m2 = h.a.m2;
+    double tw = h.a.tw_total;
+    const bool live = DEC && vertex_graph(h.a.dbt, I.c, h.a.m2, h.a.tw_total, m2, tw);
     __syncthreads();
     while (np > 0 && lo < span_hi) {
         const uint64_t rhi = span_hi - lo < width ? span_hi : lo + width;
@@ -521,12 +530,12 @@ __global__ __launch_bounds__(kBlock) void k_giant_reduce(GiantCtx h) {
                 }
 #pragma unroll
                 for (int u = 0; u < kGiantIlp; ++u)
-                    kds[u] = ds[u] != LV_EMPTY && ds[u] != I.c ? h.a.knew[ds[u]] : 0u;
+                    kds[u] = live && ds[u] != LV_EMPTY && ds[u] != I.c ? h.a.knew[ds[u]] : 0u;
 #pragma unroll
                 for (int u = 0; u < kGiantIlp; ++u) {
-                    if (ds[u] == LV_EMPTY || ds[u] == I.c) continue;
+                    if (!live || ds[u] == LV_EMPTY || ds[u] == I.c) continue;
                     ++nself;
-                    const S sc = FS::score(h.a, ws[u], kc, kds[u]);
+                    const S sc = FS::score(ws[u], kc, kds[u], m2, tw);
                     if (better_candidate(sc, ds[u], best, best_c, false)) { best = sc; best_c = ds[u]; }
                 }
             }
@@ -604,7 +613,10 @@ __global__ __launch_bounds__(kBlock) void k_giant_decide(GiantCtx h) {
             }
         }
         const uint32_t c = I.c;
-        const uint32_t t = FS::decide(h.a, c, h.a.knew[c], best, best_c);
+        int64_t m2 = h.a.m2;
+        double tw = h.a.tw_total;
+        const bool live = vertex_graph(h.a.dbt, c, h.a.m2, h.a.tw_total, m2, tw);
+        const uint32_t t = live ? FS::decide(c, h.a.knew[c], best, best_c, m2) : c;
         h.a.dtarget[c] = t;
         considered = tn > 0;
         twrite = t != c;
@@ -787,7 +799,7 @@ static AggCtx ctx_of(const AggArgs &x) {
     a.newc = x.newc; a.members = x.members; a.mstart = x.mstart; a.P = x.P; a.crow = x.crow;
     a.tcol = x.tcol; a.tw = x.tw; a.rcount = x.rcount;
     a.m2 = x.m2; a.tw_total = x.tw_total; a.newck = x.newck; a.knew = x.knew;
-    a.dtarget = x.dtarget; a.dstats = x.dstats;
+    a.dtarget = x.dtarget; a.dstats = x.dstats; a.dbt = x.dbt;
     return a;
 }
 

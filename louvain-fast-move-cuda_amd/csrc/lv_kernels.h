@@ -296,6 +296,7 @@ struct AggArgs {
     const uint32_t *knew = nullptr;                 // [n_new] new degrees
     uint32_t *dtarget = nullptr;                    // [n_new]
     unsigned long long *dstats = nullptr;           // one statistics section, zeroed
+    BatchView dbt;   // batches: the new vertices' graphs (their 2m, m and live flags)
 };
 // newc[v] = map[comm[v]]-1; key/val prepared for the vertex sort (key = newc,
 // val = v); with newck, also newc[v] | tot[comm[v]] << 32 (tot[comm[v]] is the
