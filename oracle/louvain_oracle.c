@@ -86,13 +86,14 @@ static int radix_sort_pairs(uint64_t *key, uint32_t *val, uint64_t n) {
         uint64_t *tk = ks; ks = kd; kd = tk;
         uint32_t *tv = vs; vs = vd; vd = tv;
     }
+    /* an odd number of passes leaves the result in the scratch arrays */
     if (ks != key) {
         memcpy(key, ks, n * sizeof(uint64_t));
         if (val) memcpy(val, vs, n * sizeof(uint32_t));
     }
     free(cnt);
-    free(ks == key ? k2 : kd);
-    if (val) free(vs == val ? v2 : vd);
+    free(k2);
+    free(v2);
     return LVO_OK;
 }
 

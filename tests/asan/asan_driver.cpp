@@ -162,6 +162,22 @@ int main(int argc, char **argv) {
             lvo_graph_free(og);
         }
     }
+    // keys with three populated 16-bit digits (ids past 65,535, every pair's
+    // smaller end below it): the oracle's radix sort makes an odd number of
+    // passes and copies its result back from the scratch arrays
+    {
+        const uint32_t n = 70001;
+        std::vector<int64_t> a(n - 1), b(n - 1);
+        a[0] = 0; b[0] = 1;
+        for (uint32_t y = 2; y < n; ++y) { a[y - 1] = 1; b[y - 1] = y; }
+        lvo_graph *og = nullptr;
+        CHECK(lvo_graph_from_edges(n - 1, a.data(), b.data(), nullptr, &og) == LVO_OK);
+        if (og) {
+            CHECK(og->n == n && og->E == 2ull * (n - 1));
+            CHECK(og->row[1] == 1 && og->row[2] == n);   // vertex 1 meets every other vertex
+            lvo_graph_free(og);
+        }
+    }
     // CSR errors
     const uint64_t row_bad[3] = {0, 2, 1};
     const uint32_t col_bad[2] = {1, 5};

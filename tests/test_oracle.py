@@ -299,3 +299,41 @@ def test_oracle_refine_matches_pipeline(fixture_path, seed):
                                    sweep_threshold=1e-6, level_threshold=1e-6))
         assert rb["modularity"] >= plain["modularity"]
         assert rb["sweeps"][:len(plain["sweeps"])] == plain["sweeps"]
+
+
+@pytest.mark.parametrize("kind", ["star", "random"])
+def test_from_edges_with_three_radix_digits(kind):
+    """Ids past 65,535 with every pair's smaller end below it: the oracle's CSR
+    build sorts keys (min << 32 | max) whose set bits fill three 16-bit digits,
+    an odd number of radix passes, so the sorted keys come back from the
+    scratch arrays (a Barabasi-Albert graph of 66,123 vertices took this path).
+    The CSR equals numpy's: both directions, sorted by (source, destination),
+    duplicates kept once."""
+    n = 70001
+    if kind == "star":
+        a = np.concatenate([[0], np.ones(n - 2, np.int64)])
+        b = np.concatenate([[1], np.arange(2, n, dtype=np.int64)])
+    else:
+        rng = np.random.default_rng(7)
+        m = 200_000
+        a = rng.integers(0, 60_000, m)
+        b = rng.integers(0, n, m)
+        keep = a != b
+        a, b = a[keep], b[keep]
+        # every id in use (the graph must be dense): a path over the low ids,
+        # each higher id tied to a low one
+        hi_ids = np.arange(60_000, n)
+        a = np.concatenate([a, np.arange(0, 59_999), rng.integers(0, 60_000, len(hi_ids))])
+        b = np.concatenate([b, np.arange(1, 60_000), hi_ids])
+        a, b = np.where(rng.random(len(a)) < 0.5, (a, b), (b, a))
+    assert np.minimum(a, b).max() < 65536 <= np.maximum(a, b).max()
+    row, col, w, k = O.Graph.from_edges(a, b).arrays()
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    pairs = np.unique(lo.astype(np.int64) << 32 | hi)
+    lo, hi = pairs >> 32, pairs & 0xFFFFFFFF
+    src = np.concatenate([lo, hi])
+    dst = np.concatenate([hi, lo])
+    order = np.lexsort((dst, src))
+    assert np.array_equal(col, dst[order])
+    assert np.array_equal(row, np.concatenate([[0], np.cumsum(np.bincount(src, minlength=n))]))
+    assert np.array_equal(k, np.bincount(src, minlength=n))
