@@ -985,21 +985,31 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
         fill_u64(st.pre_stats.data(), 0ull, kSection, s);
     }
     // 4. giant communities: their entries in destination hash partitions,
-    //    each partition summed per destination in an LDS table (planned on the
-    //    host first, then run on a side stream beside step 3's kernels: the
-    //    two touch disjoint communities and disjoint buffers)
+    //    each partition summed per destination in an LDS table, on a side
+    //    stream beside step 3's kernels (the two touch disjoint communities
+    //    and disjoint buffers).  The giants' ids and sizes come to the host
+    //    and their plan is made while step 3's tables already run.
     const uint32_t nh = cnt[kHubBucket];
     GiantArgs ga{};
     uint32_t n_parts = 0;
-    if (nh) {
+    const SideStreams *side = nh ? c.state->side_streams() : nullptr;
+    uint64_t *gsize_dev = nullptr;
+    if (side) {
+        LV_HIP(hipEventRecord(side->fork, s));
+        LV_HIP(hipStreamWaitEvent(side->s[0], side->fork, 0));
+        gsize_dev = ws.get<uint64_t>(W_HSIZE, nh);
+        gather_degrees(crow, lp[kHubBucket], nh, gsize_dev, side->s[0]);
+    }
+    agg_tables(x, lp, cnt, s);
+    if (side) {
+        hipStream_t gs = side->s[0];
         std::vector<uint32_t> giants(nh);
-        LV_HIP(hipMemcpyAsync(giants.data(), lp[kHubBucket], nh * sizeof(uint32_t),
-                              hipMemcpyDeviceToHost, s));
         std::vector<uint64_t> gsize(nh);
-        uint64_t *gsize_dev = ws.get<uint64_t>(W_HSIZE, nh);
-        gather_degrees(crow, lp[kHubBucket], nh, gsize_dev, s);
-        LV_HIP(hipMemcpyAsync(gsize.data(), gsize_dev, nh * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-        wait_stream(s);
+        LV_HIP(hipMemcpyAsync(giants.data(), lp[kHubBucket], nh * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, gs));
+        LV_HIP(hipMemcpyAsync(gsize.data(), gsize_dev, nh * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                              gs));
+        wait_stream(gs);
         // kept in the state: read by the async uploads below
         std::vector<GiantInfo> &info = c.state->giant_info;
         std::vector<uint64_t> &tiles = c.state->giant_tiles, &items = c.state->giant_items;
@@ -1028,11 +1038,11 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
         GiantInfo *info_dev = ws.get<GiantInfo>(W_GINFO, nh);
         uint64_t *tiles_dev = ws.get<uint64_t>(W_GTILES, tiles.size());
         uint64_t *items_dev = ws.get<uint64_t>(W_GITEMS, items.size());
-        LV_HIP(hipMemcpyAsync(info_dev, info.data(), nh * sizeof(GiantInfo), hipMemcpyHostToDevice, s));
+        LV_HIP(hipMemcpyAsync(info_dev, info.data(), nh * sizeof(GiantInfo), hipMemcpyHostToDevice, gs));
         LV_HIP(hipMemcpyAsync(tiles_dev, tiles.data(), tiles.size() * sizeof(uint64_t),
-                              hipMemcpyHostToDevice, s));
+                              hipMemcpyHostToDevice, gs));
         LV_HIP(hipMemcpyAsync(items_dev, items.data(), items.size() * sizeof(uint64_t),
-                              hipMemcpyHostToDevice, s));
+                              hipMemcpyHostToDevice, gs));
         ga.info = info_dev;
         ga.tiles = tiles_dev;
         ga.n_tiles = (uint32_t)tiles.size();
@@ -1055,17 +1065,9 @@ void AggregationPhase::contract(Community &c, const uint32_t *map, uint32_t n_ne
             ga.dpart_n = ws.get<uint32_t>(W_DPN, pbase);
         }
         n_parts = pbase;
-    }
-    const SideStreams *side = nh ? c.state->side_streams() : nullptr;
-    if (side) {
-        LV_HIP(hipEventRecord(side->fork, s));
-        LV_HIP(hipStreamWaitEvent(side->s[0], side->fork, 0));
-        fill_u32(ga.pcnt, 0u, n_parts, side->s[0]);
-        agg_giant(x, ga, n_parts, side->s[0]);
-    }
-    agg_tables(x, lp, cnt, s);
-    if (side) {
-        LV_HIP(hipEventRecord(side->join[0], side->s[0]));
+        fill_u32(ga.pcnt, 0u, n_parts, gs);
+        agg_giant(x, ga, n_parts, gs);
+        LV_HIP(hipEventRecord(side->join[0], gs));
         LV_HIP(hipStreamWaitEvent(s, side->join[0], 0));
     }
     // 5. new offsets, then every row copied to place
