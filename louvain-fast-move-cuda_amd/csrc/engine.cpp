@@ -36,6 +36,7 @@ KernelStatsAcc g_last_run_stats;
 
 void SweepState::setup_level(const GraphDevice &g, hipStream_t s) {
     n = g.n_nodes;
+    lists_static = false;
     bucket.ensure(std::max<size_t>(n, 1));
     mark.ensure(std::max<size_t>(n, 1));
     target.ensure(std::max<size_t>(n, 1));
@@ -117,6 +118,7 @@ void SweepState::build_static_lists(hipStream_t s) {
     build_lists_all(bucket.data(), n, lp, counts_dev.data(), s);
     cur = 0;
     for (int b = 0; b < kNumBuckets; ++b) count[b] = bucket_size[b];
+    lists_static = true;
 }
 
 uint32_t *const *SweepState::lists(int which) {
@@ -587,6 +589,7 @@ void launch_sweep(Community &c, const lv_params &p, int kind, bool apply, uint32
     hipStream_t s = c.stream();
     if (p.autotune && !pre_decided) autotune_tiles(c, p, kind, v_end);
     SweepArgs a = sweep_args(c, p, kind, v_end);
+    if (apply) st.lists_static = false;   // the finalize pass writes the next lists
     sweep_reset(st.stats.data(), st.stats.size(), a.n_movers, st.next_counts(), s);
     a.counters_zeroed = true;
     uint32_t *const *cur = st.lists(st.cur);
@@ -621,8 +624,9 @@ void launch_sweep(Community &c, const lv_params &p, int kind, bool apply, uint32
 
 int OptimizationPhase::run(Community &c, const lv_params &p, lv_sweep_rec *rec, int rec_cap,
                            int hash_trace, uint64_t *traversed) {
-    // phaseOptimization.cuh:52: every vertex starts active
-    c.state->build_static_lists(c.stream());
+    // phaseOptimization.cuh:52: every vertex starts active (the level's setup
+    // may have built these lists already)
+    if (!c.state->lists_static) c.state->build_static_lists(c.stream());
     return sweep_loop(c, p, 0, c.level, rec, rec_cap, hash_trace, traversed);
 }
 

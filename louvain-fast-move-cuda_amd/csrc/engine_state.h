@@ -107,6 +107,10 @@ struct SweepState {
     uint32_t count[kNumBuckets] = {};           // sizes of the current lists
     uint32_t *list_ptr[2][kNumBuckets] = {};
     int cur = 0;
+    // the current lists hold every vertex of the level by bucket (built by
+    // build_static_lists since the level's buckets and no sweep has applied
+    // moves since): a level's first sweep then needs no rebuild
+    bool lists_static = false;
     // hubs
     uint32_t n_hubs = 0;
     DeviceArray<uint32_t> hub_slot_of;

@@ -428,6 +428,7 @@ lv_status lv_best_moves(lv_community *c, int kind, const lv_params *p, const uin
                                   hipMemcpyDeviceToHost, s));
             LV_HIP(hipStreamSynchronize(s));
             st.cur ^= 1;
+            st.lists_static = false;
             std::memcpy(st.count, cnt, sizeof(cnt));
         }
         const uint32_t v_end = (P.chunk_emulation && kind != LV_SWEEP_HASH)
