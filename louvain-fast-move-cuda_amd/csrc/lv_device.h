@@ -422,6 +422,16 @@ __device__ inline void sum_next_piece(SumTable &t) {
 // Width of the first piece of [span_lo, span_hi) for np pairs: as if the
 // pairs were distinct, but at most 4 pieces, since many pairs may share a key
 // (a piece that overflows is halved, one that stays sparse lets the next grow).
+// Workgroup barrier that orders LDS only: global loads in flight (the next
+// vertex's prefetched columns) stay in flight across it, where
+// __syncthreads() would wait for them.  For loops whose workgroup shares
+// nothing but LDS.
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ inline uint64_t sum_first_width(uint64_t span_lo, uint64_t span_hi, uint32_t np) {
     uint64_t ranges = 1;
     while (ranges < 4 && ranges < span_hi - span_lo && ranges * kSumTarget < np) ranges <<= 1;

@@ -489,25 +489,38 @@ struct MoveWords {
     bool live = false;
 };
 
-template <bool PK>
+// A load the compiler may issue through the scalar unit when its address is
+// uniform: arrays a kernel only reads (row bounds, degrees, comm|tot words)
+// are read as constant memory, so a vertex's words land in scalar registers
+// and stay there while the vector registers hold its row.
+template <typename T>
+__device__ inline T const_load(const T *p) {
+    return *reinterpret_cast<const __attribute__((address_space(4))) T *>(
+        reinterpret_cast<uintptr_t>(p));
+}
+
+template <bool PK, bool SCALAR = false>
 __device__ inline MoveWords move_words(const MoveCtx &a, uint32_t v, bool valid) {
+    auto ld = [](const auto *p) { return SCALAR ? const_load(p) : *p; };
     MoveWords w;
     w.v = v;
     w.m2 = a.m2;
     w.tw = a.tw;
     w.live = valid && v < a.v_end && vertex_graph(a.bt, v, a.m2, a.tw, w.m2, w.tw);
-    if (w.live) {
+    // (scalar: loaded whether live or not, from a vertex that exists, so
+    // that no wait is needed to merge the two cases)
+    if (SCALAR || w.live) {
         if constexpr (PK) {
-            const unsigned long long p = a.ctot[v];
+            const unsigned long long p = ld(&a.ctot[v]);
             w.own = (uint32_t)p;
             w.tot_own = (uint32_t)(p >> 32);
         } else {
-            w.own = a.comm[v];
+            w.own = ld(&a.comm[v]);
         }
-        w.kv = a.kdeg[v];
-        w.beg = a.row[v];
-        w.end = a.row[v + 1];
-        if constexpr (!PK) w.tot_own = a.tot[w.own];
+        w.kv = ld(&a.kdeg[v]);
+        w.beg = ld(&a.row[v]);
+        w.end = ld(&a.row[v + 1]);
+        if constexpr (!PK) w.tot_own = ld(&a.tot[w.own]);
     }
     return w;
 }
@@ -549,7 +562,15 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? 8 : 1))) voi
     constexpr uint64_t kEmptySlot = (uint64_t)LV_EMPTY;       // key LV_EMPTY, weight 0
     uint32_t *TT = s_tot + (PK ? gi * T : 0);
     uint32_t idx = PERSIST ? blockIdx.x : blockIdx.x * GPB + gi;
-    MoveWords nw = move_words<PK>(a, idx < count ? list[idx] : 0u, idx < count);
+    MoveWords nw = move_words<PK, PERSIST>(a, idx < count ? list[idx] : 0u, idx < count);
+    // persistent: the list entry of the vertex after the next one, loaded a
+    // vertex ahead, so that the next vertex's words are requested when a
+    // vertex starts and arrive under its inserts
+    uint32_t nv = 0u;
+    if constexpr (PERSIST) {
+        const uint32_t n1 = idx + gridDim.x;
+        nv = n1 > idx && n1 < count ? list[n1] : 0u;
+    }
     uint32_t acc_self = 0, acc_cand = 0, acc_cons = 0, acc_tw = 0;
     // persistent: thread 0 counts its decisions in LDS (two registers fewer
     // in every lane)
@@ -588,14 +609,19 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? 8 : 1))) voi
         const bool live = cw.live;
         const uint32_t nidx = idx + gridDim.x;   // PERSIST only
         const bool nvalid = PERSIST && nidx > idx && nidx < count;
-        const uint32_t nv = nvalid ? list[nidx] : 0u;
+        if constexpr (PERSIST) {
+            nw = move_words<PK, true>(a, nv, nvalid);
+            const uint32_t n2 = nidx + gridDim.x;
+            nv = nvalid && n2 > nidx && n2 < count ? list[n2] : 0u;
+        }
         // the vertex's table: table_slots(degree) <= T slots
         const uint32_t tv = live ? table_slots(end - beg) : 4u;
         if constexpr (!FUSED) {
             for (uint32_t s = lane; s < tv; s += G) KV[s] = kEmptySlot;
             // a vertex's table is its group's alone: groups inside one wave
             // wait for no other wave of the workgroup
-            group_barrier<G>();
+            if constexpr (PERSIST) lds_barrier();
+            else group_barrier<G>();
         }
         uint32_t nself = 0;
         // I entries per lane in flight: loads first, then gathers, then
@@ -676,13 +702,20 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? 8 : 1))) voi
                 // short rows of a persistent kernel's bucket (group-uniform
                 // test): half the entries per lane, no masked half step
                 step(std::integral_constant<int, kIlp / 2>{}, beg + lane, true);
+            } else if constexpr (PERSIST) {
+                // the first step outside the loop: the loop's entry would wait
+                // for the next vertex's scalar words, which the first step's
+                // gathers wait for anyway
+                step(std::integral_constant<int, kIlp>{}, beg + lane, true);
+                for (uint64_t jb = beg + lane + (uint64_t)G * kIlp; jb < end; jb += (uint64_t)G * kIlp)
+                    step(std::integral_constant<int, kIlp>{}, jb, false);
             } else {
                 for (uint64_t jb = beg + lane; jb < end; jb += (uint64_t)G * kIlp)
                     step(std::integral_constant<int, kIlp>{}, jb, jb == beg + lane);
             }
         }
-        if constexpr (PERSIST) nw = move_words<PK>(a, nv, nvalid);
-        group_barrier<G>();
+        if constexpr (PERSIST) lds_barrier();
+        else group_barrier<G>();
         if constexpr (PREFETCH) first_loads(nw);
         S best = SC::none();
         uint32_t best_c = a.highest ? 0u : LV_EMPTY;
@@ -750,7 +783,8 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? 8 : 1))) voi
             const uint32_t wself = group_any<64>(nself > 0);
             const int w = threadIdx.x >> 6;
             if ((threadIdx.x & 63) == 0) { r_s[w] = best; r_c[w] = best_c; r_n[w] = wself; }
-            __syncthreads();
+            if constexpr (PERSIST) lds_barrier();
+            else __syncthreads();
             if (threadIdx.x == 0 && live) {
                 uint32_t tn = 0;
                 for (int i = 0; i < kBlock / 64; ++i) {
