@@ -589,7 +589,18 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? 8 : 1))) voi
     // (the weighted tables skip that prefetch: without its registers they run
     // at 8 waves per SIMD instead of 7, measured faster)
     constexpr bool PREFETCH = PERSIST && !WEIGHTED;
+    // ... and, packed, the (comm, total) words of those columns, gathered
+    // once this vertex's candidates are scored, before its reduction
+    constexpr bool PGATHER = PREFETCH && PK;
+    unsigned long long pg[PGATHER ? kIlp : 1];
+    auto first_gathers = [&]() {
+        if constexpr (PGATHER) {
+#pragma unroll
+            for (int u = 0; u < kIlp; ++u) pg[u] = a.ctot[px[u]];
+        }
+    };
     if constexpr (PREFETCH) first_loads(nw);
+    first_gathers();
     // FUSED (persistent, exact gains): the table is cleared once here, then
     // by the scoring loop as it reads each slot (every slot belongs to one
     // lane there), so a vertex starts on a clean table with no clearing loop
@@ -665,7 +676,8 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? 8 : 1))) voi
                 // share one address)
                 const bool g = G >= 64 || xs[u] != v;
                 if constexpr (PK) {
-                    const unsigned long long p = g ? a.ctot[xs[u]] : 0ull;
+                    const unsigned long long p =
+                        PGATHER && first ? pg[PGATHER ? u : 0] : g ? a.ctot[xs[u]] : 0ull;
                     cs[u] = (uint32_t)p;
                     ts[u] = (uint32_t)(p >> 32);
                 } else {
@@ -768,6 +780,7 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? 8 : 1))) voi
             }
         }
         acc_self += nself;
+        first_gathers();
         if constexpr (G <= 64) {
             group_best<G, S>(best, best_c, a.highest);
             const bool gself = group_any<G>(nself > 0);
