@@ -577,7 +577,7 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? 8 : 1))) voi
     __shared__ uint32_t s_cons, s_tw;
     if (PERSIST && threadIdx.x == 0) { s_cons = 0u; s_tw = 0u; }
     // persistent: the first kIlp columns per lane of the next vertex's row,
-    // loaded while this vertex is scored (its weights load beside the gathers)
+    // loaded while this vertex is scored (its weights load in its first step)
     uint32_t px[kIlp];
     auto first_loads = [&](const MoveWords &w) {
 #pragma unroll
@@ -586,9 +586,7 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? 8 : 1))) voi
             px[u] = w.live && j < w.end ? stream_load(&a.col[j]) : w.v;
         }
     };
-    // (the weighted tables skip that prefetch: without its registers they run
-    // at 8 waves per SIMD instead of 7, measured faster)
-    constexpr bool PREFETCH = PERSIST && !WEIGHTED;
+    constexpr bool PREFETCH = PERSIST;
     // ... and, packed, the (comm, total) words of those columns, gathered
     // once this vertex's candidates are scored, before its reduction
     constexpr bool PGATHER = PREFETCH && PK;
