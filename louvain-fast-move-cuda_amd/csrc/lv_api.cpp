@@ -553,7 +553,7 @@ lv_status lv_debug_mark_bytes(int32_t on) {
 
 lv_status lv_kernel_variant_name(int variant, int gain, char *buf, size_t len) {
     if (variant < 0 || variant >= lv::kMoveVariants || !buf || len == 0) return LV_ERR_ARG;
-    const int *T = lv::kTableT;
+    const int *T = lv::kMoveTableT;
     const int sel = variant % lv::kTileChoices, rest = variant / lv::kTileChoices;
     const int b = rest / 4, fast = (rest / 2) % 2, wtd = rest % 2;
     const int G = lv::kTileG[b][sel];

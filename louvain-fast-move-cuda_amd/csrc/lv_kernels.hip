@@ -623,8 +623,9 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? 8 : 1))) voi
             const uint32_t n2 = nidx + gridDim.x;
             nv = nvalid && n2 > nidx && n2 < count ? list[n2] : 0u;
         }
-        // the vertex's table: table_slots(degree) <= T slots
-        const uint32_t tv = live ? table_slots(end - beg) : 4u;
+        // the vertex's table: table_slots(degree) slots, at most T (which
+        // exceeds every degree of the bucket)
+        const uint32_t tv = live ? min(table_slots(end - beg), (uint32_t)T) : 4u;
         if constexpr (!FUSED) {
             for (uint32_t s = lane; s < tv; s += G) KV[s] = kEmptySlot;
             // a vertex's table is its group's alone: groups inside one wave
@@ -1940,7 +1941,7 @@ void launch_table_tile(int b, int G, const MoveCtx &m, bool wtd, int gain, const
             }
             break;
         case 6: LV_T(256, kTableT[6]); break;
-        default: LV_T(256, kTableT[7]); break;
+        default: LV_T(256, kMoveTableT[7]); break;
     }
 #undef LV_T
 }
