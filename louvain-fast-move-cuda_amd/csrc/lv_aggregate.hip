@@ -117,9 +117,10 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
     constexpr int GPB = kBlock / G;
     constexpr bool PK = DEC != 0 && T <= kPackedMaxT;
     constexpr int TK = PK ? T : 1;
-    // members staged at a time: G, or fewer where the degree array would
-    // otherwise cost a workgroup per CU (bucket 5 and 6 tables at 8 per CU)
-    constexpr int SM = !PK ? G : (G >= 256 ? 64 : (G >= 64 ? 16 : G));
+    // members staged at a time: G, or fewer where the staging would
+    // otherwise cost a workgroup per CU (bucket 5 and 6 tables at 8 per CU
+    // with their degree arrays, bucket 7's at 4)
+    constexpr int SM = T == kTable7 ? 64 : !PK ? G : (G >= 256 ? 64 : (G >= 64 ? 16 : G));
     __shared__ uint32_t s_key[GPB * T];
     __shared__ uint32_t s_val[GPB * T];
     __shared__ uint32_t s_kd[GPB * TK];   // PK: new degree of each destination
@@ -141,8 +142,9 @@ __global__ __launch_bounds__(kBlock) void k_agg_table(AggCtx a, const uint32_t *
     if (valid) {
         k0 = a.crow[c];
         // at most two thirds full with an empty slot (the virtual entries
-        // bound the distinct destinations); <= T for the bucket's entries
-        tv = table_slots(a.crow[c + 1] - k0);
+        // bound the distinct destinations), or T (bucket 7: above every
+        // entry count of the bucket)
+        tv = min(table_slots(a.crow[c + 1] - k0), (uint32_t)T);
     }
     for (uint32_t s = lane; s < tv; s += G) { K[s] = LV_EMPTY; V[s] = 0u; }
     if (lane == 0) { s_cnt[gi] = 0u; s_pos[gi] = 0u; }
@@ -824,8 +826,8 @@ void agg_tables(const AggArgs &x, uint32_t *const *lists, const uint32_t *counts
         else if (x.dgain == 1) LV_AGG_D(GG, TT, 2);                                             \
         else LV_AGG_D(GG, TT, 0);                                                               \
     } while (0)
-        // tables of table_slots(the bucket's largest community), the slots
-        // of the best-move tables of the same degree bound
+        // tables of table_slots(the bucket's largest community) (bucket 7:
+        // kTable7), the slots of the best-move tables of the same degree bound
         switch (b) {
             case 0: LV_AGG(2, kTableT[0]); break;
             case 1: LV_AGG(4, kTableT[1]); break;
@@ -834,7 +836,7 @@ void agg_tables(const AggArgs &x, uint32_t *const *lists, const uint32_t *counts
             case 4: LV_AGG(32, kTableT[4]); break;
             case 5: LV_AGG(64, kTableT[5]); break;
             case 6: LV_AGG(256, kTableT[6]); break;
-            default: LV_AGG(256, kTableT[7]); break;
+            default: LV_AGG(256, kTable7); break;
         }
 #undef LV_AGG
 #undef LV_AGG_D

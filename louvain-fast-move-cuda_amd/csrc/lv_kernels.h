@@ -68,15 +68,17 @@ constexpr int kTableT[kNumBuckets - 1] = {
     (int)table_slots(kBucketMaxDeg[6]), (int)table_slots(kBucketMaxDeg[7])};
 static_assert(kTableT[0] == 8 && kTableT[5] == 388 && kTableT[6] == 1540 && kTableT[7] == 6148,
               "table sizes");
-// The best-move tables of bucket 7 hold fewer slots than table_slots(4096):
-// four workgroups then fit a CU's LDS instead of three.  A row takes
-// min(table_slots(d), kMoveTableT7) slots, so the rows above 3,394 entries
-// fill their table beyond two thirds (up to 80%; a slot stays empty).
-constexpr int kMoveTableT7 = 5092;
-static_assert(kMoveTableT7 > (int)kBucketMaxDeg[7] && kMoveTableT7 * 8 + 160 <= 160 * 1024 / 4,
-              "bucket-7 best-move table");
+// The LDS tables of bucket 7 (best moves and contraction, 8 bytes a slot)
+// hold fewer slots than table_slots(4096): four workgroups then fit a CU's
+// LDS instead of three.  A row or community takes min(table_slots(d),
+// kTable7) slots, so those above 3,306 entries fill their table beyond two
+// thirds (up to 83%; a slot stays empty).  The contraction's table stages 64
+// members at a time beside it.
+constexpr int kTable7 = 4960;
+static_assert(kTable7 > (int)kBucketMaxDeg[7] && kTable7 * 8 + 64 * 16 + 256 <= 160 * 1024 / 4,
+              "bucket-7 tables");
 constexpr int kMoveTableT[kNumBuckets - 1] = {kTableT[0], kTableT[1], kTableT[2], kTableT[3],
-                                              kTableT[4], kTableT[5], kTableT[6], kMoveTableT7};
+                                              kTableT[4], kTableT[5], kTableT[6], kTable7};
 
 // Tile (lanes per vertex) candidates of each LDS bucket's best-move kernels;
 // the first is the default, 0 ends a list.  The LDS table of a bucket stays

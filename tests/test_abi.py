@@ -199,7 +199,7 @@ def test_params_layout_and_tile_variant_names():
     assert L.kernel_variant_name(((5 * 2 + 0) * 2 + 1) * 3 + 1) == "k_move_table<32, 388, 0, true, true>"
     assert L.kernel_variant_name(((3 * 2 + 1) * 2 + 0) * 3 + 2) == "k_move_fast<32, 0, false>"
     assert L.kernel_variant_name(((6 * 2 + 0) * 2 + 0) * 3 + 0) == "k_move_table<256, 1540, 0, false, true>"
-    assert L.kernel_variant_name(((7 * 2 + 0) * 2 + 0) * 3 + 0) == "k_move_table<256, 5092, 0, false, false>"
+    assert L.kernel_variant_name(((7 * 2 + 0) * 2 + 0) * 3 + 0) == "k_move_table<256, 4960, 0, false, false>"
     with pytest.raises(L.LouvainError):
         L.kernel_variant_name(((6 * 2 + 0) * 2 + 0) * 3 + 1)      # bucket 6 has one tile
 

@@ -90,7 +90,7 @@ GRAPHS = {
     "hubs": dict(n=20000, m=40000, hubs=3, hub_deg=9000),
     "mid": dict(n=5000, m=60000),
     # rows of ~4,050 distinct neighbours: the degree 1025-4096 bucket's
-    # best-move tables (5,092 slots) filled to ~80%, their fullest
+    # best-move tables (4,960 slots) filled to ~82%, their fullest
     "bucket7_full": dict(n=30000, m=45000, hubs=30, hub_deg=4050),
 }
 
