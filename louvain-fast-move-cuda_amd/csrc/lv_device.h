@@ -352,6 +352,16 @@ struct SumTable {
     uint32_t n_used;
 };
 
+// Workgroup barrier that orders LDS only: global loads and stores in flight
+// (prefetches, output writes) stay in flight across it, where
+// __syncthreads() would wait for them.  For loops whose workgroup shares
+// nothing but LDS (the persistent best-move tables, the LDS sum tables).
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Workgroup-wide: empty table (call once; a caller leaves it empty again by
 // clearing the slots it read after each sum_piece, and an overflowing
 // sum_piece clears it itself).
@@ -401,36 +411,27 @@ __device__ inline uint32_t sum_piece(SumTable &t, const unsigned long long *__re
             }
         }
     }
-    __syncthreads();
+    lds_barrier();
     const uint32_t nu = t.n_used;
     if (nu < kSumFill) return nu;
     for (int i = threadIdx.x; i < kSumLds; i += blockDim.x) { t.keys[i] = LV_EMPTY; t.vals[i] = 0u; }
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x == 0) t.n_used = 0;
-    __syncthreads();
+    lds_barrier();
     return 0xFFFFFFFFu;
 }
 
 // Workgroup-wide, after the caller has read the piece's nu used slots (and
 // cleared them): resets the claim count for the next piece.
 __device__ inline void sum_next_piece(SumTable &t) {
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x == 0) t.n_used = 0;
-    __syncthreads();
+    lds_barrier();
 }
 
 // Width of the first piece of [span_lo, span_hi) for np pairs: as if the
 // pairs were distinct, but at most 4 pieces, since many pairs may share a key
 // (a piece that overflows is halved, one that stays sparse lets the next grow).
-// Workgroup barrier that orders LDS only: global loads in flight (the next
-// vertex's prefetched columns) stay in flight across it, where
-// __syncthreads() would wait for them.  For loops whose workgroup shares
-// nothing but LDS.
-__device__ inline void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
 
 __device__ inline uint64_t sum_first_width(uint64_t span_lo, uint64_t span_hi, uint32_t np) {
     uint64_t ranges = 1;

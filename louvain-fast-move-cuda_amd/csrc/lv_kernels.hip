@@ -1150,7 +1150,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_reduce(HubCtx a) {
         int64_t m2;
         double tw;
         (void)vertex_graph(a.m.bt, v, a.m.m2, a.m.tw, m2, tw);   // items are sweeping hubs only
-        __syncthreads();   // the previous item's shared scalars are consumed
+        lds_barrier();   // the previous item's shared scalars are consumed
         if (threadIdx.x < 64) {
             const uint32_t lane = threadIdx.x;
             const uint32_t c = lane < (uint32_t)kHubParts ? a.h.cnt[(size_t)slot * kHubParts + lane] : 0u;
@@ -1158,7 +1158,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_reduce(HubCtx a) {
             const uint64_t mine = group_sum_u64<64>(lane >= g * per && lane < (g + 1) * per ? c : 0u);
             if (lane == 0) { s_beg = (uint32_t)before; s_cnt = (uint32_t)mine; }
         }
-        __syncthreads();
+        lds_barrier();
         const uint32_t own = a.m.comm[v];
         const uint32_t kv = a.m.kdeg[v];
         const uint32_t tot_own = a.m.tot[own];
@@ -1205,7 +1205,7 @@ __global__ __launch_bounds__(kBlock) void k_hub_reduce(HubCtx a) {
         group_best<64, S>(best, best_c, a.m.highest);
         const int w = threadIdx.x >> 6;
         if ((threadIdx.x & 63) == 0) { r_s[w] = best; r_c[w] = best_c; }
-        __syncthreads();
+        lds_barrier();
         if (threadIdx.x == 0) {
             for (int i = 0; i < kBlock / 64; ++i)
                 if (better_candidate(r_s[i], r_c[i], best, best_c, a.m.highest)) {
