@@ -589,12 +589,13 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? 8 : 1))) voi
     constexpr bool PREFETCH = PERSIST;
     // ... and, packed, the (comm, total) words of those columns, gathered
     // once this vertex's candidates are scored, before its reduction
-    constexpr bool PGATHER = PREFETCH && PK;
+    // (unpacked: their communities)
+    constexpr bool PGATHER = PREFETCH;
     unsigned long long pg[PGATHER ? kIlp : 1];
     auto first_gathers = [&]() {
         if constexpr (PGATHER) {
 #pragma unroll
-            for (int u = 0; u < kIlp; ++u) pg[u] = a.ctot[px[u]];
+            for (int u = 0; u < kIlp; ++u) pg[u] = PK ? a.ctot[px[u]] : a.comm[px[u]];
         }
     };
     if constexpr (PREFETCH) first_loads(nw);
@@ -680,7 +681,7 @@ __attribute__((amdgpu_waves_per_eu(G == kBlock && PK && GAIN == 0 ? 8 : 1))) voi
                     cs[u] = (uint32_t)p;
                     ts[u] = (uint32_t)(p >> 32);
                 } else {
-                    cs[u] = g ? a.comm[xs[u]] : 0u;
+                    cs[u] = PGATHER && first ? (uint32_t)pg[PGATHER ? u : 0] : g ? a.comm[xs[u]] : 0u;
                     ts[u] = 0u;
                 }
             }
